@@ -1,0 +1,311 @@
+"""ctypes binding of the HIP library ``lib/libhase_ase.so`` (include/hase_ase.h).
+
+The library is loaded from the package's ``lib/`` folder; there is no other
+implementation of the estimator behind this module, so a missing or broken
+library raises instead of falling back.  ctypes loading follows the
+reference's own C-ABI loader (pyInclude/alpakaUtils/AlpakaBackends.py:9-53).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+LIB_PATH = LIB_DIR / "libhase_ase.so"
+
+HASE_OK = 0
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
+
+# every symbol include/hase_ase.h declares
+EXPORTED_SYMBOLS = (
+    "hase_calc_phi_ase",
+    "hase_context_create",
+    "hase_context_update_beta",
+    "hase_context_evaluate",
+    "hase_context_stats",
+    "hase_context_destroy",
+    "hase_context_trace_rays",
+    "hase_context_importance",
+    "hase_last_error",
+    "hase_abi_version",
+    "hase_build_info",
+)
+
+
+class HaseError(RuntimeError):
+    """Error returned by the HIP library (negative status code)."""
+
+    def __init__(self, code, message):
+        super().__init__(f"hase_ase error {code}: {message}")
+        self.code = code
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("number_of_points", C.c_uint32),
+        ("number_of_triangles", C.c_uint32),
+        ("number_of_levels", C.c_uint32),
+        ("thickness", C.c_double),
+        ("points", _f64p),
+        ("triangle_point_indices", _u32p),
+        ("triangle_neighbors", _i32p),
+        ("forbidden_edge", _i32p),
+        ("beta_volume", _f64p),
+        ("beta_cells", _f64p),
+        ("cladding_cell_types", _u32p),
+        ("cladding_number", C.c_uint32),
+        ("cladding_absorption", C.c_double),
+        ("n_tot", C.c_double),
+        ("crystal_t_fluo", C.c_double),
+        ("reflectivities", _f32p),
+        ("refractive_indices", _f32p),
+    ]
+
+
+class Experiment(C.Structure):
+    _fields_ = [
+        ("min_rays_per_sample", C.c_uint32),
+        ("max_rays_per_sample", C.c_uint32),
+        ("number_of_wavelengths", C.c_uint32),
+        ("sigma_a", _f64p),
+        ("sigma_e", _f64p),
+        ("max_sigma_a", C.c_double),
+        ("max_sigma_e", C.c_double),
+        ("mse_threshold", C.c_double),
+        ("use_reflections", C.c_int32),
+        ("max_reflections", C.c_uint32),
+        ("defensive_fraction", C.c_double),
+    ]
+
+
+class Compute(C.Structure):
+    _fields_ = [
+        ("max_repetitions", C.c_uint32),
+        ("adaptive_steps", C.c_uint32),
+        ("min_sample_range", C.c_uint32),
+        ("max_sample_range", C.c_uint32),
+        ("rng_seed", C.c_uint32),
+        ("device", C.c_int32),
+        ("rays_per_thread", C.c_uint32),
+        ("block_size", C.c_uint32),
+        ("sample_batch", C.c_uint32),
+    ]
+
+
+class ResultC(C.Structure):
+    _fields_ = [
+        ("phi_ase", _f32p),
+        ("mse", _f64p),
+        ("total_rays", _u32p),
+        ("dndt_ase", _f64p),
+        ("standard_error", _f64p),
+        ("dropped_rays", _u32p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("importance_ms", C.c_double),
+        ("trace_ms", C.c_double),
+        ("finalize_ms", C.c_double),
+        ("total_ms", C.c_double),
+        ("rays", C.c_uint64),
+        ("ray_steps", C.c_uint64),
+        ("importance_walks", C.c_uint64),
+        ("importance_steps", C.c_uint64),
+        ("dropped_rays", C.c_uint64),
+        ("trace_launches", C.c_uint32),
+        ("importance_launches", C.c_uint32),
+    ]
+
+    def asdict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950)")
+    L = C.CDLL(str(LIB_PATH))
+    ctx_p = C.c_void_p
+    L.hase_calc_phi_ase.argtypes = [C.POINTER(Mesh), C.POINTER(Experiment), C.POINTER(Compute), C.POINTER(ResultC)]
+    L.hase_context_create.argtypes = [C.POINTER(Mesh), C.POINTER(Experiment), C.POINTER(Compute),
+                                      C.POINTER(ctx_p)]
+    L.hase_context_update_beta.argtypes = [ctx_p, _f64p, _f64p]
+    L.hase_context_evaluate.argtypes = [ctx_p, C.POINTER(ResultC)]
+    L.hase_context_stats.argtypes = [ctx_p, C.POINTER(Stats)]
+    L.hase_context_destroy.argtypes = [ctx_p]
+    L.hase_context_destroy.restype = None
+    L.hase_context_trace_rays.argtypes = [ctx_p, C.c_uint32, _f32p, _u32p, _f32p, _u32p, _u32p,
+                                          _f32p, _f32p, _f32p, _f32p, _u32p, _i32p]
+    L.hase_context_importance.argtypes = [ctx_p, C.c_uint32, _f32p, C.c_uint32]
+    L.hase_last_error.restype = C.c_char_p
+    L.hase_abi_version.restype = C.c_int
+    L.hase_build_info.restype = C.c_char_p
+    for name in ("hase_calc_phi_ase", "hase_context_create", "hase_context_update_beta",
+                 "hase_context_evaluate", "hase_context_stats", "hase_context_trace_rays",
+                 "hase_context_importance"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != HASE_OK:
+        raise HaseError(code, load().hase_last_error().decode())
+
+
+def _ptr(arr, ctype):
+    return None if arr is None else arr.ctypes.data_as(C.POINTER(ctype))
+
+
+class NativeMesh:
+    """Owns the arrays a ``hase_mesh_t`` points into."""
+
+    def __init__(self, topology, medium):
+        d = topology.topology()
+        self.points = np.ascontiguousarray(np.concatenate([topology.points[:, 0], topology.points[:, 1]]),
+                                           dtype=np.float64)
+        self.tri = np.ascontiguousarray(topology.trianglePointIndices.reshape(-1, order="F"), dtype=np.uint32)
+        self.nbr = np.ascontiguousarray(d["triangleNeighbors"], dtype=np.int32)
+        self.fe = np.ascontiguousarray(d["forbiddenEdge"], dtype=np.int32)
+        self.beta = np.ascontiguousarray(medium.value("betaVolume"), dtype=np.float64)
+        self.beta_cells = np.ascontiguousarray(medium.value("betaCells"), dtype=np.float64)
+        self.clad = np.ascontiguousarray(medium.value("claddingCellTypes"), dtype=np.uint32)
+        self.refl = np.ascontiguousarray(medium.value("reflectivities"), dtype=np.float32)
+        self.nidx = np.ascontiguousarray(medium.value("refractiveIndices"), dtype=np.float32)
+        self.struct = Mesh(
+            topology.numberOfPoints, topology.numberOfTriangles, int(topology.levels), float(topology.thickness),
+            _ptr(self.points, C.c_double), _ptr(self.tri, C.c_uint32), _ptr(self.nbr, C.c_int32),
+            _ptr(self.fe, C.c_int32), _ptr(self.beta, C.c_double), _ptr(self.beta_cells, C.c_double),
+            _ptr(self.clad, C.c_uint32), int(medium.value("claddingNumber")),
+            float(medium.value("claddingAbsorption")), float(medium.value("nTot")),
+            float(medium.value("crystalTFluo")), _ptr(self.refl, C.c_float), _ptr(self.nidx, C.c_float))
+
+
+class NativeExperiment:
+    def __init__(self, *, sigma_a, sigma_e, min_rays, max_rays, max_sigma_a=None, max_sigma_e=None,
+                 mse_threshold=0.005, use_reflections=False, max_reflections=1, defensive_fraction=0.05):
+        self.sa = np.ascontiguousarray(np.atleast_1d(sigma_a), dtype=np.float64)
+        self.se = np.ascontiguousarray(np.atleast_1d(sigma_e), dtype=np.float64)
+        if self.sa.size != self.se.size or self.sa.size == 0:
+            raise ValueError("sigma_a and sigma_e must be non-empty and of equal length")
+        msa = float(self.sa.max()) if max_sigma_a is None else float(max_sigma_a)
+        mse_ = float(self.se.max()) if max_sigma_e is None else float(max_sigma_e)
+        self.struct = Experiment(int(min_rays), int(max_rays), self.sa.size, _ptr(self.sa, C.c_double),
+                                 _ptr(self.se, C.c_double), msa, mse_, float(mse_threshold),
+                                 int(bool(use_reflections)), int(max_reflections), float(defensive_fraction))
+
+
+def make_compute(*, n_samples, max_repetitions=1, adaptive_steps=1, min_sample_range=0, max_sample_range=None,
+                 rng_seed=5489, device=0, rays_per_thread=0, block_size=0, sample_batch=0):
+    last = n_samples - 1 if max_sample_range is None else int(max_sample_range)
+    return Compute(int(max_repetitions), int(adaptive_steps), int(min_sample_range), last, int(rng_seed) & 0xFFFFFFFF,
+                   int(device), int(rays_per_thread), int(block_size), int(sample_batch))
+
+
+class ResultArrays:
+    def __init__(self, n_samples):
+        self.phiAse = np.zeros(n_samples, dtype=np.float32)
+        self.mse = np.zeros(n_samples, dtype=np.float64)
+        self.totalRays = np.zeros(n_samples, dtype=np.uint32)
+        self.dndtAse = np.zeros(n_samples, dtype=np.float64)
+        self.standardError = np.zeros(n_samples, dtype=np.float64)
+        self.droppedRays = np.zeros(n_samples, dtype=np.uint32)
+        self.struct = ResultC(_ptr(self.phiAse, C.c_float), _ptr(self.mse, C.c_double),
+                              _ptr(self.totalRays, C.c_uint32), _ptr(self.dndtAse, C.c_double),
+                              _ptr(self.standardError, C.c_double), _ptr(self.droppedRays, C.c_uint32))
+
+
+class Context:
+    """Persistent device context (mesh resident between evaluations)."""
+
+    def __init__(self, mesh: NativeMesh, experiment: NativeExperiment, compute: Compute):
+        L = load()
+        self._mesh, self._exp, self._comp = mesh, experiment, compute
+        self.n_samples = mesh.struct.number_of_points * mesh.struct.number_of_levels
+        handle = C.c_void_p()
+        check(L.hase_context_create(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
+                                    C.byref(handle)))
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().hase_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def evaluate(self):
+        res = ResultArrays(self.n_samples)
+        check(load().hase_context_evaluate(self._h, C.byref(res.struct)))
+        return res
+
+    def update_beta(self, beta_volume, beta_cells=None):
+        bv = np.ascontiguousarray(beta_volume, dtype=np.float64)
+        bc = None if beta_cells is None else np.ascontiguousarray(beta_cells, dtype=np.float64)
+        check(load().hase_context_update_beta(self._h, _ptr(bv, C.c_double), _ptr(bc, C.c_double)))
+
+    def stats(self):
+        st = Stats()
+        check(load().hase_context_stats(self._h, C.byref(st)))
+        return st.asdict()
+
+    def trace_rays(self, start, prism, end, reflections=None, first_plane=None):
+        start = np.ascontiguousarray(start, dtype=np.float32).reshape(-1)
+        end = np.ascontiguousarray(end, dtype=np.float32).reshape(-1)
+        prism = np.ascontiguousarray(prism, dtype=np.uint32)
+        n = prism.size
+        nr = np.zeros(n, np.uint32) if reflections is None else np.ascontiguousarray(reflections, dtype=np.uint32)
+        fp = np.zeros(n, np.uint32) if first_plane is None else np.ascontiguousarray(first_plane, dtype=np.uint32)
+        if start.size != 3 * n or end.size != 3 * n or nr.size != n or fp.size != n:
+            raise ValueError("start/end must be (n, 3) and prism/reflections/first_plane (n,)")
+        out = {k: np.zeros(n, np.float32) for k in ("beta_len", "gain_len", "clad_len", "refl")}
+        steps = np.zeros(n, np.uint32)
+        status = np.zeros(n, np.int32)
+        check(load().hase_context_trace_rays(
+            self._h, n, _ptr(start, C.c_float), _ptr(prism, C.c_uint32), _ptr(end, C.c_float),
+            _ptr(nr, C.c_uint32), _ptr(fp, C.c_uint32), _ptr(out["beta_len"], C.c_float),
+            _ptr(out["gain_len"], C.c_float), _ptr(out["clad_len"], C.c_float), _ptr(out["refl"], C.c_float),
+            _ptr(steps, C.c_uint32), _ptr(status, C.c_int32)))
+        out["steps"] = steps
+        out["status"] = status
+        return out
+
+    def importance(self, sample):
+        R = 1 + 2 * ((self._exp.struct.max_reflections or 1) if self._exp.struct.use_reflections else 0)
+        P = self._mesh.struct.number_of_triangles * (self._mesh.struct.number_of_levels - 1)
+        out = np.zeros(R * P, dtype=np.float32)
+        check(load().hase_context_importance(self._h, int(sample), _ptr(out, C.c_float), out.size))
+        return out
+
+
+def calc_phi_ase(mesh: NativeMesh, experiment: NativeExperiment, compute: Compute):
+    """One-shot ``hase_calc_phi_ase`` call."""
+    n = mesh.struct.number_of_points * mesh.struct.number_of_levels
+    res = ResultArrays(n)
+    check(load().hase_calc_phi_ase(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
+                                   C.byref(res.struct)))
+    return res

@@ -1,0 +1,1053 @@
+// hase_ase.hip - MI355X (gfx950) importance-sampled ASE ray tracer behind the
+// C ABI of include/hase_ase.h.
+//
+// Device pipeline per batch of sample points:
+//   1. hase_importance_kernel (one work-group per sample): walks from every
+//      prism centroid (and its mirror slices) to the sample point, forms the
+//      defensive-mixture proposal and writes its FP64 inclusive prefix.  This
+//      fuses the legacy importanceSamplingPropagation and
+//      importanceSamplingDistribution passes (retired; shells at
+//      include/kernels/importanceSampling.hpp:10-18).
+//   2. hase_trace_kernel (work-groups of one sample's ray chunk): maps each ray
+//      to its (slice, prism) by systematic sampling on the device prefix (the
+//      legacy host loop mapRaysToPrisms, shell include/kernels/mapRaysToTets.hpp),
+//      draws the start point from a counter-based Philox stream, walks the
+//      prisms to the sample point in FP32 (legacy propagateRay) and reduces
+//      the estimator's sum and sum of squares in FP64 with wavefront shuffles.
+//   3. hase_finalize_kernel: pools the chunk partials in a fixed order, forms
+//      phiAse / mse / dndtAse in FP64 and compacts the samples that still need
+//      rays (adaptive escalation, include/core/types.hpp:333-354).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hase_ase.h"
+#include "prism_walk.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kPi = 3.14159265358979f;
+
+thread_local std::string g_last_error;
+
+struct HaseError : std::runtime_error {
+    int code;
+    HaseError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                    \
+    do {                                                                                   \
+        hipError_t err_ = (expr);                                                          \
+        if (err_ != hipSuccess)                                                            \
+            throw HaseError(HASE_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(err_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ device
+using namespace hase;
+
+struct SampleGeom {
+    float px, py;
+    int plane;   // level of the sample point (its height in layer units)
+};
+
+__device__ __forceinline__ SampleGeom sample_point(const float2* pts, uint32_t np, uint32_t sample)
+{
+    uint32_t p = sample % np, lv = sample / np;
+    float2 xy = pts[p];
+    return SampleGeom{xy.x, xy.y, (int)lv};
+}
+
+// ---------------------------------------------------- wave/block reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// inclusive scan within a 64-lane wavefront
+__device__ __forceinline__ double wave_inclusive_scan(double v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+struct ImportanceParams {
+    MeshDev m;
+    const float2* pts;
+    uint32_t sample_base;     // first sample of the batch
+    uint32_t slots;           // samples in this batch
+    uint32_t R;               // reflection slices
+    uint32_t n_lambda;
+    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a)
+    const float* c2;          // [n_lambda] nTot sigma_a
+    float inv_lambda;
+    double eps;               // defensive fraction
+    double Bsum;              // sum of beta * V
+    float* imp;               // [slots * R * P] scratch
+    double* prefix;           // [slots * R * P]
+    double* Q;                // [slots]
+    unsigned long long* counters;
+};
+
+__global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
+{
+    const uint32_t slot = blockIdx.x;
+    if (slot >= p.slots) return;
+    const MeshDev& m = p.m;
+    const uint32_t T = m.T, P = T * m.layers;
+    const size_t n = (size_t)p.R * P;
+    const uint32_t sample = p.sample_base + slot;
+    SampleGeom sp = sample_point(p.pts, m.np, sample);
+    float* imp = p.imp + (size_t)slot * n;
+    double* pre = p.prefix + (size_t)slot * n;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    __shared__ double red[kBlock / 64];
+    __shared__ double carry_s;
+    __shared__ double total_s;
+
+    double local = 0.0;
+    unsigned long long walks = 0, steps = 0;
+    for (size_t e = tid; e < n; e += kBlock) {
+        uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
+        uint32_t t = j % T, l = j / T;
+        float b = m.beta[j];
+        float value = 0.0f;
+        if (b > 0.0f) {
+            uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u];
+            float cx = (bits_to_float(q0.x) + bits_to_float(q0.z) + bits_to_float(q1.x)) * (1.0f / 3.0f);
+            float cy = (bits_to_float(q0.y) + bits_to_float(q0.w) + bits_to_float(q1.y)) * (1.0f / 3.0f);
+            float cz = (float)l + 0.5f;   // layer units
+            uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
+            float ez = (float)image_plane(sp.plane, (int)m.layers, nr, fp);
+            WalkOut w = walk_ray(m, cx, cy, cz, t, l, sp.px, sp.py, ez, nr, fp);
+            walks++;
+            steps += w.steps;
+            if (w.status == 0 && w.length > 0.0f) {
+                float g = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
+                value = m.bv[j] * g * w.refl / (4.0f * kPi * w.length * w.length);
+                if (!(value >= 0.0f) || isinf(value)) value = 0.0f;
+            }
+        }
+        imp[e] = value;
+        local += (double)value;
+    }
+    local = wave_sum(local);
+    if (lane == 0) red[wid] = local;
+    unsigned long long ws = wave_sum_u64(walks), ss = wave_sum_u64(steps);
+    if (lane == 0 && ws) {
+        atomicAdd(&p.counters[2], ws);
+        atomicAdd(&p.counters[3], ss);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double I = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) I += red[i];
+        total_s = I;
+        carry_s = 0.0;
+    }
+    __syncthreads();
+    const double I = total_s;
+    const bool use_imp = I > 0.0 && isfinite(I);
+    const double wI = use_imp ? (1.0 - p.eps) / I : 0.0;
+    const double wB = (use_imp ? p.eps : 1.0) / (p.Bsum * (double)p.R);
+    // pass 2 re-reads imp[e] from the thread that wrote it (same e = base + tid)
+    for (size_t base = 0; base < n; base += kBlock) {
+        size_t e = base + tid;
+        double q = 0.0;
+        if (e < n) {
+            uint32_t j = (uint32_t)(e % P);
+            q = wI * (double)imp[e] + wB * (double)m.bv[j];
+        }
+        double inc = wave_inclusive_scan(q, lane);
+        if (lane == 63) red[wid] = inc;
+        __syncthreads();
+        double off = carry_s;
+        for (int i = 0; i < wid; ++i) off += red[i];
+        if (e < n) pre[e] = off + inc;
+        __syncthreads();
+        if (tid == kBlock - 1) carry_s = off + inc;
+        __syncthreads();
+    }
+    if (tid == 0) p.Q[slot] = carry_s;
+}
+
+struct TraceParams {
+    MeshDev m;
+    const float2* pts;
+    uint32_t sample_base;
+    uint32_t R;
+    uint32_t n_lambda;
+    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a)
+    const float* c2;          // [n_lambda] nTot sigma_a
+    float inv_lambda;
+    uint32_t seed, launch, nrays;
+    uint32_t chunks;          // ray chunks per active sample
+    uint32_t rays_per_thread;
+    const uint32_t* active;   // [active count] batch slots
+    const double* prefix;
+    const double* Q;
+    double2* partial;         // [slots * max_chunks]
+    uint32_t* partial_drop;   // [slots * max_chunks]
+    uint32_t max_chunks;
+    unsigned long long* counters;
+};
+
+// blockIdx -> logical block so that consecutive logical blocks (the chunks of
+// one sample) run on one XCD and share its L2 (blocks are dealt to the 8 XCDs
+// round-robin).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb)
+{
+    const uint32_t xcds = 8u;
+    uint32_t q = nb / xcds, rem = nb % xcds, x = b % xcds, k = b / xcds;
+    return x < rem ? x * (q + 1u) + k : rem * (q + 1u) + (x - rem) * q + k;
+}
+
+__global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
+{
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t a = lb / p.chunks, chunk = lb % p.chunks;
+    const uint32_t slot = p.active[a];
+    const MeshDev& m = p.m;
+    const uint32_t T = m.T, P = T * m.layers;
+    const size_t n = (size_t)p.R * P;
+    const uint32_t sample = p.sample_base + slot;
+    const SampleGeom sp = sample_point(p.pts, m.np, sample);
+    const double* pre = p.prefix + (size_t)slot * n;
+    const double Q = p.Q[slot];
+    const double U = systematic_offset(p.seed, sample, p.launch);
+    const double scale = Q / (double)p.nrays;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t rays_per_block = kBlock * p.rays_per_thread;
+
+    double s1 = 0.0, s2 = 0.0;
+    unsigned long long steps = 0, rays = 0;
+    uint32_t drops = 0;
+    for (uint32_t i = 0; i < p.rays_per_thread; ++i) {
+        uint32_t ray = chunk * rays_per_block + i * kBlock + tid;
+        if (ray >= p.nrays) break;
+        double target = ((double)ray + U) * scale;
+        // first entry whose prefix exceeds the target
+        size_t lo = 0, hi = n;
+        while (lo < hi) {
+            size_t mid = (lo + hi) >> 1;
+            if (pre[mid] > target) hi = mid; else lo = mid + 1;
+        }
+        size_t e = lo < n ? lo : n - 1;
+        double q = pre[e] - (e ? pre[e - 1] : 0.0);
+        uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
+        uint32_t t = j % T, l = j / T;
+        uint32_t rnd[4];
+        philox4x32_10(ray, sample, p.launch, kTagRay, p.seed, kKeyTag, rnd);
+        float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
+        if (u1 + u2 > 1.0f) { u1 = 1.0f - u1; u2 = 1.0f - u2; }
+        uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u];
+        float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
+        float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
+        float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
+        float ax = x0 + u1 * (x1 - x0) + u2 * (x2 - x0);
+        float ay = y0 + u1 * (y1 - y0) + u2 * (y2 - y0);
+        float az = (float)l + u3;   // layer units
+        uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
+        float ez = (float)image_plane(sp.plane, (int)m.layers, nr, fp);
+        WalkOut w = walk_ray(m, ax, ay, az, t, l, sp.px, sp.py, ez, nr, fp);
+        steps += w.steps;
+        rays++;
+        if (w.status != 0 || !(w.length > 0.0f)) {
+            drops++;
+            continue;
+        }
+        float g = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
+        float f = g * w.refl * m.bv[j] / (4.0f * kPi * w.length * w.length);
+        double X = (double)f * (Q / q);
+        if (!isfinite(X)) {
+            drops++;
+            continue;
+        }
+        s1 += X;
+        s2 += X * X;
+    }
+    __shared__ double r1[kBlock / 64], r2[kBlock / 64];
+    __shared__ uint32_t rd[kBlock / 64];
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    unsigned long long ws = wave_sum_u64(steps), wr = wave_sum_u64(rays);
+    unsigned long long wd = wave_sum_u64((unsigned long long)drops);
+    if (lane == 0) {
+        r1[wid] = s1;
+        r2[wid] = s2;
+        rd[wid] = (uint32_t)wd;
+        if (wr) {
+            atomicAdd(&p.counters[0], wr);
+            atomicAdd(&p.counters[1], ws);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double a1 = 0.0, a2 = 0.0;
+        uint32_t ad = 0;
+        for (int i = 0; i < kBlock / 64; ++i) { a1 += r1[i]; a2 += r2[i]; ad += rd[i]; }
+        size_t at = (size_t)slot * p.max_chunks + chunk;
+        p.partial[at] = make_double2(a1, a2);
+        p.partial_drop[at] = ad;
+    }
+}
+
+struct FinalizeParams {
+    uint32_t slots, chunks, max_chunks, nrays;
+    const double2* partial;
+    const uint32_t* partial_drop;
+    double* s1;
+    double* s2;
+    uint32_t* N;
+    uint32_t* dropped;
+    int32_t* active_flag;
+    uint32_t* active_out;
+    uint32_t* active_count;
+    double mse_threshold;
+    double phi_scale;           // nTot / tau
+    const double* beta_cells;   // [slots] of this batch
+    double dndt_a, dndt_b;      // dndt = (beta * a - b) * phi
+    float* phi;
+    double* mse;
+    double* se;
+};
+
+__global__ void hase_finalize_kernel(FinalizeParams p)
+{
+    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= p.slots || !p.active_flag[slot]) return;
+    double a1 = p.s1[slot], a2 = p.s2[slot];
+    uint32_t drop = p.dropped[slot];
+    for (uint32_t c = 0; c < p.chunks; ++c) {
+        double2 v = p.partial[(size_t)slot * p.max_chunks + c];
+        a1 += v.x;
+        a2 += v.y;
+        drop += p.partial_drop[(size_t)slot * p.max_chunks + c];
+    }
+    uint32_t N = p.N[slot] + p.nrays;
+    p.s1[slot] = a1;
+    p.s2[slot] = a2;
+    p.N[slot] = N;
+    p.dropped[slot] = drop;
+    // relative standard error, src/core/calcForwardPhiAse.cpp:81-98; mse is its square
+    double nn = (double)N, mse;
+    if (N < 2u || !isfinite(a1) || !isfinite(a2)) mse = 1.79769313486231570e308;
+    else if (a1 == 0.0) mse = __longlong_as_double(0x7ff8000000000000LL);
+    else {
+        double rv = (nn * a2 / (a1 * a1) - 1.0) / nn;
+        mse = rv > 0.0 ? rv : 0.0;
+    }
+    double phi = p.phi_scale * a1 / nn;
+    p.phi[slot] = (float)phi;
+    p.mse[slot] = mse;
+    p.se[slot] = (mse >= 0.0 && mse < 1e300) ? sqrt(mse) * fabs(phi) : mse;
+    bool converged = drop == 0u && isfinite(mse) && mse <= p.mse_threshold;
+    if (converged) {
+        p.active_flag[slot] = 0;
+    } else {
+        uint32_t at = atomicAdd(p.active_count, 1u);
+        p.active_out[at] = slot;
+    }
+}
+
+__global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* start, const uint32_t* prism,
+                                        const float* end, const uint32_t* nrefl, const uint32_t* first,
+                                        float* bl, float* gl, float* cl, float* refl, uint32_t* steps,
+                                        int32_t* status)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t j = prism[i];
+    uint32_t t = j % m.T, l = j / m.T;
+    WalkOut w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
+                         end[3 * i + 2], nrefl[i], first[i]);
+    bl[i] = w.beta_len;
+    gl[i] = w.gain_len;
+    cl[i] = w.clad_len;
+    refl[i] = w.refl;
+    steps[i] = w.steps;
+    status[i] = w.status;
+}
+
+// ------------------------------------------------------------------- host
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw HaseError(HASE_E_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                               " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void upload(const T* src, size_t count, hipStream_t st)
+    {
+        alloc(count);
+        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+};
+
+struct Schedule {
+    uint32_t min_rays, max_rays, steps, reps;
+    uint32_t target(uint32_t k) const
+    {
+        if (max_rays <= min_rays || steps == 0) return min_rays;
+        if (k >= steps) return max_rays;
+        double growth = std::pow((double)max_rays / (double)min_rays, 1.0 / (double)steps);
+        double t = std::ceil((double)min_rays * std::pow(growth, (double)k));
+        t = std::min(std::max(t, (double)min_rays), (double)max_rays);
+        return (uint32_t)t;
+    }
+};
+
+}  // namespace
+
+struct hase_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t np = 0, T = 0, L = 0, P = 0, S = 0;
+    double h = 0.0;
+    double n_tot = 0.0, tau = 0.0;
+    double clad_alpha = 0.0;
+    bool has_refl = false, has_nidx = false;
+    float nidx[4] = {1, 1, 1, 1};
+    std::vector<double> tri_area;      // [T]
+    std::vector<uint32_t> clad_flag;   // [T]
+    std::vector<double> beta_host;     // [P]
+    std::vector<double> beta_cells;    // [S]
+    double Bsum = 0.0;
+    hase_experiment_t exp{};
+    std::vector<double> sigma_a, sigma_e;
+    hase_compute_t comp{};
+    DevBuf<uint4> rec;
+    DevBuf<float2> pts;
+    DevBuf<float> beta, bv, refl, c1, c2;
+    DevBuf<float> imp;
+    DevBuf<double> prefix, Q, s1, s2, bcells;
+    DevBuf<double2> partial;
+    DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count;
+    DevBuf<int32_t> active_flag;
+    DevBuf<float> phi;
+    DevBuf<double> mse, se;
+    DevBuf<unsigned long long> counters;
+    hase_stats_t stats{};
+    std::vector<hipEvent_t> events;
+
+    ~hase_context()
+    {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void validate(const hase_mesh_t* mesh, const hase_experiment_t* e, const hase_compute_t* c)
+{
+    if (!mesh || !e || !c) throw HaseError(HASE_E_INVALID, "mesh, experiment and compute must not be NULL");
+    if (mesh->number_of_points < 3 || mesh->number_of_triangles < 1)
+        throw HaseError(HASE_E_INVALID, "mesh needs at least 3 points and 1 triangle");
+    if (mesh->number_of_levels < 2) throw HaseError(HASE_E_INVALID, "number_of_levels must be at least 2");
+    if (!(mesh->thickness > 0.0) || !std::isfinite(mesh->thickness))
+        throw HaseError(HASE_E_INVALID, "thickness must be positive");
+    if (!mesh->points || !mesh->triangle_point_indices || !mesh->triangle_neighbors || !mesh->forbidden_edge ||
+        !mesh->beta_volume)
+        throw HaseError(HASE_E_INVALID, "mesh arrays points, triangle_point_indices, triangle_neighbors, "
+                                        "forbidden_edge and beta_volume are required");
+    if (!(mesh->n_tot > 0.0) || !(mesh->crystal_t_fluo > 0.0))
+        throw HaseError(HASE_E_INVALID, "n_tot and crystal_t_fluo must be positive");
+    if (e->min_rays_per_sample == 0) throw HaseError(HASE_E_INVALID, "min_rays_per_sample must be positive");
+    if (e->max_rays_per_sample < e->min_rays_per_sample)
+        throw HaseError(HASE_E_INVALID, "max_rays_per_sample must be >= min_rays_per_sample");
+    if (e->number_of_wavelengths == 0 || !e->sigma_a || !e->sigma_e)
+        throw HaseError(HASE_E_INVALID, "at least one wavelength (sigma_a, sigma_e) is required");
+    if (!(e->mse_threshold >= 0.0)) throw HaseError(HASE_E_INVALID, "mse_threshold must be >= 0");
+    if (!(e->defensive_fraction >= 0.0 && e->defensive_fraction < 1.0))
+        throw HaseError(HASE_E_INVALID, "defensive_fraction must lie in [0, 1)");
+    uint64_t S = (uint64_t)mesh->number_of_points * mesh->number_of_levels;
+    if (S > 0xFFFFFFFFull) throw HaseError(HASE_E_INVALID, "too many sample points");
+    if (c->max_sample_range < c->min_sample_range || c->max_sample_range >= S)
+        throw HaseError(HASE_E_INVALID, "sample range must satisfy min <= max < number_of_points * number_of_levels");
+    if (c->max_repetitions == 0) throw HaseError(HASE_E_INVALID, "max_repetitions must be >= 1");
+    uint64_t P = (uint64_t)mesh->number_of_triangles * (mesh->number_of_levels - 1);
+    if (P * 3u > 0xFFFFFFFFull) throw HaseError(HASE_E_INVALID, "mesh too large");
+}
+
+void set_params(hase_context* ctx, const hase_experiment_t* e, const hase_compute_t* c)
+{
+    ctx->exp = *e;
+    ctx->sigma_a.assign(e->sigma_a, e->sigma_a + e->number_of_wavelengths);
+    ctx->sigma_e.assign(e->sigma_e, e->sigma_e + e->number_of_wavelengths);
+    ctx->exp.sigma_a = ctx->sigma_a.data();
+    ctx->exp.sigma_e = ctx->sigma_e.data();
+    ctx->comp = *c;
+    std::vector<float> c1(e->number_of_wavelengths), c2(e->number_of_wavelengths);
+    for (uint32_t i = 0; i < e->number_of_wavelengths; ++i) {
+        c1[i] = (float)(ctx->n_tot * (e->sigma_e[i] + e->sigma_a[i]));
+        c2[i] = (float)(ctx->n_tot * e->sigma_a[i]);
+    }
+    ctx->c1.upload(c1.data(), c1.size(), ctx->stream);
+    ctx->c2.upload(c2.data(), c2.size(), ctx->stream);
+}
+
+void upload_beta(hase_context* ctx, const double* beta_volume, const double* beta_cells)
+{
+    const uint32_t T = ctx->T, P = ctx->P;
+    ctx->beta_host.assign(beta_volume, beta_volume + P);
+    std::vector<float> b(P), bv(P);
+    double B = 0.0;
+    for (uint32_t j = 0; j < P; ++j) {
+        double v = beta_volume[j];
+        if (!std::isfinite(v)) throw HaseError(HASE_E_INVALID, "beta_volume must be finite");
+        b[j] = (float)v;
+        double w = v > 0.0 ? v * ctx->tri_area[j % T] * ctx->h : 0.0;
+        bv[j] = (float)w;
+        B += w;
+    }
+    ctx->Bsum = B;
+    ctx->beta.upload(b.data(), P, ctx->stream);
+    ctx->bv.upload(bv.data(), P, ctx->stream);
+    if (beta_cells) {
+        ctx->beta_cells.assign(beta_cells, beta_cells + ctx->S);
+        ctx->bcells.upload(ctx->beta_cells.data(), ctx->S, ctx->stream);
+    } else if (ctx->beta_cells.empty()) {
+        ctx->beta_cells.assign(ctx->S, 0.0);
+        ctx->bcells.upload(ctx->beta_cells.data(), ctx->S, ctx->stream);
+    }
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase_experiment_t* e,
+                                             const hase_compute_t* c)
+{
+    validate(mesh, e, c);
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    if (count <= 0) throw HaseError(HASE_E_DEVICE, "no HIP device is visible");
+    if (c->device < 0 || c->device >= count) throw HaseError(HASE_E_INVALID, "device ordinal out of range");
+    HIP_CHECK(hipSetDevice(c->device));
+    auto ctx = std::make_unique<hase_context>();
+    ctx->device = c->device;
+    HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const uint32_t np = mesh->number_of_points, T = mesh->number_of_triangles, L = mesh->number_of_levels;
+    ctx->np = np; ctx->T = T; ctx->L = L; ctx->P = T * (L - 1); ctx->S = np * L;
+    ctx->h = mesh->thickness;
+    ctx->n_tot = mesh->n_tot;
+    ctx->tau = mesh->crystal_t_fluo;
+    ctx->clad_alpha = mesh->cladding_absorption;
+    const double* px = mesh->points;
+    const double* py = mesh->points + np;
+    // triangle records: {x0,y0,x1,y1} {x2,y2,n0,n1} {n2,clad,0,0}; n_k = neighbour<<2 | shared edge
+    std::vector<uint4> rec(3u * (size_t)T);
+    ctx->tri_area.resize(T);
+    ctx->clad_flag.resize(T);
+    for (uint32_t t = 0; t < T; ++t) {
+        uint32_t v[3];
+        uint32_t nw[3];
+        for (int k = 0; k < 3; ++k) {
+            v[k] = mesh->triangle_point_indices[t + (size_t)k * T];
+            if (v[k] >= np) throw HaseError(HASE_E_INVALID, "triangle point index out of range");
+            int32_t nb = mesh->triangle_neighbors[t + (size_t)k * T];
+            int32_t fe = mesh->forbidden_edge[t + (size_t)k * T];
+            if (nb >= 0) {
+                if ((uint32_t)nb >= T || fe < 0 || fe > 2)
+                    throw HaseError(HASE_E_INVALID, "triangle_neighbors / forbidden_edge out of range");
+                nw[k] = ((uint32_t)nb << 2) | (uint32_t)fe;
+            } else {
+                nw[k] = 0xFFFFFFFFu;
+            }
+        }
+        double cr = (px[v[1]] - px[v[0]]) * (py[v[2]] - py[v[0]]) - (py[v[1]] - py[v[0]]) * (px[v[2]] - px[v[0]]);
+        if (!(cr > 0.0))
+            throw HaseError(HASE_E_INVALID, "triangles must be non-degenerate and counter-clockwise");
+        ctx->tri_area[t] = 0.5 * cr;
+        uint32_t clad = mesh->cladding_cell_types && mesh->cladding_cell_types[t] == mesh->cladding_number ? 1u : 0u;
+        ctx->clad_flag[t] = clad;
+        auto fb = [](double x) { float f = (float)x; uint32_t u; std::memcpy(&u, &f, 4); return u; };
+        rec[3u * t] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
+        rec[3u * t + 1u] = make_uint4(fb(px[v[2]]), fb(py[v[2]]), nw[0], nw[1]);
+        rec[3u * t + 2u] = make_uint4(nw[2], clad, 0u, 0u);
+    }
+    ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
+    std::vector<float2> pts(np);
+    for (uint32_t i = 0; i < np; ++i) pts[i] = make_float2((float)px[i], (float)py[i]);
+    ctx->pts.upload(pts.data(), np, ctx->stream);
+    if (mesh->reflectivities) {
+        ctx->has_refl = true;
+        ctx->refl.upload(mesh->reflectivities, 2u * (size_t)T, ctx->stream);
+    }
+    if (mesh->refractive_indices) {
+        ctx->has_nidx = true;
+        for (int i = 0; i < 4; ++i) ctx->nidx[i] = mesh->refractive_indices[i];
+    }
+    ctx->counters.alloc(4);
+    set_params(ctx.get(), e, c);
+    upload_beta(ctx.get(), mesh->beta_volume, mesh->beta_cells);
+    return ctx;
+}
+
+MeshDev mesh_view(const hase_context* ctx)
+{
+    MeshDev m;
+    m.rec = ctx->rec.p;
+    m.beta = ctx->beta.p;
+    m.bv = ctx->bv.p;
+    m.refl = ctx->has_refl ? ctx->refl.p : nullptr;
+    m.nin_b = ctx->nidx[0]; m.nout_b = ctx->nidx[1]; m.nin_t = ctx->nidx[2]; m.nout_t = ctx->nidx[3];
+    m.has_nidx = ctx->has_nidx ? 1 : 0;
+    m.T = ctx->T;
+    m.layers = ctx->L - 1;
+    m.np = ctx->np;
+    m.h = (float)ctx->h;
+    m.clad_alpha = (float)ctx->clad_alpha;
+    return m;
+}
+
+uint32_t slices(const hase_context* ctx)
+{
+    uint32_t K = ctx->exp.use_reflections ? (ctx->exp.max_reflections ? ctx->exp.max_reflections : 1u) : 0u;
+    return 1u + 2u * K;
+}
+
+hipEvent_t record(hase_context* ctx)
+{
+    hipEvent_t ev;
+    HIP_CHECK(hipEventCreate(&ev));
+    HIP_CHECK(hipEventRecord(ev, ctx->stream));
+    ctx->events.push_back(ev);
+    return ev;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+void evaluate(hase_context* ctx, hase_result_t* res)
+{
+    if (!res || !res->phi_ase || !res->mse || !res->total_rays || !res->dndt_ase)
+        throw HaseError(HASE_E_INVALID, "result arrays phi_ase, mse, total_rays and dndt_ase are required");
+    HIP_CHECK(hipSetDevice(ctx->device));
+    const hase_compute_t& c = ctx->comp;
+    const hase_experiment_t& e = ctx->exp;
+    const uint32_t S = ctx->S, P = ctx->P, R = slices(ctx);
+    const size_t n_ent = (size_t)R * P;
+    for (uint32_t s = 0; s < S; ++s) {
+        res->phi_ase[s] = 0.0f;
+        res->mse[s] = 0.0;
+        res->total_rays[s] = 0;
+        res->dndt_ase[s] = 0.0;
+        if (res->standard_error) res->standard_error[s] = 0.0;
+        if (res->dropped_rays) res->dropped_rays[s] = 0;
+    }
+    for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+    ctx->events.clear();
+    ctx->stats = hase_stats_t{};
+    if (!(ctx->Bsum > 0.0)) return;   // no emitting volume: the estimate is zero
+
+    const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
+    const uint32_t rays_per_block = kBlock * rpt;
+    const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
+    const uint32_t first = c.min_sample_range, last = c.max_sample_range;
+    const uint32_t total = last - first + 1u;
+    uint32_t batch = c.sample_batch;
+    if (batch == 0) {
+        // keep the importance prefix (8 B) and scratch (4 B) of a batch within ~6 GiB
+        size_t per_sample = n_ent * 12u + (size_t)max_chunks * 12u + 64u;
+        size_t budget = (size_t)6 << 30;
+        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
+    }
+    batch = std::min(batch, total);
+    ctx->imp.alloc((size_t)batch * n_ent);
+    ctx->prefix.alloc((size_t)batch * n_ent);
+    ctx->Q.alloc(batch);
+    ctx->s1.alloc(batch);
+    ctx->s2.alloc(batch);
+    ctx->N.alloc(batch);
+    ctx->dropped.alloc(batch);
+    ctx->active_flag.alloc(batch);
+    ctx->active_list.alloc(batch);
+    ctx->active_next.alloc(batch);
+    ctx->active_count.alloc(1);
+    ctx->partial.alloc((size_t)batch * max_chunks);
+    ctx->partial_drop.alloc((size_t)batch * max_chunks);
+    ctx->phi.alloc(batch);
+    ctx->mse.alloc(batch);
+    ctx->se.alloc(batch);
+    HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+
+    const MeshDev m = mesh_view(ctx);
+    Schedule sched{e.min_rays_per_sample, e.max_rays_per_sample, c.adaptive_steps, c.max_repetitions};
+    std::vector<uint32_t> iota(batch);
+    for (uint32_t i = 0; i < batch; ++i) iota[i] = i;
+    std::vector<float> h_phi(batch);
+    std::vector<double> h_mse(batch), h_se(batch);
+    std::vector<uint32_t> h_N(batch), h_drop(batch);
+    std::vector<int32_t> ones(batch, 1);
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_imp, ev_trace, ev_fin;
+    hipEvent_t ev_begin = record(ctx);
+
+    for (uint32_t b0 = first; b0 <= last; b0 += batch) {
+        const uint32_t slots = std::min(batch, last - b0 + 1u);
+        HIP_CHECK(hipMemsetAsync(ctx->s1.p, 0, slots * sizeof(double), ctx->stream));
+        HIP_CHECK(hipMemsetAsync(ctx->s2.p, 0, slots * sizeof(double), ctx->stream));
+        HIP_CHECK(hipMemsetAsync(ctx->N.p, 0, slots * sizeof(uint32_t), ctx->stream));
+        HIP_CHECK(hipMemsetAsync(ctx->dropped.p, 0, slots * sizeof(uint32_t), ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(ctx->active_flag.p, ones.data(), slots * sizeof(int32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(ctx->active_list.p, iota.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+
+        ImportanceParams ip;
+        ip.m = m;
+        ip.pts = ctx->pts.p;
+        ip.sample_base = b0;
+        ip.slots = slots;
+        ip.R = R;
+        ip.n_lambda = e.number_of_wavelengths;
+        ip.c1 = ctx->c1.p;
+        ip.c2 = ctx->c2.p;
+        ip.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+        ip.eps = e.defensive_fraction;
+        ip.Bsum = ctx->Bsum;
+        ip.imp = ctx->imp.p;
+        ip.prefix = ctx->prefix.p;
+        ip.Q = ctx->Q.p;
+        ip.counters = ctx->counters.p;
+        hipEvent_t i0 = record(ctx);
+        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+        HIP_CHECK(hipGetLastError());
+        ev_imp.emplace_back(i0, record(ctx));
+        ctx->stats.importance_launches++;
+
+        uint32_t active = slots;
+        uint32_t N = 0, k = 0, extra = 0, launch = 0;
+        while (active > 0) {
+            uint32_t target = sched.target(k);
+            uint32_t nrays;
+            if (target > N) {
+                nrays = target - N;
+            } else if (N >= sched.max_rays || target >= sched.max_rays) {
+                if (extra + 1u >= sched.reps) break;
+                extra++;
+                nrays = sched.max_rays;
+            } else {
+                k++;
+                continue;
+            }
+            const uint32_t chunks = (nrays + rays_per_block - 1) / rays_per_block;
+            TraceParams tp;
+            tp.m = m;
+            tp.pts = ctx->pts.p;
+            tp.sample_base = b0;
+            tp.R = R;
+            tp.n_lambda = e.number_of_wavelengths;
+            tp.c1 = ctx->c1.p;
+            tp.c2 = ctx->c2.p;
+            tp.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+            tp.seed = c.rng_seed;
+            tp.launch = launch;
+            tp.nrays = nrays;
+            tp.chunks = chunks;
+            tp.rays_per_thread = rpt;
+            tp.active = ctx->active_list.p;
+            tp.prefix = ctx->prefix.p;
+            tp.Q = ctx->Q.p;
+            tp.partial = ctx->partial.p;
+            tp.partial_drop = ctx->partial_drop.p;
+            tp.max_chunks = max_chunks;
+            tp.counters = ctx->counters.p;
+            hipEvent_t t0 = record(ctx);
+            hipLaunchKernelGGL(hase_trace_kernel, dim3(active * chunks), dim3(kBlock), 0, ctx->stream, tp);
+            HIP_CHECK(hipGetLastError());
+            ev_trace.emplace_back(t0, record(ctx));
+            ctx->stats.trace_launches++;
+
+            HIP_CHECK(hipMemsetAsync(ctx->active_count.p, 0, sizeof(uint32_t), ctx->stream));
+            FinalizeParams fp;
+            fp.slots = slots;
+            fp.chunks = chunks;
+            fp.max_chunks = max_chunks;
+            fp.nrays = nrays;
+            fp.partial = ctx->partial.p;
+            fp.partial_drop = ctx->partial_drop.p;
+            fp.s1 = ctx->s1.p;
+            fp.s2 = ctx->s2.p;
+            fp.N = ctx->N.p;
+            fp.dropped = ctx->dropped.p;
+            fp.active_flag = ctx->active_flag.p;
+            fp.active_out = ctx->active_next.p;
+            fp.active_count = ctx->active_count.p;
+            fp.mse_threshold = e.mse_threshold;
+            fp.phi_scale = ctx->n_tot / ctx->tau;
+            fp.beta_cells = ctx->bcells.p + b0;
+            fp.dndt_a = e.max_sigma_e + e.max_sigma_a;
+            fp.dndt_b = e.max_sigma_a;
+            fp.phi = ctx->phi.p;
+            fp.mse = ctx->mse.p;
+            fp.se = ctx->se.p;
+            hipEvent_t f0 = record(ctx);
+            hipLaunchKernelGGL(hase_finalize_kernel, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, fp);
+            HIP_CHECK(hipGetLastError());
+            ev_fin.emplace_back(f0, record(ctx));
+            HIP_CHECK(hipMemcpyAsync(&active, ctx->active_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            std::swap(ctx->active_list.p, ctx->active_next.p);
+            N += nrays;
+            launch++;
+            if (target < sched.max_rays) k++;
+        }
+        HIP_CHECK(hipMemcpyAsync(h_phi.data(), ctx->phi.p, slots * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(h_mse.data(), ctx->mse.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(h_se.data(), ctx->se.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(h_N.data(), ctx->N.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(h_drop.data(), ctx->dropped.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint32_t i = 0; i < slots; ++i) {
+            uint32_t s = b0 + i;
+            double phi = (double)h_phi[i];
+            res->phi_ase[s] = h_phi[i];
+            res->mse[s] = h_mse[i];
+            res->total_rays[s] = h_N[i];
+            res->dndt_ase[s] = (ctx->beta_cells[s] * (e.max_sigma_e + e.max_sigma_a) - e.max_sigma_a) * phi;
+            if (res->standard_error) res->standard_error[s] = h_se[i];
+            if (res->dropped_rays) res->dropped_rays[s] = h_drop[i];
+        }
+        if (last - b0 < batch) break;
+    }
+    hipEvent_t ev_end = record(ctx);
+    HIP_CHECK(hipEventSynchronize(ev_end));
+    for (auto& pr : ev_imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : ev_trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : ev_fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
+    ctx->stats.total_ms = elapsed(ev_begin, ev_end);
+    unsigned long long cnt[4];
+    HIP_CHECK(hipMemcpy(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    ctx->stats.rays = cnt[0];
+    ctx->stats.ray_steps = cnt[1];
+    ctx->stats.importance_walks = cnt[2];
+    ctx->stats.importance_steps = cnt[3];
+    uint64_t dropped = 0;
+    for (uint32_t s = first; s <= last; ++s)
+        if (res->dropped_rays) dropped += res->dropped_rays[s];
+    ctx->stats.dropped_rays = dropped;
+}
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        g_last_error.clear();
+        f();
+        return HASE_OK;
+    } catch (const HaseError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return HASE_E_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HASE_E_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown failure";
+        return HASE_E_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int hase_calc_phi_ase(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,
+                      hase_result_t* result)
+{
+    return guarded([&] {
+        auto ctx = create_context(mesh, experiment, compute);
+        evaluate(ctx.get(), result);
+    });
+}
+
+int hase_context_create(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,
+                        hase_context_t** out_context)
+{
+    return guarded([&] {
+        if (!out_context) throw HaseError(HASE_E_INVALID, "out_context must not be NULL");
+        *out_context = nullptr;
+        *out_context = create_context(mesh, experiment, compute).release();
+    });
+}
+
+int hase_context_update_beta(hase_context_t* context, const double* beta_volume, const double* beta_cells)
+{
+    return guarded([&] {
+        if (!context || !beta_volume) throw HaseError(HASE_E_INVALID, "context and beta_volume must not be NULL");
+        HIP_CHECK(hipSetDevice(context->device));
+        upload_beta(context, beta_volume, beta_cells);
+    });
+}
+
+int hase_context_evaluate(hase_context_t* context, hase_result_t* result)
+{
+    return guarded([&] {
+        if (!context) throw HaseError(HASE_E_INVALID, "context must not be NULL");
+        evaluate(context, result);
+    });
+}
+
+int hase_context_stats(const hase_context_t* context, hase_stats_t* stats)
+{
+    return guarded([&] {
+        if (!context || !stats) throw HaseError(HASE_E_INVALID, "context and stats must not be NULL");
+        *stats = context->stats;
+    });
+}
+
+void hase_context_destroy(hase_context_t* context)
+{
+    if (context) {
+        (void)hipSetDevice(context->device);
+        (void)hipStreamSynchronize(context->stream);
+        delete context;
+    }
+}
+
+int hase_context_trace_rays(hase_context_t* context, uint32_t n, const float* start, const uint32_t* start_prism,
+                            const float* end, const uint32_t* reflections, const uint32_t* first_plane,
+                            float* beta_length, float* gain_length, float* clad_length, float* reflectance,
+                            uint32_t* steps, int32_t* status)
+{
+    return guarded([&] {
+        if (!context) throw HaseError(HASE_E_INVALID, "context must not be NULL");
+        if (n == 0) return;
+        if (!start || !start_prism || !end || !reflections || !first_plane || !beta_length || !gain_length ||
+            !clad_length || !reflectance || !steps || !status)
+            throw HaseError(HASE_E_INVALID, "all ray arrays are required");
+        for (uint32_t i = 0; i < n; ++i)
+            if (start_prism[i] >= context->P) throw HaseError(HASE_E_INVALID, "start_prism out of range");
+        HIP_CHECK(hipSetDevice(context->device));
+        hipStream_t st = context->stream;
+        // heights to layer units; an end point within 1e-5 layers of a plane is on it
+        std::vector<float> hs(start, start + 3u * (size_t)n), he(end, end + 3u * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            hs[3 * i + 2] = (float)((double)start[3 * i + 2] / context->h);
+            double ze = (double)end[3 * i + 2] / context->h, zr = std::nearbyint(ze);
+            he[3 * i + 2] = (float)(std::fabs(ze - zr) < 1e-5 ? zr : ze);
+        }
+        DevBuf<float> d_start, d_end, d_bl, d_gl, d_cl, d_rf;
+        DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps;
+        DevBuf<int32_t> d_status;
+        d_start.upload(hs.data(), 3u * (size_t)n, st);
+        d_end.upload(he.data(), 3u * (size_t)n, st);
+        d_prism.upload(start_prism, n, st);
+        d_nr.upload(reflections, n, st);
+        d_fp.upload(first_plane, n, st);
+        d_bl.alloc(n); d_gl.alloc(n); d_cl.alloc(n); d_rf.alloc(n); d_steps.alloc(n); d_status.alloc(n);
+        hipLaunchKernelGGL(hase_single_rays_kernel, dim3((n + 127) / 128), dim3(128), 0, st, mesh_view(context), n,
+                           d_start.p, d_prism.p, d_end.p, d_nr.p, d_fp.p, d_bl.p, d_gl.p, d_cl.p, d_rf.p, d_steps.p,
+                           d_status.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(beta_length, d_bl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(gain_length, d_gl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(clad_length, d_cl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(reflectance, d_rf.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(steps, d_steps.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(status, d_status.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+int hase_context_importance(hase_context_t* context, uint32_t sample, float* importance, uint32_t capacity)
+{
+    return guarded([&] {
+        if (!context || !importance) throw HaseError(HASE_E_INVALID, "context and importance must not be NULL");
+        if (sample >= context->S) throw HaseError(HASE_E_INVALID, "sample out of range");
+        const uint32_t R = slices(context);
+        const size_t n_ent = (size_t)R * context->P;
+        if (capacity < n_ent) throw HaseError(HASE_E_INVALID, "importance capacity must be at least R * P");
+        if (!(context->Bsum > 0.0)) {
+            std::fill(importance, importance + n_ent, 0.0f);
+            return;
+        }
+        HIP_CHECK(hipSetDevice(context->device));
+        context->imp.alloc(std::max(context->imp.n, n_ent));
+        context->prefix.alloc(std::max(context->prefix.n, n_ent));
+        context->Q.alloc(std::max<size_t>(context->Q.n, 1));
+        const hase_experiment_t& e = context->exp;
+        ImportanceParams ip;
+        ip.m = mesh_view(context);
+        ip.pts = context->pts.p;
+        ip.sample_base = sample;
+        ip.slots = 1;
+        ip.R = R;
+        ip.n_lambda = e.number_of_wavelengths;
+        ip.c1 = context->c1.p;
+        ip.c2 = context->c2.p;
+        ip.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+        ip.eps = e.defensive_fraction;
+        ip.Bsum = context->Bsum;
+        ip.imp = context->imp.p;
+        ip.prefix = context->prefix.p;
+        ip.Q = context->Q.p;
+        ip.counters = context->counters.p;
+        hipLaunchKernelGGL(hase_importance_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(importance, context->imp.p, n_ent * sizeof(float), hipMemcpyDeviceToHost,
+                                 context->stream));
+        HIP_CHECK(hipStreamSynchronize(context->stream));
+    });
+}
+
+const char* hase_last_error(void) { return g_last_error.c_str(); }
+
+int hase_abi_version(void) { return HASE_ABI_VERSION; }
+
+const char* hase_build_info(void)
+{
+    return "hase_ase gfx950 wave64 fp32-geometry fp64-statistics";
+}
+
+}  // extern "C"

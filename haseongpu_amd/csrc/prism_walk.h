@@ -1,0 +1,232 @@
+// prism_walk.h - FP32 prism walk of the ASE ray tracer, shared by the HIP
+// kernels and host-side checks (the functions are __host__ __device__).
+//
+// The walk replaces the legacy propagateRay (retired: src/kernels/propagateRay.cpp
+// is an empty translation unit) and follows the gain and cladding rules of
+// include/kernels/forward/rayWalk.hpp:170-208 and the surface reflectance of
+// include/kernels/reflection.hpp:21-47.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#define HASE_HD __host__ __device__
+
+namespace hase {
+
+constexpr uint32_t kKeyTag = 0x48415345u;
+constexpr uint32_t kTagRay = 0u;
+constexpr uint32_t kTagOffset = 1u;
+constexpr uint32_t kMaxSteps = 10000u;
+
+HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+struct MeshDev {
+    const uint4* rec;      // 3 records per triangle (see pack_triangles)
+    const float* beta;     // [P] level-major
+    const float* bv;       // [P] beta * prism volume
+    const float* refl;     // [2T] or nullptr
+    float nin_b, nout_b, nin_t, nout_t;
+    int has_nidx;
+    uint32_t T, layers, np;
+    float h, clad_alpha;
+};
+
+struct WalkOut {
+    float beta_len, gain_len, clad_len, refl, length;
+    uint32_t steps;
+    int status;
+};
+
+HASE_HD inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// (2k+1) * 2^-24 from the top 23 bits: exact in FP32, strictly inside (0,1)
+HASE_HD inline float u32_to_unit(uint32_t x)
+{
+    return (float)(((x >> 9) << 1) | 1u) * (1.0f / 16777216.0f);
+}
+
+HASE_HD inline double systematic_offset(uint32_t seed, uint32_t sample, uint32_t launch)
+{
+    uint32_t r[4];
+    philox4x32_10(0xFFFFFFFFu, sample, launch, kTagOffset, seed, kKeyTag, r);
+    return ((double)(r[0] >> 5) * 67108864.0 + (double)(r[1] >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+HASE_HD inline float surface_factor(const MeshDev& m, uint32_t t, uint32_t plane, float sin2)
+{
+    if (m.has_nidx) {
+        float nin = plane ? m.nin_t : m.nin_b;
+        float nout = plane ? m.nout_t : m.nout_b;
+        if (nin > 0.0f && nout > 0.0f) {
+            float ratio = nin / nout;
+            if (ratio * ratio * sin2 > 1.0f) return 1.0f;
+        }
+    }
+    if (!m.refl) return 0.0f;
+    float r = m.refl[2u * t + plane];
+    return r > 0.0f ? r : 0.0f;
+}
+
+// Prism walk from a (inside triangle t, layer `layer` of copy 0) to the
+// unfolded end point e.  Heights are given in layer units (zeta = z / h), so
+// an end point on a level plane (every sample point and its mirror images)
+// sits on an exact integer plane and the last plane crossing lands exactly
+// on the end.  Crossing parameters are measured from a; the exit edge is the
+// nearest one whose outward normal faces the ray (entry edge excluded);
+// reflections are the copy boundaries of the unfolded z axis.
+HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
+                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+{
+    WalkOut w;
+    w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
+    w.steps = 0; w.status = 0;
+    const float h = m.h;
+    const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
+    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+    w.length = D;
+    if (!(D > 0.0f)) return w;
+    const float inv = 1.0f / D;
+    const float vx = dx * inv, vy = dy * inv, vz = dz * inv;
+    const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
+    const float tol = 2e-5f * (1.0f + D);
+    const int layers = (int)m.layers;
+    int u = (int)layer;    // unfolded layer index
+    int copy = 0, r = (int)layer;
+    int entry = -1;
+    float s = 0.0f;
+    uint32_t applied = 0;
+    for (uint32_t step = 0; step < kMaxSteps; ++step) {
+        w.steps++;
+        uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
+        const float vxs[3] = {bits_to_float(q0.x), bits_to_float(q0.z), bits_to_float(q1.x)};
+        const float vys[3] = {bits_to_float(q0.y), bits_to_float(q0.w), bits_to_float(q1.y)};
+        float X[3], Y[3];
+        bool at_end[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            X[k] = vxs[k] - ax;
+            Y[k] = vys[k] - ay;
+            at_end[k] = vxs[k] == ex && vys[k] == ey;
+        }
+        uint32_t nw[3] = {q1.z, q1.w, q2.x};
+        uint32_t clad = q2.y;
+        float best = INFINITY;
+        int bestk = -1;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            int k1 = k == 2 ? 0 : k + 1;
+            float exk = X[k1] - X[k], eyk = Y[k1] - Y[k];
+            float nv = eyk * vx - exk * vy;
+            // an edge through the end point's (x, y) meets the ray exactly at the end
+            float sk = (at_end[k] || at_end[k1]) ? D : (eyk * X[k] - exk * Y[k]) / nv;
+            bool ok = (k != entry) && (nv > 0.0f) && (sk < best);
+            best = ok ? sk : best;
+            bestk = ok ? k : bestk;
+        }
+        float sz = INFINITY;
+        int dir = 0;
+        if (dzeta > 0.0f) { sz = D * (((float)(u + 1) - za) / dzeta); dir = 1; }
+        else if (dzeta < 0.0f) { sz = D * (((float)u - za) / dzeta); dir = -1; }
+        float snext = fminf(fminf(best, sz), D);
+        snext = fmaxf(snext, s);
+        float seg = snext - s;
+        int level = (copy & 1) ? (layers - 1 - r) : r;
+        if (clad) {
+            w.clad_len += seg;
+        } else {
+            w.beta_len = fmaf(m.beta[t + (uint32_t)level * m.T], seg, w.beta_len);
+            w.gain_len += seg;
+        }
+        s = snext;
+        bool done = s >= D;
+        if (!done && sz > best && nw[bestk] == 0xFFFFFFFFu) {
+            // leaving the mesh sideways: only rounding short of the end (a sample
+            // point on the rim) counts as arrival
+            if (D - s <= tol) done = true;
+            else { w.status = 1; return w; }
+        }
+        if (done) {
+            // an end point on a reflecting plane takes its last factor here
+            while (applied < nrefl) {
+                uint32_t plane = (applied & 1u) ? 1u - first : first;
+                w.refl *= surface_factor(m, t, plane, sin2);
+                applied++;
+            }
+            return w;
+        }
+        if (sz <= best) {
+            if (dir > 0) {
+                if (++r == layers) {
+                    if (applied < nrefl) {
+                        w.refl *= surface_factor(m, t, (copy & 1) ? 0u : 1u, sin2);
+                        applied++;
+                    }
+                    r = 0;
+                    copy++;
+                }
+                u++;
+            } else {
+                if (--r < 0) {
+                    if (applied < nrefl) {
+                        w.refl *= surface_factor(m, t, (copy & 1) ? 1u : 0u, sin2);
+                        applied++;
+                    }
+                    r = layers - 1;
+                    copy--;
+                }
+                u--;
+            }
+        } else {
+            uint32_t word = nw[bestk];
+            t = word >> 2;
+            entry = (int)(word & 3u);
+        }
+    }
+    w.status = 2;
+    return w;
+}
+
+// Height, in layer units, of the mirror image of a sample point on plane
+// `lv` after `nrefl` alternating reflections at z = 0 (plane 0) and
+// z = layers * h (plane 1), the first at `first`.  Integer arithmetic, so
+// the image lies exactly on a plane of the unfolded axis.
+HASE_HD inline int image_plane(int lv, int layers, uint32_t nrefl, uint32_t first)
+{
+    int mr = (int)nrefl;
+    if (nrefl == 0) return lv;
+    if (first == 1u) return (nrefl & 1u) ? mr * layers + (layers - lv) : mr * layers + lv;
+    return (nrefl & 1u) ? -(mr - 1) * layers - lv : -mr * layers + lv;
+}
+
+// Gain of a walked path averaged over the spectral samples.  The exponent is
+// linear in the walk's length sums, so one walk serves every wavelength:
+// E = nTot (se + sa) sum(beta l) - nTot sa sum(l) - alpha_clad sum(l_clad)
+// (g = nTot (beta (se + sa) - sa), include/kernels/forward/rayWalk.hpp:170-182).
+HASE_HD inline float spectral_gain(const float* c1, const float* c2, uint32_t n_lambda,
+                                               float inv_lambda, float clad_alpha, const WalkOut& w)
+{
+    const float tail = clad_alpha * w.clad_len;
+    float g = 0.0f;
+    for (uint32_t lam = 0; lam < n_lambda; ++lam)
+        g += __expf(c1[lam] * w.beta_len - c2[lam] * w.gain_len - tail);
+    return g * inv_lambda;
+}
+
+}  // namespace hase

@@ -1,0 +1,116 @@
+"""Seeded synthetic workloads: the small plain slab and the production disk.
+
+* ``plain_small`` - square slab 1 x 1 cm, 10 x 10 grid split into 200
+  triangles / 121 points, 10 levels over 0.7 cm (1800 prisms), per-prism
+  beta ~ U(0, 0.3) from numpy's default_rng(seed), one wavelength
+  (sigma_a 1.16e-21, sigma_e 2.48e-20 cm^2), nTot 2.76e20 cm^-3,
+  tau 9.5e-4 s, no reflections, no cladding, fixed rays per sample.
+* ``production_disk`` - disk of radius 3 cm, ~2k seeded points (jittered
+  hexagonal lattice plus a boundary ring) Delaunay-triangulated into ~4k
+  triangles, 20 levels over 0.7 cm, beta = 0.3 exp(-r^2 / (2 * 1.2^2)) +
+  N(0, 0.01) clipped to [0, 1] at prism centroids, 64 wavelengths of
+  Gaussian-bump spectra (peaks ~1e-21 / 2.5e-20 cm^2), per-triangle top and
+  bottom reflectivity ~U(0, 0.1), the outer ring of triangles (those with an
+  exterior edge) as cladding with absorption 5.5 cm^-1 and beta 0, adaptive
+  rays 1e5 -> 1e7 per sample with mse threshold 0.005.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .geometry import GainMedium, Grid, MeshTopology
+from .laser import CrossSectionData
+
+
+@dataclass
+class Workload:
+    name: str
+    topology: MeshTopology
+    medium: GainMedium
+    spectra: CrossSectionData
+    settings: dict = field(default_factory=dict)
+
+    @property
+    def samples(self):
+        return self.topology.numberOfSamples
+
+
+def plain_small(seed=5489, rays=100_000):
+    grid = Grid(x=1.0, y=1.0, z=0.7, tileSizeX=0.1, tileSizeZ=0.7 / 9)
+    topo = MeshTopology.fromGrid(grid)
+    rng = np.random.default_rng(seed)
+    beta = rng.uniform(0.0, 0.3, size=topo.numberOfPrisms)
+    medium = GainMedium(topo).withPhysicalProperties(
+        betaVolume=beta,
+        betaCells=np.full(topo.numberOfSamples, 0.15),
+        claddingCellTypes=np.zeros(topo.numberOfTriangles, np.uint32),
+        claddingNumber=1,
+        claddingAbsorption=0.0,
+        nTot=2.76e20,
+        crystalTFluo=9.5e-4,
+        refractiveIndices=[1.83, 1.0, 1.83, 1.0],
+    )
+    spectra = CrossSectionData.monochromatic(wavelength=1030e-9, crossSectionAbsorption=1.16e-21,
+                                             crossSectionEmission=2.48e-20)
+    settings = dict(minRaysPerSample=rays, maxRaysPerSample=rays, mseThreshold=0.005, repetitions=1,
+                    adaptiveSteps=1, useReflections=False, monochromatic=True, rngSeed=seed)
+    return Workload("plain_small", topo, medium, spectra, settings)
+
+
+def disk_points(radius=3.0, n_target=2060, seed=2, ring=None):
+    """Jittered hexagonal lattice inside a disk plus a uniform boundary ring."""
+    rng = np.random.default_rng(seed)
+    area = np.pi * radius * radius
+    a = np.sqrt(2.0 * area / (np.sqrt(3.0) * n_target))
+    ring = int(round(2.0 * np.pi * radius / a)) if ring is None else int(ring)
+    nx = int(np.ceil(2 * radius / a)) + 2
+    ny = int(np.ceil(2 * radius / (a * np.sqrt(3) / 2))) + 2
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    x = (ii - nx / 2) * a + (jj % 2) * a / 2
+    y = (jj - ny / 2) * a * np.sqrt(3) / 2
+    pts = np.column_stack([x.ravel(), y.ravel()])
+    pts = pts + rng.uniform(-0.2 * a, 0.2 * a, size=pts.shape)
+    pts = pts[np.hypot(pts[:, 0], pts[:, 1]) < radius - 0.6 * a]
+    phi = (np.arange(ring) + rng.uniform(0, 1)) * 2 * np.pi / ring
+    boundary = np.column_stack([radius * np.cos(phi), radius * np.sin(phi)])
+    return np.vstack([pts, boundary])
+
+
+def gaussian_spectra(n=64):
+    lam = np.linspace(900.0, 1100.0, n)
+    sa = 1.0e-21 * np.exp(-0.5 * ((lam - 975.0) / 18.0) ** 2) + 4.0e-23
+    se = 2.5e-20 * np.exp(-0.5 * ((lam - 1030.0) / 14.0) ** 2) + 1.0e-22
+    return CrossSectionData(lam, sa, lam, se, resolution=n)
+
+
+def production_disk(seed=2, levels=20, n_points=2060, wavelengths=64, min_rays=100_000, max_rays=10_000_000,
+                    mse_threshold=0.005, radius=3.0, thickness_total=0.7, max_reflections=1):
+    pts = disk_points(radius, n_points, seed)
+    topo = MeshTopology.fromPoints(pts, numberOfLevels=levels, thickness=thickness_total / (levels - 1))
+    rng = np.random.default_rng(seed + 1000)
+    cent = topo.prismCentroids()
+    r = np.hypot(cent[:, 0], cent[:, 1])
+    beta = 0.3 * np.exp(-r * r / (2 * 1.2 ** 2)) + rng.normal(0.0, 0.01, size=r.size)
+    beta = np.clip(beta, 0.0, 1.0)
+    clad_tri = topo.boundaryTriangles()
+    beta[np.tile(clad_tri, levels - 1)] = 0.0
+    pr = np.hypot(topo.points[:, 0], topo.points[:, 1])
+    beta_cells = np.tile(0.3 * np.exp(-pr * pr / (2 * 1.2 ** 2)), levels)
+    refl = rng.uniform(0.0, 0.1, size=2 * topo.numberOfTriangles)
+    medium = GainMedium(topo).withPhysicalProperties(
+        betaVolume=beta,
+        betaCells=beta_cells,
+        claddingCellTypes=clad_tri.astype(np.uint32),
+        claddingNumber=1,
+        claddingAbsorption=5.5,
+        nTot=2.76e20,
+        crystalTFluo=9.5e-4,
+        reflectivities=refl,
+        refractiveIndices=[1.0, 1.0, 1.0, 1.0],
+    )
+    settings = dict(minRaysPerSample=min_rays, maxRaysPerSample=max_rays, mseThreshold=mse_threshold,
+                    repetitions=1, adaptiveSteps=4, useReflections=True, maxReflections=max_reflections,
+                    monochromatic=False, rngSeed=seed)
+    return Workload("production_disk", topo, medium, gaussian_spectra(wavelengths), settings)

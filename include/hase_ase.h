@@ -1,0 +1,212 @@
+/*
+ * hase_ase.h - C ABI of the MI355X importance-sampled ASE ray tracer.
+ *
+ * One entry point computes, for every sample point of an extruded
+ * triangle-prism gain medium, the amplified-spontaneous-emission flux
+ * phiAse, its convergence statistic mse, the number of rays spent and the
+ * population derivative dndtAse.  It is the C form of the legacy
+ * calcPhiAse entry whose slot is now an empty header
+ * (include/core/calcPhiAse.hpp) and whose present C++ shape is
+ * `int startSimulation<bool>(ExperimentParameters&, ComputeParameters&,
+ * Result&, HostMesh&)` (include/core/simulation.hpp:42-47, returns 0 on
+ * success at :332).
+ *
+ * Conventions (all arrays are borrowed for the duration of a call):
+ *   T  = number_of_triangles, L = number_of_levels, P = T * (L - 1) prisms,
+ *   Np = number_of_points,    S = Np * L sample points, W = wavelengths.
+ *   Prism j = t + l * T (level-major).  Sample s = p + l * Np, located at
+ *   (x_p, y_p, l * thickness).  Per-triangle edge arrays are column-major:
+ *   entry (t, k) is stored at t + k * T (pyInclude/geometry/core.py:769-773).
+ *   Edge k of triangle t runs from vertex k to vertex (k + 1) % 3.
+ *
+ * Errors: no C++ exception crosses this boundary.  Every function returns
+ * HASE_OK (0) or a negative HASE_E_* code and records a message that
+ * hase_last_error() returns (thread-local).  Precondition checks follow
+ * src/openpmd/OpenPmdParser.cpp:1408-1426 (minRays > 0, maxRays >= minRays,
+ * threshold >= 0).
+ *
+ * Threading: one GPU, one HIP stream per context.  Calls on different
+ * contexts may run concurrently; calls on one context must be serialised by
+ * the caller.
+ */
+#ifndef HASE_ASE_H
+#define HASE_ASE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HASE_ABI_VERSION 1
+
+enum hase_status {
+    HASE_OK = 0,
+    HASE_E_INVALID = -1,   /* argument or precondition violated */
+    HASE_E_DEVICE = -2,    /* HIP runtime error (no device, launch failure) */
+    HASE_E_MEMORY = -3,    /* device or host allocation failed */
+    HASE_E_INTERNAL = -4   /* unexpected internal failure */
+};
+
+/* Extruded triangle-prism mesh.  Mirrors the legacy prism fields that
+ * HostMesh still carries (include/core/mesh.hpp:532-538, 301-332) and the
+ * arrays built by pyInclude/geometry/core.py:671-774. */
+typedef struct hase_mesh {
+    uint32_t number_of_points;      /* Np: transverse points per level */
+    uint32_t number_of_triangles;   /* T */
+    uint32_t number_of_levels;      /* L >= 2 z planes */
+    double thickness;               /* distance between adjacent planes, cm */
+    const double* points;           /* [2*Np]: x[0..Np) then y[0..Np) (core.py:1022) */
+    const uint32_t* triangle_point_indices; /* [3T] column-major, counter-clockwise */
+    const int32_t* triangle_neighbors;      /* [3T] neighbour across edge k, -1 = boundary */
+    const int32_t* forbidden_edge;          /* [3T] index of the shared edge inside that neighbour */
+    const double* beta_volume;      /* [P] excited-state fraction per prism */
+    const double* beta_cells;       /* [S] excited-state fraction per sample point (dndtAse) */
+    const uint32_t* cladding_cell_types; /* [T] or NULL */
+    uint32_t cladding_number;       /* triangles whose type equals this absorb */
+    double cladding_absorption;     /* cm^-1 */
+    double n_tot;                   /* active-ion density, cm^-3 */
+    double crystal_t_fluo;          /* fluorescence lifetime tau, s */
+    const float* reflectivities;    /* [2T] or NULL: index t*2 + {0 bottom, 1 top}
+                                       (legacy index cell*2 + face%2, mesh.hpp:308) */
+    const float* refractive_indices;/* [4] or NULL: {bottom inside, bottom outside,
+                                       top inside, top outside} (mesh.hpp:312-332) */
+} hase_mesh_t;
+
+/* Mirrors ExperimentParameters (include/core/types.hpp:248-331) with the
+ * legacy mseThreshold in place of relativeStandardErrorThreshold. */
+typedef struct hase_experiment {
+    uint32_t min_rays_per_sample;
+    uint32_t max_rays_per_sample;
+    uint32_t number_of_wavelengths; /* W >= 1 */
+    const double* sigma_a;          /* [W] absorption cross sections, cm^2 */
+    const double* sigma_e;          /* [W] emission cross sections, cm^2 */
+    double max_sigma_a;             /* pair used by the importance map and dndtAse */
+    double max_sigma_e;
+    double mse_threshold;           /* stop when the relative standard error <= this */
+    int32_t use_reflections;        /* top/bottom reflections by mirrored images */
+    uint32_t max_reflections;       /* reflection order cap when use_reflections (0 -> 1) */
+    double defensive_fraction;      /* share of the proposal drawn from beta*V, in [0, 1) */
+} hase_experiment_t;
+
+/* Mirrors ComputeParameters (include/core/types.hpp:79-158). */
+typedef struct hase_compute {
+    uint32_t max_repetitions;       /* >= 1: pooled launches at the maximum ray count */
+    uint32_t adaptive_steps;        /* geometric increases from min to max rays */
+    uint32_t min_sample_range;      /* inclusive first sample computed */
+    uint32_t max_sample_range;      /* inclusive last sample computed */
+    uint32_t rng_seed;
+    int32_t device;                 /* HIP device ordinal */
+    uint32_t rays_per_thread;       /* tuning: rays a lane takes per work item (0 = auto) */
+    uint32_t block_size;            /* tuning: threads per ray work-group (0 = auto) */
+    uint32_t sample_batch;          /* tuning: samples resident at once (0 = auto) */
+} hase_compute_t;
+
+/* Per-sample outputs, caller-owned, each with S entries.  Samples outside
+ * [min_sample_range, max_sample_range] are written as zero.  Mirrors
+ * Result (include/core/types.hpp:201-246): mse takes the slot of
+ * relativeStandardError, standard_error = mse * phiAse. */
+typedef struct hase_result {
+    float* phi_ase;
+    double* mse;
+    uint32_t* total_rays;
+    double* dndt_ase;
+    double* standard_error;         /* optional, may be NULL */
+    uint32_t* dropped_rays;         /* optional, may be NULL */
+} hase_result_t;
+
+/* Counters and device timings of the last evaluation of a context. */
+typedef struct hase_stats {
+    double importance_ms;           /* summed device time of the importance-map kernel */
+    double trace_ms;                /* summed device time of the ray-trace kernel */
+    double finalize_ms;             /* summed device time of the statistics kernel */
+    double total_ms;                /* device time from first to last launch */
+    uint64_t rays;                  /* rays traced (all samples, all launches) */
+    uint64_t ray_steps;             /* prism steps walked by those rays */
+    uint64_t importance_walks;      /* centroid walks of the importance map */
+    uint64_t importance_steps;      /* prism steps of those walks */
+    uint64_t dropped_rays;
+    uint32_t trace_launches;        /* launches of the ray-trace kernel */
+    uint32_t importance_launches;
+} hase_stats_t;
+
+typedef struct hase_context hase_context_t;
+
+/* Whole evaluation in one call: upload, importance map, adaptive rays,
+ * results.  Replaces calcPhiAse / startSimulation
+ * (include/core/simulation.hpp:42-333). */
+int hase_calc_phi_ase(const hase_mesh_t* mesh,
+                      const hase_experiment_t* experiment,
+                      const hase_compute_t* compute,
+                      hase_result_t* result);
+
+/* Persistent context: keeps the mesh resident on the device between
+ * evaluations, like ForwardPhiAseContext
+ * (include/core/forwardPhiAseEvaluator.hpp:180-204). */
+int hase_context_create(const hase_mesh_t* mesh,
+                        const hase_experiment_t* experiment,
+                        const hase_compute_t* compute,
+                        hase_context_t** out_context);
+
+/* Replace beta per prism and per sample point without re-uploading the
+ * geometry (refreshDynamicMeshes, forwardPhiAseEvaluator.hpp:407-433).
+ * beta_cells may be NULL to keep the current values. */
+int hase_context_update_beta(hase_context_t* context,
+                             const double* beta_volume,
+                             const double* beta_cells);
+
+/* Run one evaluation with the context's parameters. */
+int hase_context_evaluate(hase_context_t* context, hase_result_t* result);
+
+/* Counters and device timings of the last evaluation. */
+int hase_context_stats(const hase_context_t* context, hase_stats_t* stats);
+
+void hase_context_destroy(hase_context_t* context);
+
+/* Single-ray check: walk n rays from explicit start points to explicit
+ * sample points and return, per ray, the FP32 gain exponent parts and the
+ * reflectivity product.  start/end are [3n] (x, y, z interleaved; end may be
+ * an unfolded mirror image when reflections > 0), start_prism [n],
+ * reflections [n] and first_plane [n] (0 bottom, 1 top).  Outputs are [n]:
+ * beta_length = sum(beta*l) over gain prisms, gain_length = sum(l) over
+ * gain prisms, clad_length = sum(l) over cladding prisms, reflectance =
+ * product of surface factors, steps = prisms visited, status = 0 ok.
+ * Plays the role of the legacy propagateRay (retired,
+ * src/kernels/propagateRay.cpp:9-10) seen from the host, like the
+ * reference's one-thread TraverseOneRay test kernel
+ * (tests/forwardRayTraversal.cpp:98-152). */
+int hase_context_trace_rays(hase_context_t* context,
+                            uint32_t n,
+                            const float* start,
+                            const uint32_t* start_prism,
+                            const float* end,
+                            const uint32_t* reflections,
+                            const uint32_t* first_plane,
+                            float* beta_length,
+                            float* gain_length,
+                            float* clad_length,
+                            float* reflectance,
+                            uint32_t* steps,
+                            int32_t* status);
+
+/* Importance map of one sample: the unnormalised importance of every
+ * (reflection slice, prism) pair, [R*P] with R = 1 + 2*K, K = reflection
+ * order cap.  Replaces importanceSamplingPropagation (retired, shell at
+ * include/kernels/importanceSampling.hpp:10-18). */
+int hase_context_importance(hase_context_t* context,
+                            uint32_t sample,
+                            float* importance,
+                            uint32_t capacity);
+
+/* Most recent error message of this thread ("" if none). */
+const char* hase_last_error(void);
+
+/* ABI version (HASE_ABI_VERSION) and a build description string. */
+int hase_abi_version(void);
+const char* hase_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HASE_ASE_H */

@@ -1,0 +1,98 @@
+"""The C-ABI library: it loads, exports every declared symbol, its struct
+layouts match include/hase_ase.h, and host-side validation rejects bad
+arguments before any device work (no compute calls here)."""
+import ctypes as C
+import json
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from haseongpu_amd import _native
+from haseongpu_amd.workloads import plain_small
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "hase_ase.h"
+
+
+def _declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(hase_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.load()
+    declared = _declared_functions()
+    assert len(declared) >= 10
+    assert set(declared) == set(_native.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.hase_abi_version() == 1
+    assert b"gfx950" in lib.hase_build_info()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    structs = {"hase_mesh_t": _native.Mesh, "hase_experiment_t": _native.Experiment,
+               "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
+    first = True
+    for cname, cls in structs.items():
+        for field, _ in cls._fields_:
+            sep = "" if first else ","
+            first = False
+            lines.append(f'printf("{sep}\\"{cname}.{field}\\": %zu", offsetof({cname}, {field}));')
+        lines.append(f'printf(",\\"{cname}\\": %zu", sizeof({cname}));')
+    lines += ['printf("}");', "return 0;}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    layout = json.loads(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    for cname, cls in structs.items():
+        assert layout[cname] == C.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert layout[f"{cname}.{field}"] == getattr(cls, field).offset, f"{cname}.{field}"
+
+
+def _args(**overrides):
+    w = plain_small(rays=1000)
+    mesh = _native.NativeMesh(w.topology, w.medium)
+    exp = _native.NativeExperiment(sigma_a=[1.16e-21], sigma_e=[2.48e-20], min_rays=1000, max_rays=1000)
+    comp = _native.make_compute(n_samples=w.samples)
+    for key, value in overrides.items():
+        for obj in (mesh.struct, exp.struct, comp):
+            if any(f == key for f, _ in obj._fields_):
+                setattr(obj, key, value)
+    return w, mesh, exp, comp
+
+
+@pytest.mark.parametrize("overrides,message", [
+    ({"min_rays_per_sample": 0}, "min_rays_per_sample"),
+    ({"max_rays_per_sample": 10}, "max_rays_per_sample"),
+    ({"number_of_levels": 1}, "number_of_levels"),
+    ({"mse_threshold": -1.0}, "mse_threshold"),
+    ({"max_sample_range": 10**6}, "sample range"),
+    ({"defensive_fraction": 1.0}, "defensive_fraction"),
+    ({"max_repetitions": 0}, "max_repetitions"),
+    ({"n_tot": 0.0}, "n_tot"),
+])
+def test_invalid_arguments_are_rejected_on_the_host(overrides, message):
+    lib = _native.load()
+    w, mesh, exp, comp = _args(**overrides)
+    res = _native.ResultArrays(w.samples)
+    code = lib.hase_calc_phi_ase(C.byref(mesh.struct), C.byref(exp.struct), C.byref(comp), C.byref(res.struct))
+    assert code == -1
+    assert message in lib.hase_last_error().decode()
+
+
+def test_null_arguments_are_rejected():
+    lib = _native.load()
+    assert lib.hase_calc_phi_ase(None, None, None, None) == -1
+    assert b"NULL" in lib.hase_last_error()
+    assert lib.hase_context_evaluate(None, None) == -1
+    assert lib.hase_context_stats(None, None) == -1
+    lib.hase_context_destroy(None)
+    with pytest.raises(_native.HaseError):
+        _native.check(-1)

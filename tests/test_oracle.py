@@ -1,0 +1,302 @@
+"""CPU checks that pin the FP64 oracle before it is trusted as the checker.
+
+Each function of the oracle is checked against something it does not share
+code with: published known-answer vectors (Philox), brute force (segment /
+prism overlap lengths, reflection counting), closed forms (angular
+quadrature of the point flux), the reference's own edge cases (relative
+standard error, adaptive schedule) and the legacy golden fixtures.
+"""
+import numpy as np
+import pytest
+
+from haseongpu_amd.geometry import GainMedium, Grid, MeshTopology
+from haseongpu_amd.laser import CrossSectionData
+from oracle import oracle as O
+from tests.helpers import load_compare_serial, load_true_cladding
+from tests.quadrature import box_point_flux
+
+
+# ---------------------------------------------------------------- Philox
+def test_philox_known_answer_vectors():
+    """Random123 kat_vectors for philox4x32 with 10 rounds (Salmon et al., SC'11)."""
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, expected in kat:
+        assert tuple(int(v) for v in O.philox4x32_10(ctr, key)) == expected
+
+
+def test_unit_conversion_is_open_and_exact():
+    for x in (0, 1, 511, 512, 0x7FFFFFFF, 0xFFFFFFFF):
+        u = O.u32_to_unit(x)
+        assert 0.0 < u < 1.0
+        assert float(np.float32(u)) == u      # exactly representable in FP32
+    assert O.u32_to_unit(0) == 2.0 ** -24
+    assert O.u32_to_unit(0xFFFFFFFF) == 1.0 - 2.0 ** -24
+    # uniformity of the stream used for start points
+    vals = np.array([O.u32_to_unit(int(O.philox4x32_10((i, 3, 0, 0), (5489, 0x48415345))[0]))
+                     for i in range(20000)])
+    assert abs(vals.mean() - 0.5) < 0.01 and abs(vals.var() - 1 / 12) < 0.005
+
+
+# ------------------------------------------------------- walk brute force
+def _small_mesh(seed=0, levels=4, clad=True, refl=False):
+    topo = MeshTopology.fromGrid(Grid(x=0.6, y=0.5, z=0.3, tileSizeX=0.2, tileSizeY=0.25, tileSizeZ=0.1))
+    rng = np.random.default_rng(seed)
+    med = GainMedium(topo).withPhysicalProperties(
+        betaVolume=rng.uniform(0, 0.4, topo.numberOfPrisms),
+        claddingCellTypes=(rng.uniform(size=topo.numberOfTriangles) < 0.3).astype(np.uint32) if clad
+        else np.zeros(topo.numberOfTriangles, np.uint32),
+        claddingNumber=1, claddingAbsorption=3.0, nTot=1e20, crystalTFluo=1e-3,
+        reflectivities=rng.uniform(0.1, 0.9, 2 * topo.numberOfTriangles) if refl else
+        np.zeros(2 * topo.numberOfTriangles),
+        refractiveIndices=[1.5, 1.0, 1.5, 1.0] if refl else [1.0, 1.0, 1.0, 1.0])
+    return topo, med
+
+
+def _locate(topo, x, y):
+    """Triangle containing (x, y) by barycentric sign tests (brute force)."""
+    P, tri = topo.points, topo.trianglePointIndices.astype(np.int64)
+    a, b, c = P[tri[:, 0]], P[tri[:, 1]], P[tri[:, 2]]
+
+    def side(p, q):
+        return (q[:, 0] - p[:, 0])[:, None] * (y[None, :] - p[:, 1][:, None]) - \
+               (q[:, 1] - p[:, 1])[:, None] * (x[None, :] - p[:, 0][:, None])
+
+    inside = (side(a, b) >= -1e-15) & (side(b, c) >= -1e-15) & (side(c, a) >= -1e-15)
+    return np.argmax(inside, axis=0)
+
+
+def _brute_force(topo, med, a, e, n=400000):
+    """Length sums of a straight unfolded segment by dense sampling; heights
+    are folded back into the slab, mirror copies alternate orientation."""
+    T, layers, h = topo.numberOfTriangles, int(topo.levels) - 1, float(topo.thickness)
+    s = (np.arange(n) + 0.5) / n
+    pts = a[None, :] + s[:, None] * (e - a)[None, :]
+    D = np.linalg.norm(e - a)
+    tri = _locate(topo, pts[:, 0], pts[:, 1])
+    u = np.floor(pts[:, 2] / h).astype(np.int64)
+    copy = np.floor_divide(u, layers)
+    r = u - copy * layers
+    level = np.where(copy % 2 == 1, layers - 1 - r, r)
+    beta = med.value("betaVolume")[tri + level * T]
+    clad = med.value("claddingCellTypes")[tri] == med.value("claddingNumber")
+    ds = D / n
+    return {"beta_len": float(np.sum(np.where(clad, 0.0, beta)) * ds),
+            "gain_len": float(np.sum(~clad) * ds), "clad_len": float(np.sum(clad) * ds),
+            "copies": copy, "tri": tri}
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_walk_lengths_match_brute_force(seed):
+    topo, med = _small_mesh(seed)
+    om = O.OracleMesh(topo, med)
+    rng = np.random.default_rng(100 + seed)
+    sp = topo.samplePoints()
+    for _ in range(12):
+        j = int(rng.integers(topo.numberOfPrisms))
+        a = O.ray_start(om, 7, 0, 0, int(rng.integers(1 << 20)), j)
+        e = sp[int(rng.integers(len(sp)))]
+        w = O.walk_ray(om, a, j, e)
+        assert w["status"] == 0
+        bf = _brute_force(topo, med, a, e)
+        D = np.linalg.norm(e - a)
+        for key in ("beta_len", "gain_len", "clad_len"):
+            assert w[key] == pytest.approx(bf[key], abs=2e-4 * D)
+        assert w["gain_len"] + w["clad_len"] == pytest.approx(D, rel=1e-12)
+
+
+@pytest.mark.parametrize("first", [0, 1])
+@pytest.mark.parametrize("reflections", [1, 2, 3])
+def test_reflected_walk_matches_brute_force(reflections, first):
+    topo, med = _small_mesh(4, refl=True)
+    om = O.OracleMesh(topo, med)
+    rng = np.random.default_rng(reflections * 10 + first)
+    sp = topo.samplePoints()
+    refl = med.value("reflectivities").astype(np.float64)
+    nidx = med.value("refractiveIndices").astype(np.float64)
+    for _ in range(8):
+        j = int(rng.integers(topo.numberOfPrisms))
+        a = O.ray_start(om, 9, 0, 0, int(rng.integers(1 << 20)), j)
+        p = sp[int(rng.integers(len(sp)))]
+        e = O.image_point(om, p, reflections, first)
+        w = O.walk_ray(om, a, j, e, reflections, first)
+        assert w["status"] == 0
+        bf = _brute_force(topo, med, a, e)
+        D = np.linalg.norm(e - a)
+        for key in ("beta_len", "gain_len", "clad_len"):
+            assert w[key] == pytest.approx(bf[key], abs=2e-4 * D)
+        # surface factors: one per copy boundary crossed, at the triangle there
+        copies = bf["copies"]
+        change = np.flatnonzero(np.diff(copies) != 0)
+        crossings = [(int(copies[i]), int(copies[i + 1]), int(bf["tri"][i])) for i in change]
+        if len(crossings) < reflections:   # end point on the last plane
+            crossings.append((int(copies[-1]), int(copies[-1]) + (1 if e[2] > a[2] else -1), int(bf["tri"][-1])))
+        assert len(crossings) == reflections
+        vz = (e[2] - a[2]) / D
+        sin2 = 1.0 - vz * vz
+        expect = 1.0
+        for c0, c1, t in crossings:
+            plane = (0 if c0 % 2 else 1) if c1 > c0 else (1 if c0 % 2 else 0)
+            tir = (nidx[2 * plane] / nidx[2 * plane + 1]) ** 2 * sin2 > 1.0
+            expect *= 1.0 if tir else refl[2 * t + plane]
+        assert w["refl"] == pytest.approx(expect, rel=1e-9)
+
+
+def test_image_points():
+    topo, med = _small_mesh()
+    om = O.OracleMesh(topo, med)
+    Z = (int(topo.levels) - 1) * topo.thickness
+    p = np.array([0.1, 0.2, 0.1])
+    assert O.image_point(om, p, 0, 0)[2] == pytest.approx(0.1)
+    assert O.image_point(om, p, 1, 1)[2] == pytest.approx(2 * Z - 0.1)   # mirror in the top plane
+    assert O.image_point(om, p, 1, 0)[2] == pytest.approx(-0.1)          # mirror in the bottom plane
+    assert O.image_point(om, p, 2, 1)[2] == pytest.approx(2 * Z + 0.1)
+    assert O.image_point(om, p, 2, 0)[2] == pytest.approx(-2 * Z + 0.1)
+
+
+# ------------------------------------------------- closed-form estimator
+def _box_case(beta, sigma_a, sigma_e, *, reflect=False, refl=0.0, nin=1.0, tile=0.125):
+    topo = MeshTopology.fromGrid(Grid(x=1.0, y=0.75, z=0.5, tileSizeX=tile, tileSizeZ=0.125))
+    med = GainMedium(topo).withPhysicalProperties(
+        betaVolume=np.full(topo.numberOfPrisms, beta), betaCells=np.full(topo.numberOfSamples, beta),
+        nTot=2e20, crystalTFluo=1e-3, reflectivities=np.full(2 * topo.numberOfTriangles, refl),
+        refractiveIndices=[nin, 1.0, nin, 1.0])
+    return topo, med
+
+
+@pytest.mark.parametrize("sigma_e,label", [(0.5e-21 / 0.5 - 1e-21, "zero gain"), (2.0e-20, "gain"),
+                                           (0.0, "absorption")])
+def test_estimator_matches_angular_quadrature(sigma_e, label):
+    beta, sa = 0.5, 1e-21
+    topo, med = _box_case(beta, sa, sigma_e)
+    g = 2e20 * (beta * (sigma_e + sa) - sa)
+    om = O.OracleMesh(topo, med)
+    samples = [0 + 2 * topo.numberOfPoints + 12, 40, topo.numberOfSamples - 1, 3 * topo.numberOfPoints + 30]
+    sp = topo.samplePoints()
+    for s in samples:
+        op = O.OracleParams(sigma_a=sa, sigma_e=sigma_e, n_tot=2e20, tau=1e-3, beta_cells=med.value("betaCells"),
+                            min_rays=300000, sample_min=s, sample_max=s, seed=11)
+        r = O.calc_phi_ase(om, op)
+        exact = box_point_flux(sp[s], (0.0, 1.0, 0.0, 0.75), 0.5, beta, g, 2e20 / 1e-3)
+        rse = np.sqrt(r["mse"][s])
+        assert r["phiAse"][s] == pytest.approx(exact, rel=max(5 * rse, 2e-3)), (label, s)
+
+
+@pytest.mark.parametrize("nin", [1.0, 1.8])
+def test_estimator_reflections_match_image_quadrature(nin):
+    beta, sa, se = 0.4, 1e-21, 1.2e-20
+    topo, med = _box_case(beta, sa, se, reflect=True, refl=0.3, nin=nin)
+    g = 2e20 * (beta * (se + sa) - sa)
+    om = O.OracleMesh(topo, med)
+    sp = topo.samplePoints()
+    for s in [5 + 2 * topo.numberOfPoints, topo.numberOfPoints + 20, 4 * topo.numberOfPoints + 33]:
+        op = O.OracleParams(sigma_a=sa, sigma_e=se, n_tot=2e20, tau=1e-3, beta_cells=med.value("betaCells"),
+                            min_rays=300000, sample_min=s, sample_max=s, seed=13, use_reflections=True,
+                            max_reflections=1)
+        r = O.calc_phi_ase(om, op)
+        exact = box_point_flux(sp[s], (0.0, 1.0, 0.0, 0.75), 0.5, beta, g, 2e20 / 1e-3, reflect=True,
+                               refl_bottom=0.3, refl_top=0.3, n_in=nin, n_out=1.0)
+        direct = box_point_flux(sp[s], (0.0, 1.0, 0.0, 0.75), 0.5, beta, g, 2e20 / 1e-3)
+        assert exact > direct * 1.05          # the reflected part is not negligible
+        rse = np.sqrt(r["mse"][s])
+        assert r["phiAse"][s] == pytest.approx(exact, rel=max(5 * rse, 3e-3))
+
+
+def test_estimator_is_unbiased_for_any_defensive_fraction():
+    topo, med = _box_case(0.3, 1e-21, 2e-20)
+    om = O.OracleMesh(topo, med)
+    s = 2 * topo.numberOfPoints + 17
+    vals = []
+    for eps in (0.0, 0.05, 0.5, 0.95):
+        op = O.OracleParams(sigma_a=1e-21, sigma_e=2e-20, n_tot=2e20, tau=1e-3, beta_cells=med.value("betaCells"),
+                            min_rays=200000, sample_min=s, sample_max=s, seed=17, defensive=eps)
+        r = O.calc_phi_ase(om, op)
+        vals.append((r["phiAse"][s], np.sqrt(r["mse"][s]) * r["phiAse"][s]))
+    ref = np.mean([v for v, _ in vals])
+    for v, se in vals:
+        assert abs(v - ref) < 5 * se + 2e-3 * ref
+
+
+# ------------------------------------------------ statistics and schedule
+def test_relative_standard_error_edge_cases():
+    """tests/forwardMse.cpp:104-139."""
+    assert O.relative_standard_error(4.0, 10.0, 4.0) == pytest.approx(np.sqrt((4 * 10 / 16 - 1) / 4))
+    assert O.relative_standard_error(1.0, 1.0, 1.0) == np.finfo(np.float64).max
+    assert np.isnan(O.relative_standard_error(0.0, 0.0, 2.0))
+    assert O.relative_standard_error(np.inf, 1.0, 2.0) == np.finfo(np.float64).max
+
+
+def test_adaptive_ray_target():
+    """include/core/types.hpp:333-354: geometric growth from min to max."""
+    assert [O.adaptive_target(1000, 1000000, 3, k) for k in range(5)] == [1000, 10000, 100000, 1000000, 1000000]
+    assert O.adaptive_target(1000, 1000, 4, 2) == 1000
+    assert O.adaptive_target(1000, 5000, 0, 3) == 1000
+    assert O.adaptive_target(100, 1000, 2, 1) == 317   # ceil(100 * sqrt(10))
+
+
+def test_adaptive_loop_pools_launches():
+    topo, med = _box_case(0.3, 1e-21, 2e-20)
+    om = O.OracleMesh(topo, med)
+    op = O.OracleParams(sigma_a=1e-21, sigma_e=2e-20, n_tot=2e20, tau=1e-3, beta_cells=med.value("betaCells"),
+                        min_rays=1000, max_rays=16000, adaptive_steps=4, max_repetitions=3, mse_threshold=1e-4,
+                        seed=3, sample_max=60)
+    r = O.calc_phi_ase(om, op)
+    rays = r["totalRays"][:61]
+    assert set(np.unique(rays)) <= {1000, 2000, 4000, 8000, 16000, 32000, 48000}
+    assert np.all((r["mse"][:61] <= 1e-4) | (rays == 48000))
+    # a sample that stopped early met the threshold
+    assert np.all(r["mse"][:61][rays < 48000] <= 1e-4)
+
+
+# -------------------------------------------------------------- fixtures
+@pytest.mark.parametrize("name", ["cuboid", "cylindrical"])
+def test_oracle_reproduces_legacy_serial_solver(name):
+    """compareSerial: legacy serial solver, 1e6 rays per sample."""
+    topo, med, sp, ref = load_compare_serial(name)
+    om = O.OracleMesh(topo, med)
+    idx = np.arange(0, topo.numberOfSamples, 13)
+    op = O.OracleParams(sigma_a=sp["sigmaA"], sigma_e=sp["sigmaE"], n_tot=float(med.value("nTot")),
+                        tau=float(med.value("crystalTFluo")), beta_cells=med.value("betaCells"), min_rays=20000,
+                        seed=5489)
+    # subset of samples through the sample range, one call per contiguous block
+    phi = []
+    for s in idx:
+        op.s.sample_min = int(s)
+        op.s.sample_max = int(s)
+        phi.append(O.calc_phi_ase(om, op)["phiAse"][s])
+    ratio = np.asarray(phi) / ref["phiASE"][idx]
+    assert abs(ratio.mean() - 1.0) < 0.01
+    assert abs(np.median(ratio) - 1.0) < 0.01
+    # dndtAse = (beta (sigma_e + sigma_a) - sigma_a) * phiASE holds exactly in the fixture
+    gain = med.value("betaCells") * (sp["sigmaE"] + sp["sigmaA"]) - sp["sigmaA"]
+    np.testing.assert_allclose(ref["dndtAse"], gain * ref["phiASE"], rtol=1e-12)
+
+
+def test_oracle_matches_true_cladding_fixture_scale():
+    """Legacy importance-sampled phiASE/mse at 2000 rays/sample, cladding
+    absorption 5.5 cm^-1: the 2000-ray oracle sits on the same level, and the
+    legacy mse has the scale of the squared relative standard error (the exact
+    legacy mse formula is not in the reference and stays unpinned)."""
+    topo, med, sp, ref = load_true_cladding(1)
+    om = O.OracleMesh(topo, med)
+    op = O.OracleParams(sigma_a=sp["sigmaA"], sigma_e=sp["sigmaE"], n_tot=float(med.value("nTot")),
+                        tau=float(med.value("crystalTFluo")), beta_cells=med.value("betaCells"),
+                        min_rays=ref["raysPerSample"], seed=ref["rngSeed"], mse_threshold=1.0)
+    r = O.calc_phi_ase(om, op)
+    assert np.all(r["totalRays"] == ref["totalRays"])
+    assert abs(np.median(r["phiAse"] / ref["phiASE"]) - 1.0) < 0.01
+    assert abs(r["phiAse"].mean() / ref["phiASE"].mean() - 1.0) < 0.02
+    ratio = np.median(r["mse"]) / np.median(ref["mse"])
+    assert 1 / 3 < ratio < 3
+
+
+def test_cladding_absorption_lowers_flux_monotonically():
+    means = []
+    for idx in range(3):
+        topo, med, sp, ref = load_true_cladding(idx)
+        means.append(ref["phiASE"].mean())
+    assert means[0] > means[1] > means[2]
