@@ -450,8 +450,10 @@ struct hase_context {
     double clad_alpha = 0.0;
     bool has_refl = false, has_nidx = false;
     float nidx[4] = {1, 1, 1, 1};
-    std::vector<double> tri_area;      // [T]
-    std::vector<uint32_t> clad_flag;   // [T]
+    std::vector<double> tri_area;      // [T] device order
+    std::vector<uint32_t> clad_flag;   // [T] device order
+    std::vector<uint32_t> dev2user;    // device triangle -> caller triangle
+    std::vector<uint32_t> user2dev;    // caller triangle -> device triangle
     std::vector<double> beta_host;     // [P]
     std::vector<double> beta_cells;    // [S]
     double Bsum = 0.0;
@@ -536,7 +538,8 @@ void upload_beta(hase_context* ctx, const double* beta_volume, const double* bet
     std::vector<float> b(P), bv(P);
     double B = 0.0;
     for (uint32_t j = 0; j < P; ++j) {
-        double v = beta_volume[j];
+        // device prism j = t_dev + l * T reads the caller's prism dev2user[t_dev] + l * T
+        double v = beta_volume[ctx->dev2user[j % T] + (j / T) * (size_t)T];
         if (!std::isfinite(v)) throw HaseError(HASE_E_INVALID, "beta_volume must be finite");
         b[j] = (float)v;
         double w = v > 0.0 ? v * ctx->tri_area[j % T] * ctx->h : 0.0;
@@ -554,6 +557,58 @@ void upload_beta(hase_context* ctx, const double* beta_volume, const double* bet
         ctx->bcells.upload(ctx->beta_cells.data(), ctx->S, ctx->stream);
     }
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+// Index of (x, y) on a 2^16 x 2^16 Hilbert curve.
+uint64_t hilbert_index(uint32_t x, uint32_t y)
+{
+    const uint32_t n = 1u << 16;
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// Device triangle order: triangles sorted by the Hilbert index of their
+// centroid (quantised to 16 bits per axis over the centroids' bounding box),
+// ties by caller index.  Lanes that take consecutive device triangles then
+// walk from neighbouring prisms, which keeps their records in the same cache
+// lines and their step counts alike.  The proposal prefix runs in this order.
+std::vector<uint32_t> triangle_order(const hase_mesh_t* mesh)
+{
+    const uint32_t T = mesh->number_of_triangles, np = mesh->number_of_points;
+    const double* px = mesh->points;
+    const double* py = mesh->points + np;
+    std::vector<double> cx(T), cy(T);
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (uint32_t t = 0; t < T; ++t) {
+        uint32_t a = mesh->triangle_point_indices[t], b = mesh->triangle_point_indices[t + (size_t)T],
+                 c = mesh->triangle_point_indices[t + 2 * (size_t)T];
+        cx[t] = (px[a] + px[b] + px[c]) / 3.0;
+        cy[t] = (py[a] + py[b] + py[c]) / 3.0;
+        x0 = std::min(x0, cx[t]); x1 = std::max(x1, cx[t]);
+        y0 = std::min(y0, cy[t]); y1 = std::max(y1, cy[t]);
+    }
+    auto quant = [](double v, double lo, double hi) -> uint32_t {
+        if (!(hi > lo)) return 0u;
+        double q = std::floor((v - lo) / (hi - lo) * 65535.0);
+        return (uint32_t)std::min(65535.0, std::max(0.0, q));
+    };
+    std::vector<uint64_t> key(T);
+    for (uint32_t t = 0; t < T; ++t) key[t] = hilbert_index(quant(cx[t], x0, x1), quant(cy[t], y0, y1));
+    std::vector<uint32_t> order(T);
+    for (uint32_t t = 0; t < T; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    return order;
 }
 
 std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase_experiment_t* e,
@@ -576,22 +631,29 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     ctx->clad_alpha = mesh->cladding_absorption;
     const double* px = mesh->points;
     const double* py = mesh->points + np;
-    // triangle records: {x0,y0,x1,y1} {x2,y2,n0,n1} {n2,clad,0,0}; n_k = neighbour<<2 | shared edge
+    for (size_t i = 0; i < 3u * (size_t)T; ++i)
+        if (mesh->triangle_point_indices[i] >= np)
+            throw HaseError(HASE_E_INVALID, "triangle point index out of range");
+    ctx->dev2user = triangle_order(mesh);
+    ctx->user2dev.assign(T, 0u);
+    for (uint32_t t = 0; t < T; ++t) ctx->user2dev[ctx->dev2user[t]] = t;
+    // triangle records in device order: {x0,y0,x1,y1} {x2,y2,n0,n1} {n2,clad,0,0};
+    // n_k = device neighbour << 2 | shared edge index inside the neighbour
     std::vector<uint4> rec(3u * (size_t)T);
     ctx->tri_area.resize(T);
     ctx->clad_flag.resize(T);
-    for (uint32_t t = 0; t < T; ++t) {
+    for (uint32_t td = 0; td < T; ++td) {
+        const uint32_t t = ctx->dev2user[td];
         uint32_t v[3];
         uint32_t nw[3];
         for (int k = 0; k < 3; ++k) {
             v[k] = mesh->triangle_point_indices[t + (size_t)k * T];
-            if (v[k] >= np) throw HaseError(HASE_E_INVALID, "triangle point index out of range");
             int32_t nb = mesh->triangle_neighbors[t + (size_t)k * T];
             int32_t fe = mesh->forbidden_edge[t + (size_t)k * T];
             if (nb >= 0) {
                 if ((uint32_t)nb >= T || fe < 0 || fe > 2)
                     throw HaseError(HASE_E_INVALID, "triangle_neighbors / forbidden_edge out of range");
-                nw[k] = ((uint32_t)nb << 2) | (uint32_t)fe;
+                nw[k] = (ctx->user2dev[(uint32_t)nb] << 2) | (uint32_t)fe;
             } else {
                 nw[k] = 0xFFFFFFFFu;
             }
@@ -599,13 +661,13 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         double cr = (px[v[1]] - px[v[0]]) * (py[v[2]] - py[v[0]]) - (py[v[1]] - py[v[0]]) * (px[v[2]] - px[v[0]]);
         if (!(cr > 0.0))
             throw HaseError(HASE_E_INVALID, "triangles must be non-degenerate and counter-clockwise");
-        ctx->tri_area[t] = 0.5 * cr;
+        ctx->tri_area[td] = 0.5 * cr;
         uint32_t clad = mesh->cladding_cell_types && mesh->cladding_cell_types[t] == mesh->cladding_number ? 1u : 0u;
-        ctx->clad_flag[t] = clad;
+        ctx->clad_flag[td] = clad;
         auto fb = [](double x) { float f = (float)x; uint32_t u; std::memcpy(&u, &f, 4); return u; };
-        rec[3u * t] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
-        rec[3u * t + 1u] = make_uint4(fb(px[v[2]]), fb(py[v[2]]), nw[0], nw[1]);
-        rec[3u * t + 2u] = make_uint4(nw[2], clad, 0u, 0u);
+        rec[3u * td] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
+        rec[3u * td + 1u] = make_uint4(fb(px[v[2]]), fb(py[v[2]]), nw[0], nw[1]);
+        rec[3u * td + 2u] = make_uint4(nw[2], clad, 0u, 0u);
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
     std::vector<float2> pts(np);
@@ -613,7 +675,12 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     ctx->pts.upload(pts.data(), np, ctx->stream);
     if (mesh->reflectivities) {
         ctx->has_refl = true;
-        ctx->refl.upload(mesh->reflectivities, 2u * (size_t)T, ctx->stream);
+        std::vector<float> refl(2u * (size_t)T);
+        for (uint32_t td = 0; td < T; ++td) {
+            refl[2u * td] = mesh->reflectivities[2u * (size_t)ctx->dev2user[td]];
+            refl[2u * td + 1u] = mesh->reflectivities[2u * (size_t)ctx->dev2user[td] + 1u];
+        }
+        ctx->refl.upload(refl.data(), refl.size(), ctx->stream);
     }
     if (mesh->refractive_indices) {
         ctx->has_nidx = true;
@@ -977,12 +1044,17 @@ int hase_context_trace_rays(hase_context_t* context, uint32_t n, const float* st
             double ze = (double)end[3 * i + 2] / context->h, zr = std::nearbyint(ze);
             he[3 * i + 2] = (float)(std::fabs(ze - zr) < 1e-5 ? zr : ze);
         }
+        std::vector<uint32_t> dev_prism(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t j = start_prism[i];
+            dev_prism[i] = context->user2dev[j % context->T] + (j / context->T) * context->T;
+        }
         DevBuf<float> d_start, d_end, d_bl, d_gl, d_cl, d_rf;
         DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps;
         DevBuf<int32_t> d_status;
         d_start.upload(hs.data(), 3u * (size_t)n, st);
         d_end.upload(he.data(), 3u * (size_t)n, st);
-        d_prism.upload(start_prism, n, st);
+        d_prism.upload(dev_prism.data(), n, st);
         d_nr.upload(reflections, n, st);
         d_fp.upload(first_plane, n, st);
         d_bl.alloc(n); d_gl.alloc(n); d_cl.alloc(n); d_rf.alloc(n); d_steps.alloc(n); d_status.alloc(n);
@@ -1035,9 +1107,15 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
         ip.counters = context->counters.p;
         hipLaunchKernelGGL(hase_importance_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(importance, context->imp.p, n_ent * sizeof(float), hipMemcpyDeviceToHost,
+        std::vector<float> dev(n_ent);
+        HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, n_ent * sizeof(float), hipMemcpyDeviceToHost,
                                  context->stream));
         HIP_CHECK(hipStreamSynchronize(context->stream));
+        const uint32_t T = context->T, P = context->P;
+        for (size_t e = 0; e < n_ent; ++e) {
+            size_t k = e / P, j = e % P;
+            importance[k * P + context->dev2user[j % T] + (j / T) * T] = dev[e];
+        }
     });
 }
 

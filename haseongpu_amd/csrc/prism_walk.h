@@ -84,13 +84,24 @@ HASE_HD inline float surface_factor(const MeshDev& m, uint32_t t, uint32_t plane
     return r > 0.0f ? r : 0.0f;
 }
 
+// 1/x with the hardware reciprocal (1 ulp) on the device.
+HASE_HD inline float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // Prism walk from a (inside triangle t, layer `layer` of copy 0) to the
 // unfolded end point e.  Heights are given in layer units (zeta = z / h), so
 // an end point on a level plane (every sample point and its mirror images)
-// sits on an exact integer plane and the last plane crossing lands exactly
-// on the end.  Crossing parameters are measured from a; the exit edge is the
-// nearest one whose outward normal faces the ray (entry edge excluded);
-// reflections are the copy boundaries of the unfolded z axis.
+// sits on an exact integer plane, and the crossing of that plane is the end
+// of the walk.  Crossing parameters are measured from a; the exit edge is
+// the nearest one whose outward normal faces the ray (entry edge excluded);
+// an exit edge with an end point at the end's (x, y) meets the ray exactly
+// at the end; reflections are the copy boundaries of the unfolded z axis.
 HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
                                 float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
 {
@@ -106,6 +117,11 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
     const float vx = dx * inv, vy = dy * inv, vz = dz * inv;
     const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
     const float tol = 2e-5f * (1.0f + D);
+    // parameter per layer unit of height, and the end's plane index
+    const float kz = dzeta != 0.0f ? D / dzeta : 0.0f;
+    const int zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
+    const int end_plane = (int)floorf(ze + 0.5f);
+    const bool end_on_plane = (float)end_plane == ze;
     const int layers = (int)m.layers;
     int u = (int)layer;    // unfolded layer index
     int copy = 0, r = (int)layer;
@@ -114,49 +130,55 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
     uint32_t applied = 0;
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
         w.steps++;
-        uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
-        const float vxs[3] = {bits_to_float(q0.x), bits_to_float(q0.z), bits_to_float(q1.x)};
-        const float vys[3] = {bits_to_float(q0.y), bits_to_float(q0.w), bits_to_float(q1.y)};
-        float X[3], Y[3];
-        bool at_end[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            X[k] = vxs[k] - ax;
-            Y[k] = vys[k] - ay;
-            at_end[k] = vxs[k] == ex && vys[k] == ey;
+        const int level = (copy & 1) ? (layers - 1 - r) : r;
+        // the triangle record and this prism's beta are independent loads
+        const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
+        const float beta = m.beta[t + (uint32_t)level * m.T];
+        const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
+        const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
+        const float X2 = bits_to_float(q1.x) - ax, Y2 = bits_to_float(q1.y) - ay;
+        // edge k joins vertex k and k+1; crossing parameter num / nv for nv > 0
+        const float ex0 = X1 - X0, ey0 = Y1 - Y0;
+        const float ex1 = X2 - X1, ey1 = Y2 - Y1;
+        const float ex2 = X0 - X2, ey2 = Y0 - Y2;
+        const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
+        float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
+        float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
+        float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
+        s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
+        s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
+        s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
+        int bestk = 0;
+        float best = s0;
+        if (s1 < best) { best = s1; bestk = 1; }
+        if (s2 < best) { best = s2; bestk = 2; }
+        // the winning edge through the end's (x, y) meets the ray exactly there;
+        // compared on the stored coordinates, which no contraction of the
+        // relative ones can perturb
+        {
+            const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
+            const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
+            const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
+            const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
+            const bool touches = (bits_to_float(axb) == ex && bits_to_float(ayb) == ey) ||
+                                 (bits_to_float(bxb) == ex && bits_to_float(byb) == ey);
+            best = (touches && best < INFINITY) ? D : best;
         }
-        uint32_t nw[3] = {q1.z, q1.w, q2.x};
-        uint32_t clad = q2.y;
-        float best = INFINITY;
-        int bestk = -1;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            int k1 = k == 2 ? 0 : k + 1;
-            float exk = X[k1] - X[k], eyk = Y[k1] - Y[k];
-            float nv = eyk * vx - exk * vy;
-            // an edge through the end point's (x, y) meets the ray exactly at the end
-            float sk = (at_end[k] || at_end[k1]) ? D : (eyk * X[k] - exk * Y[k]) / nv;
-            bool ok = (k != entry) && (nv > 0.0f) && (sk < best);
-            best = ok ? sk : best;
-            bestk = ok ? k : bestk;
-        }
-        float sz = INFINITY;
-        int dir = 0;
-        if (dzeta > 0.0f) { sz = D * (((float)(u + 1) - za) / dzeta); dir = 1; }
-        else if (dzeta < 0.0f) { sz = D * (((float)u - za) / dzeta); dir = -1; }
+        // next level plane along the ray (exactly D for the end's plane)
+        const int plane = u + (zdir > 0 ? 1 : 0);
+        float sz = zdir != 0 ? ((float)plane - za) * kz : INFINITY;
+        sz = (end_on_plane && plane == end_plane && zdir != 0) ? D : sz;
         float snext = fminf(fminf(best, sz), D);
         snext = fmaxf(snext, s);
-        float seg = snext - s;
-        int level = (copy & 1) ? (layers - 1 - r) : r;
-        if (clad) {
-            w.clad_len += seg;
-        } else {
-            w.beta_len = fmaf(m.beta[t + (uint32_t)level * m.T], seg, w.beta_len);
-            w.gain_len += seg;
-        }
+        const float seg = snext - s;
+        const uint32_t clad = q2.y;
+        w.clad_len += clad ? seg : 0.0f;
+        w.gain_len += clad ? 0.0f : seg;
+        w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
         s = snext;
+        const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
         bool done = s >= D;
-        if (!done && sz > best && nw[bestk] == 0xFFFFFFFFu) {
+        if (!done && sz > best && nw == 0xFFFFFFFFu) {
             // leaving the mesh sideways: only rounding short of the end (a sample
             // point on the rim) counts as arrival
             if (D - s <= tol) done = true;
@@ -165,14 +187,14 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
         if (done) {
             // an end point on a reflecting plane takes its last factor here
             while (applied < nrefl) {
-                uint32_t plane = (applied & 1u) ? 1u - first : first;
-                w.refl *= surface_factor(m, t, plane, sin2);
+                uint32_t pl = (applied & 1u) ? 1u - first : first;
+                w.refl *= surface_factor(m, t, pl, sin2);
                 applied++;
             }
             return w;
         }
         if (sz <= best) {
-            if (dir > 0) {
+            if (zdir > 0) {
                 if (++r == layers) {
                     if (applied < nrefl) {
                         w.refl *= surface_factor(m, t, (copy & 1) ? 0u : 1u, sin2);
@@ -194,9 +216,8 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
                 u--;
             }
         } else {
-            uint32_t word = nw[bestk];
-            t = word >> 2;
-            entry = (int)(word & 3u);
+            t = nw >> 2;
+            entry = (int)(nw & 3u);
         }
     }
     w.status = 2;

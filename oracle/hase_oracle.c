@@ -325,9 +325,75 @@ int orc_importance(const orc_mesh* m, const orc_params* prm, uint32_t sample, do
     return 0;
 }
 
-/* Inclusive FP64 prefix of the defensive mixture proposal. */
+/* Index of (x, y) on a 2^16 x 2^16 Hilbert curve. */
+static uint64_t orc_hilbert_index(uint32_t x, uint32_t y)
+{
+    const uint32_t n = 1u << 16;
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) { x = n - 1 - x; y = n - 1 - y; }
+            uint32_t t = x; x = y; y = t;
+        }
+    }
+    return d;
+}
+
+typedef struct { uint64_t key; uint32_t index; } orc_keyed;
+
+static int orc_keyed_cmp(const void* a, const void* b)
+{
+    const orc_keyed* x = (const orc_keyed*)a;
+    const orc_keyed* y = (const orc_keyed*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->index < y->index ? -1 : (x->index > y->index);
+}
+
+static uint32_t orc_quant(double v, double lo, double hi)
+{
+    if (!(hi > lo)) return 0u;
+    double q = floor((v - lo) / (hi - lo) * 65535.0);
+    if (q < 0.0) q = 0.0;
+    if (q > 65535.0) q = 65535.0;
+    return (uint32_t)q;
+}
+
+/* Order in which the proposal prefix visits the triangles of a level:
+ * ascending Hilbert index of the centroid (16 bits per axis over the
+ * centroids' bounding box), ties by index. */
+void orc_triangle_order(const orc_mesh* m, uint32_t* order)
+{
+    const uint32_t T = m->nt;
+    double* cx = (double*)malloc(T * sizeof(double));
+    double* cy = (double*)malloc(T * sizeof(double));
+    orc_keyed* k = (orc_keyed*)malloc(T * sizeof(orc_keyed));
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (uint32_t t = 0; t < T; ++t) {
+        uint32_t a = m->tri[t], b = m->tri[t + T], c = m->tri[t + 2 * (size_t)T];
+        cx[t] = (m->px[a] + m->px[b] + m->px[c]) / 3.0;
+        cy[t] = (m->py[a] + m->py[b] + m->py[c]) / 3.0;
+        if (cx[t] < x0) x0 = cx[t];
+        if (cx[t] > x1) x1 = cx[t];
+        if (cy[t] < y0) y0 = cy[t];
+        if (cy[t] > y1) y1 = cy[t];
+    }
+    for (uint32_t t = 0; t < T; ++t) {
+        k[t].key = orc_hilbert_index(orc_quant(cx[t], x0, x1), orc_quant(cy[t], y0, y1));
+        k[t].index = t;
+    }
+    qsort(k, T, sizeof(orc_keyed), orc_keyed_cmp);
+    for (uint32_t t = 0; t < T; ++t) order[t] = k[t].index;
+    free(cx);
+    free(cy);
+    free(k);
+}
+
+/* Inclusive FP64 prefix of the defensive mixture proposal over entries
+ * (slice k, level l, triangle order[i]) in that nesting. */
 int orc_proposal_prefix(const orc_mesh* m, const orc_params* prm, const double* imp,
-                        double* prefix)
+                        const uint32_t* order, double* prefix)
 {
     const uint32_t T = m->nt, P = T * (m->nl - 1), R = orc_slices(prm);
     double I = 0.0, B = 0.0;
@@ -341,13 +407,14 @@ int orc_proposal_prefix(const orc_mesh* m, const orc_params* prm, const double* 
     int use_imp = I > 0.0 && isfinite(I);
     double acc = 0.0;
     for (uint32_t k = 0; k < R; ++k) {
-        for (uint32_t j = 0; j < P; ++j) {
+        for (uint32_t i = 0; i < P; ++i) {
+            uint32_t j = order[i % T] + (i / T) * T;
             double b = m->beta[j];
             double bv = b > 0.0 ? b * orc_tri_area(m, j % T) * m->h : 0.0;
             double q = use_imp ? (1.0 - eps) * imp[(size_t)k * P + j] / I + eps * bv / (B * (double)R)
                                : bv / (B * (double)R);
             acc += q;
-            prefix[(size_t)k * P + j] = acc;
+            prefix[(size_t)k * P + i] = acc;
         }
     }
     return 0;
@@ -392,7 +459,7 @@ static size_t orc_upper_bound(const double* prefix, size_t n, double target)
 
 static void orc_trace_launch(const orc_mesh* m, const orc_params* prm, uint32_t sample,
                              uint32_t launch, uint32_t nrays, const double* prefix,
-                             double* s1, double* s2, uint32_t* dropped)
+                             const uint32_t* order, double* s1, double* s2, uint32_t* dropped)
 {
     const uint32_t T = m->nt, P = T * (m->nl - 1), R = orc_slices(prm);
     const size_t n = (size_t)R * P;
@@ -406,7 +473,8 @@ static void orc_trace_launch(const orc_mesh* m, const orc_params* prm, uint32_t 
         double target = ((double)r + U) / (double)nrays * Q;
         size_t ent = orc_upper_bound(prefix, n, target);
         double q = prefix[ent] - (ent ? prefix[ent - 1] : 0.0);
-        uint32_t k = (uint32_t)(ent / P), j = (uint32_t)(ent % P);
+        uint32_t k = (uint32_t)(ent / P), i = (uint32_t)(ent % P);
+        uint32_t j = order[i % T] + (i / T) * T;
         uint32_t nr, fp;
         orc_slice(k, &nr, &fp);
         double e[3], a[3];
@@ -456,13 +524,15 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
     {
         double* imp = (double*)malloc(n * sizeof(double));
         double* prefix = (double*)malloc(n * sizeof(double));
+        uint32_t* order = (uint32_t*)malloc(T * sizeof(uint32_t));
+        orc_triangle_order(m, order);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
         for (int64_t si = (int64_t)prm->sample_min; si <= (int64_t)prm->sample_max; ++si) {
             uint32_t s = (uint32_t)si;
             orc_importance(m, prm, s, imp);
-            if (orc_proposal_prefix(m, prm, imp, prefix) != 0) continue; /* no source */
+            if (orc_proposal_prefix(m, prm, imp, order, prefix) != 0) continue; /* no source */
             double s1 = 0.0, s2 = 0.0;
             uint32_t N = 0, drop = 0, launch = 0, k = 0, extra = 0;
             double rse = DBL_MAX;
@@ -476,7 +546,7 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
                     extra++;
                     nr = prm->max_rays;
                 } else { k++; continue; }
-                orc_trace_launch(m, prm, s, launch, nr, prefix, &s1, &s2, &drop);
+                orc_trace_launch(m, prm, s, launch, nr, prefix, order, &s1, &s2, &drop);
                 N += nr;
                 launch++;
                 rse = orc_relative_standard_error(s1, s2, (double)N);
@@ -498,6 +568,7 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
         }
         free(imp);
         free(prefix);
+        free(order);
     }
     return 0;
 }

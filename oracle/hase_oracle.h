@@ -85,8 +85,9 @@ uint32_t orc_adaptive_target(uint32_t min_rays, uint32_t max_rays, uint32_t step
 double orc_relative_standard_error(double s1, double s2, double n);
 
 int orc_importance(const orc_mesh* m, const orc_params* prm, uint32_t sample, double* imp);
+void orc_triangle_order(const orc_mesh* m, uint32_t* order);
 int orc_proposal_prefix(const orc_mesh* m, const orc_params* prm, const double* imp,
-                        double* prefix);
+                        const uint32_t* order, double* prefix);
 void orc_ray_start(const orc_mesh* m, uint32_t seed, uint32_t sample, uint32_t launch,
                    uint32_t ray, uint32_t prism, double out[3]);
 double orc_systematic_offset(uint32_t seed, uint32_t sample, uint32_t launch);

