@@ -111,6 +111,162 @@ struct ImportanceParams {
     unsigned long long* counters;
 };
 
+// Levels whose centroid walks share one xy walk in the importance kernel.
+constexpr int kLevelGroup = 4;
+
+// Importance of the prisms (t0, lv0 .. lv0+nlev-1) for one reflection slice.
+// Every level (and every mirror slice) of a triangle column projects onto the
+// same xy segment from the triangle's centroid to the sample point, so the
+// column is walked once in xy and each level's ray is advanced along it,
+// split at that level's own plane crossings.  The per-level sums are those of
+// walk_ray on the same ray: sum(beta l), sum(l) over gain prisms, sum(l) over
+// cladding prisms, the surface factors at the copy boundaries, and the
+// prisms visited.
+template <bool kReflect>
+__device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0, int lv0, int nlev, float px,
+                                                  float py, int end_plane, uint32_t nrefl, uint32_t first,
+                                                  float out_bl[kLevelGroup], float out_gl[kLevelGroup],
+                                                  float out_cl[kLevelGroup], float out_rf[kLevelGroup],
+                                                  float D[kLevelGroup], unsigned long long& steps, bool& ok)
+{
+    const int layers = (int)m.layers;
+    const float h = m.h;
+    const uint4 c0 = m.rec[3u * t0], c1 = m.rec[3u * t0 + 1u];
+    const float cx = (bits_to_float(c0.x) + bits_to_float(c0.z) + bits_to_float(c1.x)) * (1.0f / 3.0f);
+    const float cy = (bits_to_float(c0.y) + bits_to_float(c0.w) + bits_to_float(c1.y)) * (1.0f / 3.0f);
+    const float dx = px - cx, dy = py - cy;
+    const float Dxy = sqrtf(dx * dx + dy * dy);
+    const float inv_xy = 1.0f / Dxy;
+    const float vx = dx * inv_xy, vy = dy * inv_xy;
+    const float ze = (float)end_plane;
+    // per level: parameter of a plane = (plane - za) * idz; unfolded layer u = cp * layers + r
+    float idz[kLevelGroup];
+    int r[kLevelGroup], cp[kLevelGroup];
+    uint32_t applied[kLevelGroup];
+#pragma unroll
+    for (int g = 0; g < kLevelGroup; ++g) {
+        const float dz = ze - ((float)(lv0 + g) + 0.5f);
+        idz[g] = 1.0f / dz;
+        D[g] = sqrtf(Dxy * Dxy + dz * h * dz * h);
+        r[g] = lv0 + g;
+        cp[g] = 0;
+        applied[g] = 0;
+        out_bl[g] = 0.0f; out_gl[g] = 0.0f; out_cl[g] = 0.0f; out_rf[g] = 1.0f;
+    }
+    const float tol = 2e-5f * (1.0f + Dxy);
+    uint32_t t = t0;
+    int entry = -1;
+    float s = 0.0f;
+    ok = true;
+    for (uint32_t step = 0; step < kMaxSteps; ++step) {
+        const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
+        const bool clad = q2.y != 0u;
+        const float* bcol = m.beta_col + (size_t)t * (uint32_t)layers;
+        // this column's beta at every level's current layer: independent loads, one round trip
+        float bcur[kLevelGroup];
+#pragma unroll
+        for (int g = 0; g < kLevelGroup; ++g) {
+            const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
+            bcur[g] = (g < nlev && !clad) ? bcol[lvl] : 0.0f;
+        }
+        const float X0 = bits_to_float(q0.x) - cx, Y0 = bits_to_float(q0.y) - cy;
+        const float X1 = bits_to_float(q0.z) - cx, Y1 = bits_to_float(q0.w) - cy;
+        const float X2 = bits_to_float(q1.x) - cx, Y2 = bits_to_float(q1.y) - cy;
+        const float ex0 = X1 - X0, ey0 = Y1 - Y0;
+        const float ex1 = X2 - X1, ey1 = Y2 - Y1;
+        const float ex2 = X0 - X2, ey2 = Y0 - Y2;
+        const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
+        float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
+        float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
+        float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
+        s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
+        s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
+        s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
+        int bestk = 0;
+        float best = s0;
+        if (s1 < best) { best = s1; bestk = 1; }
+        if (s2 < best) { best = s2; bestk = 2; }
+        {
+            const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
+            const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
+            const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
+            const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
+            const bool touches = (bits_to_float(axb) == px && bits_to_float(ayb) == py) ||
+                                 (bits_to_float(bxb) == px && bits_to_float(byb) == py);
+            best = (touches && best < INFINITY) ? Dxy : best;
+        }
+        const float snext = fmaxf(fminf(best, Dxy), s);
+        const bool at_end = snext >= Dxy;
+        const float sig0 = s * inv_xy;
+        const float sig1 = at_end ? 1.0f : snext * inv_xy;
+#pragma unroll
+        for (int g = 0; g < kLevelGroup; ++g) {
+            if (g < nlev) {
+                const float za = (float)(lv0 + g) + 0.5f;
+                const bool up = idz[g] > 0.0f;
+                float a = sig0;
+                float b = bcur[g];
+                for (;;) {
+                    const int plane = cp[g] * layers + r[g] + (up ? 1 : 0);
+                    const float sp = ((float)plane - za) * idz[g];
+                    // the end plane is reached at sigma = 1, never crossed
+                    if (plane == end_plane || !(sp <= sig1)) break;
+                    const float len = fmaxf(sp - a, 0.0f) * D[g];
+                    out_cl[g] += clad ? len : 0.0f;
+                    out_gl[g] += clad ? 0.0f : len;
+                    out_bl[g] = fmaf(b, len, out_bl[g]);
+                    steps++;
+                    a = fmaxf(a, sp);
+                    const int rr = r[g] + (up ? 1 : -1);
+                    if (kReflect && (rr == layers || rr < 0)) {
+                        if (applied[g] < nrefl) {
+                            const float vz = (ze - za) * h / D[g];
+                            const uint32_t pl = up ? ((cp[g] & 1) ? 0u : 1u) : ((cp[g] & 1) ? 1u : 0u);
+                            out_rf[g] *= surface_factor(m, t, pl, fmaxf(0.0f, 1.0f - vz * vz));
+                            applied[g]++;
+                        }
+                        r[g] = up ? 0 : layers - 1;
+                        cp[g] += up ? 1 : -1;
+                    } else {
+                        r[g] = rr;
+                    }
+                    const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
+                    b = clad ? 0.0f : bcol[lvl];
+                }
+                const float len = fmaxf(sig1 - a, 0.0f) * D[g];
+                out_cl[g] += clad ? len : 0.0f;
+                out_gl[g] += clad ? 0.0f : len;
+                out_bl[g] = fmaf(b, len, out_bl[g]);
+                steps++;
+            }
+        }
+        s = snext;
+        const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
+        bool done = at_end;
+        if (!done && nw == 0xFFFFFFFFu) {
+            if (Dxy - s <= tol) done = true;
+            else { ok = false; return; }
+        }
+        if (done) {
+            if (!kReflect) return;
+#pragma unroll
+            for (int g = 0; g < kLevelGroup; ++g) {
+                const float vz = (ze - ((float)(lv0 + g) + 0.5f)) * h / D[g];
+                const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
+                while (applied[g] < nrefl) {
+                    const uint32_t pl = (applied[g] & 1u) ? 1u - first : first;
+                    out_rf[g] *= surface_factor(m, t, pl, sin2);
+                    applied[g]++;
+                }
+            }
+            return;
+        }
+        t = nw >> 2;
+        entry = (int)(nw & 3u);
+    }
+    ok = false;
+}
+
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
     const uint32_t slot = blockIdx.x;
@@ -129,29 +285,40 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
 
     double local = 0.0;
     unsigned long long walks = 0, steps = 0;
-    for (size_t e = tid; e < n; e += kBlock) {
-        uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
-        uint32_t t = j % T, l = j / T;
-        float b = m.beta[j];
-        float value = 0.0f;
-        if (b > 0.0f) {
-            uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u];
-            float cx = (bits_to_float(q0.x) + bits_to_float(q0.z) + bits_to_float(q1.x)) * (1.0f / 3.0f);
-            float cy = (bits_to_float(q0.y) + bits_to_float(q0.w) + bits_to_float(q1.y)) * (1.0f / 3.0f);
-            float cz = (float)l + 0.5f;   // layer units
-            uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
-            float ez = (float)image_plane(sp.plane, (int)m.layers, nr, fp);
-            WalkOut w = walk_ray(m, cx, cy, cz, t, l, sp.px, sp.py, ez, nr, fp);
-            walks++;
-            steps += w.steps;
-            if (w.status == 0 && w.length > 0.0f) {
-                float g = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
-                value = m.bv[j] * g * w.refl / (4.0f * kPi * w.length * w.length);
-                if (!(value >= 0.0f) || isinf(value)) value = 0.0f;
+    const uint32_t groups = (m.layers + kLevelGroup - 1) / kLevelGroup;
+    const uint32_t items = T * p.R * groups;
+    for (uint32_t it = tid; it < items; it += kBlock) {
+        const uint32_t t0 = it % T, rest = it / T;
+        const uint32_t gi = rest % groups, k = rest / groups;
+        const int lv0 = (int)(gi * kLevelGroup);
+        const int nlev = min(kLevelGroup, (int)m.layers - lv0);
+        const uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
+        const int end_plane = image_plane(sp.plane, (int)m.layers, nr, fp);
+        float bl[kLevelGroup], gl[kLevelGroup], cl[kLevelGroup], rf[kLevelGroup], len[kLevelGroup];
+        bool ok;
+        if (nr == 0)
+            column_importance<false>(m, t0, lv0, nlev, sp.px, sp.py, end_plane, 0u, fp, bl, gl, cl, rf, len, steps, ok);
+        else
+            column_importance<true>(m, t0, lv0, nlev, sp.px, sp.py, end_plane, nr, fp, bl, gl, cl, rf, len, steps, ok);
+
+        walks += (unsigned long long)nlev;
+#pragma unroll
+        for (int g = 0; g < kLevelGroup; ++g) {
+            if (g < nlev) {
+                const uint32_t j = t0 + (uint32_t)(lv0 + g) * T;
+                const float bv = m.bv[j];
+                float value = 0.0f;
+                if (ok && bv > 0.0f) {
+                    WalkOut w;
+                    w.beta_len = bl[g]; w.gain_len = gl[g]; w.clad_len = cl[g];
+                    const float gain = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
+                    value = bv * gain * rf[g] / (4.0f * kPi * len[g] * len[g]);
+                    if (!(value >= 0.0f) || isinf(value)) value = 0.0f;
+                }
+                imp[(size_t)k * P + j] = value;
+                local += (double)value;
             }
         }
-        imp[e] = value;
-        local += (double)value;
     }
     local = wave_sum(local);
     if (lane == 0) red[wid] = local;
@@ -172,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     const bool use_imp = I > 0.0 && isfinite(I);
     const double wI = use_imp ? (1.0 - p.eps) / I : 0.0;
     const double wB = (use_imp ? p.eps : 1.0) / (p.Bsum * (double)p.R);
-    // pass 2 re-reads imp[e] from the thread that wrote it (same e = base + tid)
+    // pass 2 reads entries written by other threads of the block before the barrier above
     for (size_t base = 0; base < n; base += kBlock) {
         size_t e = base + tid;
         double q = 0.0;
@@ -462,7 +629,7 @@ struct hase_context {
     hase_compute_t comp{};
     DevBuf<uint4> rec;
     DevBuf<float2> pts;
-    DevBuf<float> beta, bv, refl, c1, c2;
+    DevBuf<float> beta, beta_col, bv, refl, c1, c2;
     DevBuf<float> imp;
     DevBuf<double> prefix, Q, s1, s2, bcells;
     DevBuf<double2> partial;
@@ -549,6 +716,10 @@ void upload_beta(hase_context* ctx, const double* beta_volume, const double* bet
     ctx->Bsum = B;
     ctx->beta.upload(b.data(), P, ctx->stream);
     ctx->bv.upload(bv.data(), P, ctx->stream);
+    std::vector<float> bc(P);
+    const uint32_t layers = ctx->L - 1;
+    for (uint32_t j = 0; j < P; ++j) bc[(size_t)(j % T) * layers + j / T] = b[j];
+    ctx->beta_col.upload(bc.data(), P, ctx->stream);
     if (beta_cells) {
         ctx->beta_cells.assign(beta_cells, beta_cells + ctx->S);
         ctx->bcells.upload(ctx->beta_cells.data(), ctx->S, ctx->stream);
@@ -697,6 +868,7 @@ MeshDev mesh_view(const hase_context* ctx)
     MeshDev m;
     m.rec = ctx->rec.p;
     m.beta = ctx->beta.p;
+    m.beta_col = ctx->beta_col.p;
     m.bv = ctx->bv.p;
     m.refl = ctx->has_refl ? ctx->refl.p : nullptr;
     m.nin_b = ctx->nidx[0]; m.nout_b = ctx->nidx[1]; m.nin_t = ctx->nidx[2]; m.nout_t = ctx->nidx[3];

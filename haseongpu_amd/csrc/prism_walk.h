@@ -25,8 +25,9 @@ HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float
 
 struct MeshDev {
     const uint4* rec;      // 3 records per triangle (see pack_triangles)
-    const float* beta;     // [P] level-major
-    const float* bv;       // [P] beta * prism volume
+    const float* beta;     // [P] level-major: t + l * T
+    const float* beta_col; // [P] column-major: t * layers + l
+    const float* bv;       // [P] beta * prism volume, level-major
     const float* refl;     // [2T] or nullptr
     float nin_b, nout_b, nin_t, nout_t;
     int has_nidx;
@@ -102,8 +103,11 @@ HASE_HD inline float fast_rcp(float x)
 // the nearest one whose outward normal faces the ray (entry edge excluded);
 // an exit edge with an end point at the end's (x, y) meets the ray exactly
 // at the end; reflections are the copy boundaries of the unfolded z axis.
-HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+// kReflect = false is the direct walk (start and end inside the slab), which
+// needs neither the mirror folding nor surface factors.
+template <bool kReflect>
+HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
+                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
 {
     WalkOut w;
     w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
@@ -130,7 +134,7 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
     uint32_t applied = 0;
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
         w.steps++;
-        const int level = (copy & 1) ? (layers - 1 - r) : r;
+        const int level = kReflect ? ((copy & 1) ? (layers - 1 - r) : r) : u;
         // the triangle record and this prism's beta are independent loads
         const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
         const float beta = m.beta[t + (uint32_t)level * m.T];
@@ -177,51 +181,61 @@ HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, 
         w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
         s = snext;
         const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
+        const bool zstep = sz <= best;
         bool done = s >= D;
-        if (!done && sz > best && nw == 0xFFFFFFFFu) {
+        if (!done && !zstep && nw == 0xFFFFFFFFu) {
             // leaving the mesh sideways: only rounding short of the end (a sample
             // point on the rim) counts as arrival
             if (D - s <= tol) done = true;
             else { w.status = 1; return w; }
         }
         if (done) {
-            // an end point on a reflecting plane takes its last factor here
-            while (applied < nrefl) {
-                uint32_t pl = (applied & 1u) ? 1u - first : first;
-                w.refl *= surface_factor(m, t, pl, sin2);
-                applied++;
+            if (kReflect) {
+                // an end point on a reflecting plane takes its last factor here
+                while (applied < nrefl) {
+                    uint32_t pl = (applied & 1u) ? 1u - first : first;
+                    w.refl *= surface_factor(m, t, pl, sin2);
+                    applied++;
+                }
             }
             return w;
         }
-        if (sz <= best) {
-            if (zdir > 0) {
-                if (++r == layers) {
-                    if (applied < nrefl) {
-                        w.refl *= surface_factor(m, t, (copy & 1) ? 0u : 1u, sin2);
-                        applied++;
+        if (kReflect) {
+            if (zstep) {
+                if (zdir > 0) {
+                    if (++r == layers) {
+                        if (applied < nrefl) {
+                            w.refl *= surface_factor(m, t, (copy & 1) ? 0u : 1u, sin2);
+                            applied++;
+                        }
+                        r = 0;
+                        copy++;
                     }
-                    r = 0;
-                    copy++;
-                }
-                u++;
-            } else {
-                if (--r < 0) {
-                    if (applied < nrefl) {
-                        w.refl *= surface_factor(m, t, (copy & 1) ? 1u : 0u, sin2);
-                        applied++;
+                } else {
+                    if (--r < 0) {
+                        if (applied < nrefl) {
+                            w.refl *= surface_factor(m, t, (copy & 1) ? 1u : 0u, sin2);
+                            applied++;
+                        }
+                        r = layers - 1;
+                        copy--;
                     }
-                    r = layers - 1;
-                    copy--;
                 }
-                u--;
             }
-        } else {
-            t = nw >> 2;
-            entry = (int)(nw & 3u);
         }
+        u = zstep ? u + zdir : u;
+        t = zstep ? t : (nw >> 2);
+        entry = zstep ? entry : (int)(nw & 3u);
     }
     w.status = 2;
     return w;
+}
+
+HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
+                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+{
+    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first)
+                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first);
 }
 
 // Height, in layer units, of the mirror image of a sample point on plane
