@@ -31,6 +31,15 @@ EXPORTED_SYMBOLS = (
     "hase_context_destroy",
     "hase_context_trace_rays",
     "hase_context_importance",
+    "hase_mlp_param_count",
+    "hase_mlp_forward",
+    "hase_mlp_loss_gradient",
+    "hase_mlp_train",
+    "hase_mlp_forward_bf16",
+    "hase_context_set_proposal",
+    "hase_context_transfer_map",
+    "hase_context_proposal_logits",
+    "hase_context_proposal_stats",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -127,6 +136,49 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Train(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_uint32),
+        ("batch", C.c_uint32),
+        ("learning_rate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("weight_decay", C.c_float),
+        ("seed", C.c_uint32),
+    ]
+
+
+class Proposal(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("params", _f32p),
+        ("input_shift", _f32p),
+        ("input_scale", _f32p),
+        ("cluster_triangles", C.c_uint32),
+        ("cluster_levels", C.c_uint32),
+        ("fallback", C.c_int32),
+    ]
+
+
+class ProposalStats(C.Structure):
+    _fields_ = [
+        ("inference_ms", C.c_double),
+        ("proposal_ms", C.c_double),
+        ("evaluations", C.c_uint64),
+        ("fallback_samples", C.c_uint32),
+        ("clusters", C.c_uint32),
+    ]
+
+    def asdict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+MLP_INPUTS = 10
+MLP_HIDDEN = 128
+PROPOSAL_PLAIN = 0
+PROPOSAL_LEARNED = 1
+
 _lib = None
 
 
@@ -151,12 +203,24 @@ def load():
     L.hase_context_trace_rays.argtypes = [ctx_p, C.c_uint32, _f32p, _u32p, _f32p, _u32p, _u32p,
                                           _f32p, _f32p, _f32p, _f32p, _u32p, _i32p]
     L.hase_context_importance.argtypes = [ctx_p, C.c_uint32, _f32p, C.c_uint32]
+    L.hase_mlp_param_count.argtypes = []
+    L.hase_mlp_param_count.restype = C.c_uint32
+    L.hase_mlp_forward.argtypes = [_f32p, C.c_uint32, _f32p, _f32p, C.c_int32]
+    L.hase_mlp_loss_gradient.argtypes = [_f32p, C.c_uint32, _f32p, _f32p, _f32p, _f64p, _f32p, C.c_int32]
+    L.hase_mlp_train.argtypes = [_f32p, C.c_uint32, _f32p, _f32p, _f32p, C.POINTER(Train), _f32p, C.c_int32]
+    L.hase_mlp_forward_bf16.argtypes = [_f32p, C.c_uint32, _f32p, _f32p, C.c_int32]
+    L.hase_context_set_proposal.argtypes = [ctx_p, C.POINTER(Proposal)]
+    L.hase_context_transfer_map.argtypes = [ctx_p, C.c_uint32, C.c_uint32, _f32p, C.c_size_t]
+    L.hase_context_proposal_logits.argtypes = [ctx_p, C.c_uint32, C.c_uint32, _f32p, C.c_size_t, _f32p, _u32p]
+    L.hase_context_proposal_stats.argtypes = [ctx_p, C.POINTER(ProposalStats)]
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
     for name in ("hase_calc_phi_ase", "hase_context_create", "hase_context_update_beta",
                  "hase_context_evaluate", "hase_context_stats", "hase_context_trace_rays",
-                 "hase_context_importance"):
+                 "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
+                 "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
+                 "hase_context_proposal_logits", "hase_context_proposal_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -294,12 +358,120 @@ class Context:
         out["status"] = status
         return out
 
+    @property
+    def slices(self):
+        return 1 + 2 * ((self._exp.struct.max_reflections or 1) if self._exp.struct.use_reflections else 0)
+
+    @property
+    def prisms(self):
+        return self._mesh.struct.number_of_triangles * (self._mesh.struct.number_of_levels - 1)
+
+    def set_proposal(self, proposal=None):
+        """Select the learned proposal (a ``LearnedProposal``) or, with None, the walked map."""
+        if proposal is None:
+            st = Proposal(PROPOSAL_PLAIN, None, None, None, 1, 1, 0)
+            check(load().hase_context_set_proposal(self._h, C.byref(st)))
+            self._proposal = None
+            return
+        params = np.ascontiguousarray(proposal.params, dtype=np.float32)
+        shift = np.ascontiguousarray(proposal.shift, dtype=np.float32)
+        scale = np.ascontiguousarray(proposal.scale, dtype=np.float32)
+        if params.size != mlp_param_count() or shift.size != MLP_INPUTS or scale.size != MLP_INPUTS:
+            raise ValueError("proposal arrays do not match the network layout")
+        st = Proposal(PROPOSAL_LEARNED, _ptr(params, C.c_float), _ptr(shift, C.c_float), _ptr(scale, C.c_float),
+                      int(proposal.cluster_triangles), int(proposal.cluster_levels), int(bool(proposal.fallback)))
+        check(load().hase_context_set_proposal(self._h, C.byref(st)))
+        self._proposal = (params, shift, scale)
+
+    def transfer_map(self, first, count):
+        """(count, R, P) transfer factors of the centroid walks, caller prism order."""
+        R, P = self.slices, self.prisms
+        out = np.zeros(count * R * P, dtype=np.float32)
+        check(load().hase_context_transfer_map(self._h, int(first), int(count), _ptr(out, C.c_float), out.size))
+        return out.reshape(count, R, P)
+
+    def proposal_logits(self, first, count):
+        """Network outputs (count, R, C) and the cluster representatives (C, 4)."""
+        R = self.slices
+        Cn = int(self.proposal_stats()["clusters"])
+        if Cn == 0:
+            raise RuntimeError("no learned proposal is set")
+        n_cl = C.c_uint32(0)
+        out = np.zeros(count * R * Cn, dtype=np.float32)
+        clusters = np.zeros(Cn * 4, dtype=np.float32)
+        check(load().hase_context_proposal_logits(self._h, int(first), int(count), _ptr(out, C.c_float), out.size,
+                                                  _ptr(clusters, C.c_float), C.byref(n_cl)))
+        return out.reshape(count, R, Cn), clusters.reshape(Cn, 4)
+
+    def proposal_stats(self):
+        st = ProposalStats()
+        check(load().hase_context_proposal_stats(self._h, C.byref(st)))
+        return st.asdict()
+
     def importance(self, sample):
         R = 1 + 2 * ((self._exp.struct.max_reflections or 1) if self._exp.struct.use_reflections else 0)
         P = self._mesh.struct.number_of_triangles * (self._mesh.struct.number_of_levels - 1)
         out = np.zeros(R * P, dtype=np.float32)
         check(load().hase_context_importance(self._h, int(sample), _ptr(out, C.c_float), out.size))
         return out
+
+
+def mlp_param_count():
+    return int(load().hase_mlp_param_count())
+
+
+def _rows(inputs):
+    x = np.ascontiguousarray(inputs, dtype=np.float32)
+    if x.ndim != 2 or x.shape[1] != MLP_INPUTS:
+        raise ValueError(f"inputs must be (n, {MLP_INPUTS})")
+    return x
+
+
+def mlp_forward(params, inputs, device=0):
+    """FP32 device forward pass of the proposal network on normalised inputs."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = _rows(inputs)
+    y = np.zeros(x.shape[0], dtype=np.float32)
+    check(load().hase_mlp_forward(_ptr(p, C.c_float), x.shape[0], _ptr(x, C.c_float), _ptr(y, C.c_float), device))
+    return y
+
+
+def mlp_forward_bf16(params, inputs, device=0):
+    """BF16-MFMA device inference of the proposal network on normalised inputs."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = _rows(inputs)
+    y = np.zeros(x.shape[0], dtype=np.float32)
+    check(load().hase_mlp_forward_bf16(_ptr(p, C.c_float), x.shape[0], _ptr(x, C.c_float), _ptr(y, C.c_float),
+                                       device))
+    return y
+
+
+def mlp_loss_gradient(params, inputs, targets, weights=None, device=0):
+    """Weighted squared error and its gradient from the device backward kernels."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = _rows(inputs)
+    t = np.ascontiguousarray(targets, dtype=np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    g = np.zeros_like(p)
+    loss = C.c_double(0.0)
+    check(load().hase_mlp_loss_gradient(_ptr(p, C.c_float), x.shape[0], _ptr(x, C.c_float), _ptr(t, C.c_float),
+                                        _ptr(w, C.c_float), C.byref(loss), _ptr(g, C.c_float), device))
+    return loss.value, g
+
+
+def mlp_train(params, inputs, targets, weights=None, *, epochs=10, batch=4096, learning_rate=1e-3, beta1=0.9,
+              beta2=0.999, epsilon=1e-8, weight_decay=0.0, seed=0, device=0):
+    """AdamW training on the device; returns (params, per-epoch loss)."""
+    p = np.array(params, dtype=np.float32, copy=True)
+    x = _rows(inputs)
+    t = np.ascontiguousarray(targets, dtype=np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    hist = np.zeros(int(epochs), dtype=np.float32)
+    cfg = Train(int(epochs), int(batch), float(learning_rate), float(beta1), float(beta2), float(epsilon),
+                float(weight_decay), int(seed) & 0xFFFFFFFF)
+    check(load().hase_mlp_train(_ptr(p, C.c_float), x.shape[0], _ptr(x, C.c_float), _ptr(t, C.c_float),
+                                _ptr(w, C.c_float), C.byref(cfg), _ptr(hist, C.c_float), device))
+    return p, hist
 
 
 def calc_phi_ase(mesh: NativeMesh, experiment: NativeExperiment, compute: Compute):

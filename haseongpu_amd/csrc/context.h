@@ -1,0 +1,73 @@
+// context.h - the persistent device context behind hase_context_t.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/hase_ase.h"
+#include "common.h"
+#include "prism_walk.h"
+
+struct hase_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t np = 0, T = 0, L = 0, P = 0, S = 0;
+    double h = 0.0;
+    double n_tot = 0.0, tau = 0.0;
+    double clad_alpha = 0.0;
+    bool has_refl = false, has_nidx = false;
+    float nidx[4] = {1, 1, 1, 1};
+    std::vector<double> tri_area;      // [T] device order
+    std::vector<uint32_t> clad_flag;   // [T] device order
+    std::vector<uint32_t> dev2user;    // device triangle -> caller triangle
+    std::vector<uint32_t> user2dev;    // caller triangle -> device triangle
+    std::vector<double> beta_host;     // [P]
+    std::vector<double> beta_cells;    // [S]
+    double Bsum = 0.0;
+    hase_experiment_t exp{};
+    std::vector<double> sigma_a, sigma_e;
+    hase_compute_t comp{};
+    hase::DevBuf<uint4> rec;
+    hase::DevBuf<float2> pts;
+    hase::DevBuf<float> beta, beta_col, bv, refl, c1, c2;
+    hase::DevBuf<float> imp;
+    hase::DevBuf<double> prefix, Q, s1, s2, bcells;
+    hase::DevBuf<double2> partial;
+    hase::DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count;
+    hase::DevBuf<int32_t> active_flag;
+    hase::DevBuf<float> phi;
+    hase::DevBuf<double> mse, se;
+    hase::DevBuf<unsigned long long> counters;
+    hase_stats_t stats{};
+    std::vector<hipEvent_t> events;
+
+    // learned proposal
+    int proposal_kind = HASE_PROPOSAL_PLAIN;
+    bool proposal_fallback = true;
+    uint32_t cluster_t = 1, cluster_l = 1;   // triangles / layers per cluster
+    uint32_t TG = 0, LG = 0, C = 0;          // triangle groups, layer groups, clusters per slice
+    float in_shift[HASE_MLP_INPUTS] = {}, in_scale[HASE_MLP_INPUTS] = {};
+    float log_sigma = 0.0f;                  // log(max_sigma_e / max_sigma_a)
+    std::vector<float2> centroid_host;       // [T] triangle centroids, device order
+    hase::DevBuf<float2> centroid;
+    hase::DevBuf<uint4> mlp_frags;           // BF16 operand fragments
+    hase::DevBuf<float> mlp_vec;             // FP32 biases and output weights
+    std::vector<float4> clusters_host;       // [C] (x, y, level, beta) representatives
+    hase::DevBuf<float4> clusters;
+    hase::DevBuf<float> logits;              // [slots * R * C]
+    hase_proposal_stats_t pstats{};
+
+    ~hase_context()
+    {
+        for (auto e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+
+namespace hase {
+MeshDev mesh_view(const hase_context* ctx);
+uint32_t slices(const hase_context* ctx);
+}  // namespace hase

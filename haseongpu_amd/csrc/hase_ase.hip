@@ -31,29 +31,16 @@
 #include <vector>
 
 #include "../../include/hase_ase.h"
+#include "common.h"
+#include "context.h"
 #include "prism_walk.h"
+#include "proposal.h"
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr float kPi = 3.14159265358979f;
-
-thread_local std::string g_last_error;
-
-struct HaseError : std::runtime_error {
-    int code;
-    HaseError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_CHECK(expr)                                                                    \
-    do {                                                                                   \
-        hipError_t err_ = (expr);                                                          \
-        if (err_ != hipSuccess)                                                            \
-            throw HaseError(HASE_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(err_)); \
-    } while (0)
+using namespace hase;
 
 // ------------------------------------------------------------------ device
-using namespace hase;
 
 struct SampleGeom {
     float px, py;
@@ -93,6 +80,53 @@ __device__ __forceinline__ double wave_inclusive_scan(double v, int lane)
     return v;
 }
 
+// Defensive-mixture proposal and its FP64 inclusive prefix for one sample
+// (one work-group): q_e = (1 - eps) imp_e / I + eps * bv_j / (R * Bsum), with
+// I the work-group sum of the thread partials `local`.  value(e) returns
+// imp_e and must be readable by every thread (the entries of other threads
+// are read after a barrier).  red holds kBlock/64 + 2 doubles of LDS.
+template <class F>
+__device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t P, const float* bv, double eps,
+                                               double Bsum, uint32_t R, F value, double* pre, double* Qout,
+                                               double* red)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double* carry_s = red + kBlock / 64;
+    double* total_s = red + kBlock / 64 + 1;
+    local = wave_sum(local);
+    if (lane == 0) red[wid] = local;
+    __syncthreads();
+    if (tid == 0) {
+        double I = 0.0;
+        for (int i = 0; i < kBlock / 64; ++i) I += red[i];
+        *total_s = I;
+        *carry_s = 0.0;
+    }
+    __syncthreads();
+    const double I = *total_s;
+    const bool use_imp = I > 0.0 && isfinite(I);
+    const double wI = use_imp ? (1.0 - eps) / I : 0.0;
+    const double wB = (use_imp ? eps : 1.0) / (Bsum * (double)R);
+    for (size_t base = 0; base < n; base += kBlock) {
+        size_t e = base + tid;
+        double q = 0.0;
+        if (e < n) {
+            uint32_t j = (uint32_t)(e % P);
+            q = wI * (double)value(e) + wB * (double)bv[j];
+        }
+        double inc = wave_inclusive_scan(q, lane);
+        if (lane == 63) red[wid] = inc;
+        __syncthreads();
+        double off = *carry_s;
+        for (int i = 0; i < wid; ++i) off += red[i];
+        if (e < n) pre[e] = off + inc;
+        __syncthreads();
+        if (tid == kBlock - 1) *carry_s = off + inc;
+        __syncthreads();
+    }
+    if (tid == 0) *Qout = *carry_s;
+}
+
 struct ImportanceParams {
     MeshDev m;
     const float2* pts;
@@ -109,6 +143,8 @@ struct ImportanceParams {
     double* prefix;           // [slots * R * P]
     double* Q;                // [slots]
     unsigned long long* counters;
+    const uint32_t* active;   // block -> slot (nullptr: block index is the slot)
+    int transfer;             // write the transfer factor gain * reflectance instead of the importance
 };
 
 // Levels whose centroid walks share one xy walk in the importance kernel.
@@ -269,7 +305,7 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
 
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
-    const uint32_t slot = blockIdx.x;
+    const uint32_t slot = p.active ? p.active[blockIdx.x] : blockIdx.x;
     if (slot >= p.slots) return;
     const MeshDev& m = p.m;
     const uint32_t T = m.T, P = T * m.layers;
@@ -278,10 +314,8 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     SampleGeom sp = sample_point(p.pts, m.np, sample);
     float* imp = p.imp + (size_t)slot * n;
     double* pre = p.prefix + (size_t)slot * n;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    __shared__ double red[kBlock / 64];
-    __shared__ double carry_s;
-    __shared__ double total_s;
+    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ double red[kBlock / 64 + 2];
 
     double local = 0.0;
     unsigned long long walks = 0, steps = 0;
@@ -308,11 +342,11 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
                 const uint32_t j = t0 + (uint32_t)(lv0 + g) * T;
                 const float bv = m.bv[j];
                 float value = 0.0f;
-                if (ok && bv > 0.0f) {
+                if (ok && (bv > 0.0f || p.transfer)) {
                     WalkOut w;
                     w.beta_len = bl[g]; w.gain_len = gl[g]; w.clad_len = cl[g];
                     const float gain = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
-                    value = bv * gain * rf[g] / (4.0f * kPi * len[g] * len[g]);
+                    value = p.transfer ? gain * rf[g] : bv * gain * rf[g] / (4.0f * kPi * len[g] * len[g]);
                     if (!(value >= 0.0f) || isinf(value)) value = 0.0f;
                 }
                 imp[(size_t)k * P + j] = value;
@@ -320,44 +354,76 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
             }
         }
     }
-    local = wave_sum(local);
-    if (lane == 0) red[wid] = local;
     unsigned long long ws = wave_sum_u64(walks), ss = wave_sum_u64(steps);
     if (lane == 0 && ws) {
         atomicAdd(&p.counters[2], ws);
         atomicAdd(&p.counters[3], ss);
     }
-    __syncthreads();
-    if (tid == 0) {
-        double I = 0.0;
-        for (int i = 0; i < kBlock / 64; ++i) I += red[i];
-        total_s = I;
-        carry_s = 0.0;
-    }
-    __syncthreads();
-    const double I = total_s;
-    const bool use_imp = I > 0.0 && isfinite(I);
-    const double wI = use_imp ? (1.0 - p.eps) / I : 0.0;
-    const double wB = (use_imp ? p.eps : 1.0) / (p.Bsum * (double)p.R);
-    // pass 2 reads entries written by other threads of the block before the barrier above
-    for (size_t base = 0; base < n; base += kBlock) {
-        size_t e = base + tid;
-        double q = 0.0;
-        if (e < n) {
-            uint32_t j = (uint32_t)(e % P);
-            q = wI * (double)imp[e] + wB * (double)m.bv[j];
-        }
-        double inc = wave_inclusive_scan(q, lane);
-        if (lane == 63) red[wid] = inc;
-        __syncthreads();
-        double off = carry_s;
-        for (int i = 0; i < wid; ++i) off += red[i];
-        if (e < n) pre[e] = off + inc;
-        __syncthreads();
-        if (tid == kBlock - 1) carry_s = off + inc;
-        __syncthreads();
-    }
-    if (tid == 0) p.Q[slot] = carry_s;
+    if (p.transfer) return;
+    mixture_prefix(local, n, P, m.bv, p.eps, p.Bsum, p.R, [&](size_t e) { return imp[e]; }, pre, p.Q + slot, red);
+}
+
+// Learned proposal for one sample per work-group: the network's log transfer
+// factor y of the prism's cluster (one value per reflection slice and
+// cluster) times the prism's own beta * V / (4 pi d^2), d the distance from
+// the prism centroid to the sample's image for that slice; then the same
+// defensive mixture and prefix as the walked importance map.
+struct LearnedParams {
+    MeshDev m;
+    const float2* pts;
+    const float2* centroid;   // [T] device order
+    const float* logits;      // [slots * R * C]
+    uint32_t slots, sample_base, R, C, TG, ct, cl;
+    double eps, Bsum;
+    double* prefix;
+    double* Q;
+};
+
+__device__ __forceinline__ float learned_importance(const LearnedParams& p, const SampleGeom& sp, const float* lg,
+                                                    size_t e)
+{
+    const MeshDev& m = p.m;
+    const uint32_t P = m.T * m.layers;
+    const uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
+    const uint32_t t = j % m.T, l = j / m.T;
+    const uint32_t c = (l / p.cl) * p.TG + t / p.ct;
+    const float y = fminf(lg[(size_t)k * p.C + c], 80.0f);
+    const float2 cxy = p.centroid[t];
+    const float zi = (float)image_plane(sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
+    const float dx = cxy.x - sp.px, dy = cxy.y - sp.py, dz = ((float)l + 0.5f - zi) * m.h;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    float v = m.bv[j] * __expf(y) / (4.0f * kPi * d2);
+    return (v >= 0.0f && !isinf(v)) ? v : 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(LearnedParams p)
+{
+    const uint32_t slot = blockIdx.x;
+    if (slot >= p.slots) return;
+    const MeshDev& m = p.m;
+    const uint32_t P = m.T * m.layers;
+    const size_t n = (size_t)p.R * P;
+    const SampleGeom sp = sample_point(p.pts, m.np, p.sample_base + slot);
+    const float* lg = p.logits + (size_t)slot * p.R * p.C;
+    __shared__ double red[kBlock / 64 + 2];
+    double local = 0.0;
+    for (size_t e = threadIdx.x; e < n; e += kBlock) local += (double)learned_importance(p, sp, lg, e);
+    mixture_prefix(local, n, P, m.bv, p.eps, p.Bsum, p.R,
+                   [&](size_t e) { return learned_importance(p, sp, lg, e); }, p.prefix + (size_t)slot * n,
+                   p.Q + slot, red);
+}
+
+// Clears the pooled statistics of the listed slots (a proposal restart).
+__global__ void hase_reset_slots_kernel(const uint32_t* list, uint32_t count, double* s1, double* s2, uint32_t* N,
+                                        uint32_t* dropped)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t slot = list[i];
+    s1[slot] = 0.0;
+    s2[slot] = 0.0;
+    N[slot] = 0u;
+    dropped[slot] = 0u;
 }
 
 struct TraceParams {
@@ -559,40 +625,6 @@ __global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* star
 }
 
 // ------------------------------------------------------------------- host
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count)
-    {
-        if (count <= n && p) return;
-        release();
-        if (count == 0) return;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw HaseError(HASE_E_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                               " bytes failed: " + hipGetErrorString(e));
-        }
-        n = count;
-    }
-    void upload(const T* src, size_t count, hipStream_t st)
-    {
-        alloc(count);
-        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, st));
-    }
-};
-
 struct Schedule {
     uint32_t min_rays, max_rays, steps, reps;
     uint32_t target(uint32_t k) const
@@ -607,46 +639,6 @@ struct Schedule {
 };
 
 }  // namespace
-
-struct hase_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t np = 0, T = 0, L = 0, P = 0, S = 0;
-    double h = 0.0;
-    double n_tot = 0.0, tau = 0.0;
-    double clad_alpha = 0.0;
-    bool has_refl = false, has_nidx = false;
-    float nidx[4] = {1, 1, 1, 1};
-    std::vector<double> tri_area;      // [T] device order
-    std::vector<uint32_t> clad_flag;   // [T] device order
-    std::vector<uint32_t> dev2user;    // device triangle -> caller triangle
-    std::vector<uint32_t> user2dev;    // caller triangle -> device triangle
-    std::vector<double> beta_host;     // [P]
-    std::vector<double> beta_cells;    // [S]
-    double Bsum = 0.0;
-    hase_experiment_t exp{};
-    std::vector<double> sigma_a, sigma_e;
-    hase_compute_t comp{};
-    DevBuf<uint4> rec;
-    DevBuf<float2> pts;
-    DevBuf<float> beta, beta_col, bv, refl, c1, c2;
-    DevBuf<float> imp;
-    DevBuf<double> prefix, Q, s1, s2, bcells;
-    DevBuf<double2> partial;
-    DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count;
-    DevBuf<int32_t> active_flag;
-    DevBuf<float> phi;
-    DevBuf<double> mse, se;
-    DevBuf<unsigned long long> counters;
-    hase_stats_t stats{};
-    std::vector<hipEvent_t> events;
-
-    ~hase_context()
-    {
-        for (auto e : events) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace {
 
@@ -696,6 +688,8 @@ void set_params(hase_context* ctx, const hase_experiment_t* e, const hase_comput
     }
     ctx->c1.upload(c1.data(), c1.size(), ctx->stream);
     ctx->c2.upload(c2.data(), c2.size(), ctx->stream);
+    ctx->log_sigma = (e->max_sigma_a > 0.0 && e->max_sigma_e > 0.0)
+                         ? (float)std::log(e->max_sigma_e / e->max_sigma_a) : 0.0f;
 }
 
 void upload_beta(hase_context* ctx, const double* beta_volume, const double* beta_cells)
@@ -727,6 +721,7 @@ void upload_beta(hase_context* ctx, const double* beta_volume, const double* bet
         ctx->beta_cells.assign(ctx->S, 0.0);
         ctx->bcells.upload(ctx->beta_cells.data(), ctx->S, ctx->stream);
     }
+    if (ctx->proposal_kind == HASE_PROPOSAL_LEARNED) build_clusters(ctx);
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -813,6 +808,7 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     std::vector<uint4> rec(3u * (size_t)T);
     ctx->tri_area.resize(T);
     ctx->clad_flag.resize(T);
+    ctx->centroid_host.resize(T);
     for (uint32_t td = 0; td < T; ++td) {
         const uint32_t t = ctx->dev2user[td];
         uint32_t v[3];
@@ -839,8 +835,12 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         rec[3u * td] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
         rec[3u * td + 1u] = make_uint4(fb(px[v[2]]), fb(py[v[2]]), nw[0], nw[1]);
         rec[3u * td + 2u] = make_uint4(nw[2], clad, 0u, 0u);
+        const float x0 = (float)px[v[0]], x1 = (float)px[v[1]], x2 = (float)px[v[2]];
+        const float y0 = (float)py[v[0]], y1 = (float)py[v[1]], y2 = (float)py[v[2]];
+        ctx->centroid_host[td] = make_float2((x0 + x1 + x2) * (1.0f / 3.0f), (y0 + y1 + y2) * (1.0f / 3.0f));
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
+    ctx->centroid.upload(ctx->centroid_host.data(), T, ctx->stream);
     std::vector<float2> pts(np);
     for (uint32_t i = 0; i < np; ++i) pts[i] = make_float2((float)px[i], (float)py[i]);
     ctx->pts.upload(pts.data(), np, ctx->stream);
@@ -862,6 +862,10 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     upload_beta(ctx.get(), mesh->beta_volume, mesh->beta_cells);
     return ctx;
 }
+
+}  // namespace
+
+namespace hase {
 
 MeshDev mesh_view(const hase_context* ctx)
 {
@@ -886,6 +890,10 @@ uint32_t slices(const hase_context* ctx)
     uint32_t K = ctx->exp.use_reflections ? (ctx->exp.max_reflections ? ctx->exp.max_reflections : 1u) : 0u;
     return 1u + 2u * K;
 }
+
+}  // namespace hase
+
+namespace {
 
 hipEvent_t record(hase_context* ctx)
 {
@@ -923,7 +931,10 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     for (auto ev : ctx->events) (void)hipEventDestroy(ev);
     ctx->events.clear();
     ctx->stats = hase_stats_t{};
+    ctx->pstats = hase_proposal_stats_t{};
+    ctx->pstats.clusters = ctx->C;
     if (!(ctx->Bsum > 0.0)) return;   // no emitting volume: the estimate is zero
+    const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
 
     const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
     const uint32_t rays_per_block = kBlock * rpt;
@@ -933,13 +944,14 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     uint32_t batch = c.sample_batch;
     if (batch == 0) {
         // keep the importance prefix (8 B) and scratch (4 B) of a batch within ~6 GiB
-        size_t per_sample = n_ent * 12u + (size_t)max_chunks * 12u + 64u;
+        size_t per_sample = n_ent * 12u + (size_t)max_chunks * 12u + 64u + (learned ? (size_t)R * ctx->C * 4u : 0u);
         size_t budget = (size_t)6 << 30;
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
     batch = std::min(batch, total);
     ctx->imp.alloc((size_t)batch * n_ent);
     ctx->prefix.alloc((size_t)batch * n_ent);
+    if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
     ctx->Q.alloc(batch);
     ctx->s1.alloc(batch);
     ctx->s2.alloc(batch);
@@ -964,7 +976,7 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     std::vector<double> h_mse(batch), h_se(batch);
     std::vector<uint32_t> h_N(batch), h_drop(batch);
     std::vector<int32_t> ones(batch, 1);
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_imp, ev_trace, ev_fin;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_imp, ev_trace, ev_fin, ev_inf, ev_prop;
     hipEvent_t ev_begin = record(ctx);
 
     for (uint32_t b0 = first; b0 <= last; b0 += batch) {
@@ -994,11 +1006,39 @@ void evaluate(hase_context* ctx, hase_result_t* res)
         ip.prefix = ctx->prefix.p;
         ip.Q = ctx->Q.p;
         ip.counters = ctx->counters.p;
-        hipEvent_t i0 = record(ctx);
-        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
-        HIP_CHECK(hipGetLastError());
-        ev_imp.emplace_back(i0, record(ctx));
-        ctx->stats.importance_launches++;
+        ip.active = nullptr;
+        ip.transfer = 0;
+        if (learned) {
+            hipEvent_t a0 = record(ctx);
+            launch_proposal_inference(ctx, b0, slots, ctx->logits.p);
+            hipEvent_t a1 = record(ctx);
+            LearnedParams lp;
+            lp.m = m;
+            lp.pts = ctx->pts.p;
+            lp.centroid = ctx->centroid.p;
+            lp.logits = ctx->logits.p;
+            lp.slots = slots;
+            lp.sample_base = b0;
+            lp.R = R;
+            lp.C = ctx->C;
+            lp.TG = ctx->TG;
+            lp.ct = ctx->cluster_t;
+            lp.cl = ctx->cluster_l;
+            lp.eps = e.defensive_fraction;
+            lp.Bsum = ctx->Bsum;
+            lp.prefix = ctx->prefix.p;
+            lp.Q = ctx->Q.p;
+            hipLaunchKernelGGL(hase_learned_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, lp);
+            HIP_CHECK(hipGetLastError());
+            ev_inf.emplace_back(a0, a1);
+            ev_prop.emplace_back(a1, record(ctx));
+        } else {
+            hipEvent_t i0 = record(ctx);
+            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+            HIP_CHECK(hipGetLastError());
+            ev_imp.emplace_back(i0, record(ctx));
+            ctx->stats.importance_launches++;
+        }
 
         uint32_t active = slots;
         uint32_t N = 0, k = 0, extra = 0, launch = 0;
@@ -1077,6 +1117,25 @@ void evaluate(hase_context* ctx, hase_result_t* res)
             N += nrays;
             launch++;
             if (target < sched.max_rays) k++;
+            if (learned && ctx->proposal_fallback && launch == 1u && active > 0u) {
+                // samples the learned proposal did not settle with the first budget restart on the
+                // walked importance map; their learned rays are discarded
+                ImportanceParams fb = ip;
+                fb.active = ctx->active_list.p;
+                hipEvent_t i0 = record(ctx);
+                hipLaunchKernelGGL(hase_importance_kernel, dim3(active), dim3(kBlock), 0, ctx->stream, fb);
+                HIP_CHECK(hipGetLastError());
+                ev_imp.emplace_back(i0, record(ctx));
+                ctx->stats.importance_launches++;
+                hipLaunchKernelGGL(hase_reset_slots_kernel, dim3((active + 255u) / 256u), dim3(256), 0, ctx->stream,
+                                   (const uint32_t*)ctx->active_list.p, active, ctx->s1.p, ctx->s2.p, ctx->N.p,
+                                   ctx->dropped.p);
+                HIP_CHECK(hipGetLastError());
+                ctx->pstats.fallback_samples += active;
+                N = 0;
+                k = 0;
+                extra = 0;
+            }
         }
         HIP_CHECK(hipMemcpyAsync(h_phi.data(), ctx->phi.p, slots * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipMemcpyAsync(h_mse.data(), ctx->mse.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1102,6 +1161,8 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     for (auto& pr : ev_imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
     for (auto& pr : ev_trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
     for (auto& pr : ev_fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : ev_inf) ctx->pstats.inference_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : ev_prop) ctx->pstats.proposal_ms += elapsed(pr.first, pr.second);
     ctx->stats.total_ms = elapsed(ev_begin, ev_end);
     unsigned long long cnt[4];
     HIP_CHECK(hipMemcpy(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -1115,29 +1176,11 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     ctx->stats.dropped_rays = dropped;
 }
 
-template <class F>
-int guarded(F&& f)
-{
-    try {
-        g_last_error.clear();
-        f();
-        return HASE_OK;
-    } catch (const HaseError& e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
-        return HASE_E_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return HASE_E_INTERNAL;
-    } catch (...) {
-        g_last_error = "unknown failure";
-        return HASE_E_INTERNAL;
-    }
-}
-
 }  // namespace
+
+namespace hase {
+thread_local std::string g_last_error;
+}
 
 extern "C" {
 
@@ -1277,6 +1320,8 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
         ip.prefix = context->prefix.p;
         ip.Q = context->Q.p;
         ip.counters = context->counters.p;
+        ip.active = nullptr;
+        ip.transfer = 0;
         hipLaunchKernelGGL(hase_importance_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
         HIP_CHECK(hipGetLastError());
         std::vector<float> dev(n_ent);
@@ -1291,7 +1336,138 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
     });
 }
 
-const char* hase_last_error(void) { return g_last_error.c_str(); }
+int hase_context_set_proposal(hase_context_t* context, const hase_proposal_t* proposal)
+{
+    return guarded([&] {
+        if (!context || !proposal) throw HaseError(HASE_E_INVALID, "context and proposal must not be NULL");
+        if (proposal->kind == HASE_PROPOSAL_PLAIN) {
+            context->proposal_kind = HASE_PROPOSAL_PLAIN;
+            return;
+        }
+        if (proposal->kind != HASE_PROPOSAL_LEARNED) throw HaseError(HASE_E_INVALID, "unknown proposal kind");
+        if (!proposal->params || !proposal->input_shift || !proposal->input_scale)
+            throw HaseError(HASE_E_INVALID, "a learned proposal needs params, input_shift and input_scale");
+        if (proposal->cluster_triangles == 0 || proposal->cluster_levels == 0)
+            throw HaseError(HASE_E_INVALID, "cluster_triangles and cluster_levels must be at least 1");
+        for (size_t i = 0; i < kMlpParams; ++i)
+            if (!std::isfinite(proposal->params[i])) throw HaseError(HASE_E_INVALID, "params must be finite");
+        for (int i = 0; i < kMlpIn; ++i)
+            if (!std::isfinite(proposal->input_shift[i]) || !std::isfinite(proposal->input_scale[i]))
+                throw HaseError(HASE_E_INVALID, "input_shift and input_scale must be finite");
+        HIP_CHECK(hipSetDevice(context->device));
+        std::vector<uint4> frags((size_t)kFrags * 64);
+        std::vector<float> vec(kMlpVec);
+        pack_mlp_bf16(proposal->params, frags.data(), vec.data());
+        context->mlp_frags.upload(frags.data(), frags.size(), context->stream);
+        context->mlp_vec.upload(vec.data(), vec.size(), context->stream);
+        for (int i = 0; i < kMlpIn; ++i) {
+            context->in_shift[i] = proposal->input_shift[i];
+            context->in_scale[i] = proposal->input_scale[i];
+        }
+        context->cluster_t = std::min(proposal->cluster_triangles, context->T);
+        context->cluster_l = std::min(proposal->cluster_levels, context->L - 1);
+        context->proposal_fallback = proposal->fallback != 0;
+        build_clusters(context);
+        context->proposal_kind = HASE_PROPOSAL_LEARNED;
+        HIP_CHECK(hipStreamSynchronize(context->stream));
+    });
+}
+
+int hase_context_transfer_map(hase_context_t* context, uint32_t first, uint32_t count, float* transfer,
+                              size_t capacity)
+{
+    return guarded([&] {
+        if (!context || !transfer) throw HaseError(HASE_E_INVALID, "context and transfer must not be NULL");
+        if (count == 0 || first >= context->S || count > context->S - first)
+            throw HaseError(HASE_E_INVALID, "sample range out of bounds");
+        const uint32_t R = slices(context), T = context->T, P = context->P;
+        const size_t n_ent = (size_t)R * P;
+        if (capacity < (size_t)count * n_ent) throw HaseError(HASE_E_INVALID, "transfer capacity must be count * R * P");
+        HIP_CHECK(hipSetDevice(context->device));
+        const uint32_t batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(count, ((size_t)1 << 28) / n_ent));
+        context->imp.alloc(std::max(context->imp.n, (size_t)batch * n_ent));
+        context->counters.alloc(4);
+        const hase_experiment_t& e = context->exp;
+        std::vector<float> dev((size_t)batch * n_ent);
+        for (uint32_t b0 = 0; b0 < count; b0 += batch) {
+            const uint32_t slots = std::min(batch, count - b0);
+            ImportanceParams ip;
+            ip.m = mesh_view(context);
+            ip.pts = context->pts.p;
+            ip.sample_base = first + b0;
+            ip.slots = slots;
+            ip.R = R;
+            ip.n_lambda = e.number_of_wavelengths;
+            ip.c1 = context->c1.p;
+            ip.c2 = context->c2.p;
+            ip.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+            ip.eps = e.defensive_fraction;
+            ip.Bsum = context->Bsum;
+            ip.imp = context->imp.p;
+            ip.prefix = nullptr;
+            ip.Q = nullptr;
+            ip.counters = context->counters.p;
+            ip.active = nullptr;
+            ip.transfer = 1;
+            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, context->stream, ip);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, (size_t)slots * n_ent * sizeof(float),
+                                     hipMemcpyDeviceToHost, context->stream));
+            HIP_CHECK(hipStreamSynchronize(context->stream));
+            for (uint32_t i = 0; i < slots; ++i) {
+                const float* src = dev.data() + (size_t)i * n_ent;
+                float* dst = transfer + (size_t)(b0 + i) * n_ent;
+                for (size_t x = 0; x < n_ent; ++x) {
+                    const size_t k = x / P, j = x % P;
+                    dst[k * P + context->dev2user[j % T] + (j / T) * (size_t)T] = src[x];
+                }
+            }
+        }
+    });
+}
+
+int hase_context_proposal_logits(hase_context_t* context, uint32_t first, uint32_t count, float* logits,
+                                 size_t capacity, float* clusters, uint32_t* n_clusters)
+{
+    return guarded([&] {
+        if (!context || !logits) throw HaseError(HASE_E_INVALID, "context and logits must not be NULL");
+        if (context->proposal_kind != HASE_PROPOSAL_LEARNED)
+            throw HaseError(HASE_E_INVALID, "no learned proposal is set on this context");
+        if (count == 0 || first >= context->S || count > context->S - first)
+            throw HaseError(HASE_E_INVALID, "sample range out of bounds");
+        const uint32_t R = slices(context), C = context->C;
+        const size_t per = (size_t)R * C;
+        if (capacity < (size_t)count * per) throw HaseError(HASE_E_INVALID, "logits capacity must be count * R * C");
+        HIP_CHECK(hipSetDevice(context->device));
+        const uint32_t batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(count, ((size_t)1 << 26) / per));
+        context->logits.alloc(std::max(context->logits.n, (size_t)batch * per));
+        for (uint32_t b0 = 0; b0 < count; b0 += batch) {
+            const uint32_t slots = std::min(batch, count - b0);
+            launch_proposal_inference(context, first + b0, slots, context->logits.p);
+            HIP_CHECK(hipMemcpyAsync(logits + (size_t)b0 * per, context->logits.p, (size_t)slots * per * sizeof(float),
+                                     hipMemcpyDeviceToHost, context->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(context->stream));
+        if (clusters)
+            for (uint32_t c = 0; c < C; ++c) {
+                clusters[4 * c] = context->clusters_host[c].x;
+                clusters[4 * c + 1] = context->clusters_host[c].y;
+                clusters[4 * c + 2] = context->clusters_host[c].z;
+                clusters[4 * c + 3] = context->clusters_host[c].w;
+            }
+        if (n_clusters) *n_clusters = C;
+    });
+}
+
+int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_stats_t* stats)
+{
+    return guarded([&] {
+        if (!context || !stats) throw HaseError(HASE_E_INVALID, "context and stats must not be NULL");
+        *stats = context->pstats;
+    });
+}
+
+const char* hase_last_error(void) { return hase::g_last_error.c_str(); }
 
 int hase_abi_version(void) { return HASE_ABI_VERSION; }
 

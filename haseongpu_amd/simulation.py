@@ -21,7 +21,7 @@ from . import _native
 from .geometry import GainMedium
 from .laser import CrossSectionData
 
-__all__ = ["PhiASE", "Result"]
+__all__ = ["PhiASE", "Result", "native_problem"]
 
 
 @dataclass
@@ -72,6 +72,7 @@ class PhiASE:
     defensiveFraction: float = 0.05
     raysPerThread: int = 0
     sampleBatch: int = 0
+    proposal: object | None = None   # a proposal.LearnedProposal; None = walked importance map
 
     _result: Result | None = field(default=None, init=False, repr=False)
 
@@ -129,7 +130,8 @@ class PhiASE:
             if not float(medium.value(name)) > 0.0:
                 raise ValueError(f"GainMedium.{name} must be positive")
 
-    def run(self, gainMedium=None, crossSections=None):
+    def nativeArguments(self, gainMedium=None, crossSections=None):
+        """(mesh, experiment, compute) C-ABI structures for a medium and spectra."""
         medium = gainMedium if gainMedium is not None else self.gainMedium
         spectra = crossSections if crossSections is not None else self.crossSections
         self._validate(medium, spectra)
@@ -150,9 +152,17 @@ class PhiASE:
                                     max_sample_range=None if self.maxSampleRange is None else int(self.maxSampleRange),
                                     rng_seed=seed, device=int(self.device), rays_per_thread=int(self.raysPerThread),
                                     sample_batch=int(self.sampleBatch))
+        return mesh, exp, comp
+
+    def run(self, gainMedium=None, crossSections=None):
+        mesh, exp, comp = self.nativeArguments(gainMedium, crossSections)
         with _native.Context(mesh, exp, comp) as ctx:
+            if self.proposal is not None:
+                ctx.set_proposal(self.proposal)
             res = ctx.evaluate()
             stats = ctx.stats()
+            if self.proposal is not None:
+                stats["proposal"] = ctx.proposal_stats()
         rse = np.sqrt(np.where(np.isfinite(res.mse) & (res.mse >= 0), res.mse, np.nan))
         self._result = Result(phiAse=res.phiAse, mse=res.mse, totalRays=res.totalRays, dndtAse=res.dndtAse,
                               standardError=res.standardError, relativeStandardError=rse,
@@ -163,3 +173,11 @@ class PhiASE:
         if self._result is None:
             raise RuntimeError("PhiASE.run() has not been called")
         return self._result
+
+
+def native_problem(workload, device=0, **overrides):
+    """C-ABI structures of a seeded workload plus log(max_sigma_e / max_sigma_a)."""
+    phi = PhiASE(crossSections=workload.spectra, **{**workload.settings, "device": device, **overrides})
+    mesh, exp, comp = phi.nativeArguments(workload.medium, workload.spectra)
+    log_sigma = float(np.log(exp.struct.max_sigma_e / exp.struct.max_sigma_a))
+    return mesh, exp, comp, log_sigma

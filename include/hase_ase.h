@@ -32,13 +32,14 @@
 #ifndef HASE_ASE_H
 #define HASE_ASE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
-#define HASE_ABI_VERSION 1
+#define HASE_ABI_VERSION 2
 
 enum hase_status {
     HASE_OK = 0,
@@ -197,6 +198,96 @@ int hase_context_importance(hase_context_t* context,
                             uint32_t sample,
                             float* importance,
                             uint32_t capacity);
+
+/* ------------------------------------------------------------------------
+ * Learned proposal.  The reference has no counterpart (its importance map is
+ * the centroid-propagation pass, retired shell
+ * include/kernels/importanceSampling.hpp:10-18); these entry points extend
+ * the map above.
+ *
+ * Network: HASE_MLP_INPUTS inputs -> 4 hidden layers of HASE_MLP_HIDDEN
+ * SiLU units -> 1 output, the natural log of the transfer factor (spectral
+ * gain times reflectance) from a prism centroid to a sample point.  The
+ * importance of prism j for sample s is then beta_j V_j exp(y) / (4 pi d^2).
+ * Inputs, in order: sample (or mirror image) x, y, z; prism centroid x, y, z
+ * (cm); level; beta; centroid-to-sample distance (cm); log(max_sigma_e /
+ * max_sigma_a).  Parameters are FP32, row-major W[out][in] per layer, in the
+ * order W1 b1 W2 b2 W3 b3 W4 b4 w5 b5 (hase_mlp_param_count() floats).
+ * Every hase_mlp_* call takes inputs already normalised by the caller.
+ * ---------------------------------------------------------------------- */
+#define HASE_MLP_INPUTS 10
+#define HASE_MLP_HIDDEN 128
+
+#define HASE_PROPOSAL_PLAIN 0
+#define HASE_PROPOSAL_LEARNED 1
+
+typedef struct hase_train {
+    uint32_t epochs;
+    uint32_t batch;                 /* rows per Adam step */
+    float learning_rate;
+    float beta1, beta2, epsilon;    /* Adam moments */
+    float weight_decay;             /* decoupled (AdamW) */
+    uint32_t seed;                  /* row shuffling */
+} hase_train_t;
+
+typedef struct hase_proposal {
+    int32_t kind;                   /* HASE_PROPOSAL_PLAIN or HASE_PROPOSAL_LEARNED */
+    const float* params;            /* [hase_mlp_param_count()] FP32 weights (kept as BF16 on the device) */
+    const float* input_shift;       /* [HASE_MLP_INPUTS] input normalisation: (x - shift) * scale */
+    const float* input_scale;       /* [HASE_MLP_INPUTS] */
+    uint32_t cluster_triangles;     /* triangles per evaluated cluster (device order), >= 1 */
+    uint32_t cluster_levels;        /* prism layers per evaluated cluster, >= 1 */
+    int32_t fallback;               /* 1: samples that miss mse_threshold after the first
+                                       launch restart on the plain importance map */
+} hase_proposal_t;
+
+/* Counters of the learned proposal during the last evaluation. */
+typedef struct hase_proposal_stats {
+    double inference_ms;            /* BF16 network kernel */
+    double proposal_ms;             /* per-prism proposal and prefix from the network output */
+    uint64_t evaluations;           /* network rows evaluated */
+    uint32_t fallback_samples;      /* samples moved to the plain map */
+    uint32_t clusters;              /* clusters per reflection slice */
+} hase_proposal_stats_t;
+
+uint32_t hase_mlp_param_count(void);
+
+/* FP32 forward pass on the device: outputs[i] = net(inputs[i*HASE_MLP_INPUTS ..]). */
+int hase_mlp_forward(const float* params, uint32_t n, const float* inputs, float* outputs, int32_t device);
+
+/* Weighted squared error L = sum w_i (y_i - t_i)^2 / sum w_i (weights NULL:
+ * all 1) and its gradient with respect to params, by the device backward
+ * kernels. */
+int hase_mlp_loss_gradient(const float* params, uint32_t n, const float* inputs, const float* targets,
+                           const float* weights, double* loss, float* gradient, int32_t device);
+
+/* Adam(W) training on the device; params are updated in place and the mean
+ * batch loss of every epoch is written to epoch_loss[epochs] (may be NULL). */
+int hase_mlp_train(float* params, uint32_t n, const float* inputs, const float* targets, const float* weights,
+                   const hase_train_t* config, float* epoch_loss, int32_t device);
+
+/* BF16-MFMA inference (FP32 accumulate) on explicit rows, the arithmetic
+ * the learned proposal uses. */
+int hase_mlp_forward_bf16(const float* params, uint32_t n, const float* inputs, float* outputs, int32_t device);
+
+/* Select the proposal of later evaluations (kind PLAIN restores the map). */
+int hase_context_set_proposal(hase_context_t* context, const hase_proposal_t* proposal);
+
+/* Transfer factor (spectral gain times reflectance) of the centroid walk of
+ * every (reflection slice, prism) pair for samples [first, first+count):
+ * transfer[(s*R + k)*P + j], caller prism order; the network's training
+ * target is its logarithm.  Non-walkable pairs are written as 0. */
+int hase_context_transfer_map(hase_context_t* context, uint32_t first, uint32_t count, float* transfer,
+                              size_t capacity);
+
+/* Network outputs of the learned proposal for samples [first, first+count):
+ * logits[(s*R + k)*C + c], C clusters per slice; clusters (may be NULL)
+ * receives C rows of (x, y, level, beta) of each cluster's representative
+ * prism (area-weighted centroid, mean level, volume-weighted beta). */
+int hase_context_proposal_logits(hase_context_t* context, uint32_t first, uint32_t count, float* logits,
+                                 size_t capacity, float* clusters, uint32_t* n_clusters);
+
+int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_stats_t* stats);
 
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);

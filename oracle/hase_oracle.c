@@ -531,7 +531,12 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
 #endif
         for (int64_t si = (int64_t)prm->sample_min; si <= (int64_t)prm->sample_max; ++si) {
             uint32_t s = (uint32_t)si;
-            orc_importance(m, prm, s, imp);
+            if (prm->importance) {
+                const double* src = prm->importance + (size_t)(s - prm->sample_min) * n;
+                for (size_t e = 0; e < n; ++e) imp[e] = src[e];
+            } else {
+                orc_importance(m, prm, s, imp);
+            }
             if (orc_proposal_prefix(m, prm, imp, order, prefix) != 0) continue; /* no source */
             double s1 = 0.0, s2 = 0.0;
             uint32_t N = 0, drop = 0, launch = 0, k = 0, extra = 0;
@@ -555,6 +560,13 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
                 int ok = drop == 0 && isfinite(rse) && rse <= prm->mse_threshold;
                 if (ok) break;
                 if (target < prm->max_rays) k++;
+                if (prm->importance && prm->fallback && launch == 1) {
+                    /* external proposal missed the threshold: restart on the walked map */
+                    orc_importance(m, prm, s, imp);
+                    if (orc_proposal_prefix(m, prm, imp, order, prefix) != 0) break;
+                    s1 = 0.0; s2 = 0.0; N = 0; drop = 0; k = 0; extra = 0;
+                    rse = DBL_MAX;
+                }
             }
             double ph = scale * s1 / (double)N;
             phi[s] = ph;

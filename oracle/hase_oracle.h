@@ -71,6 +71,10 @@ typedef struct {
     double n_tot, tau;
     const double* beta_cells;   /* [np*nl] */
     int32_t threads;            /* OpenMP threads (<=0: runtime default) */
+    const double* importance;   /* NULL: walked map; else an external proposal importance,
+                                   [(s - sample_min) * R*P + k*P + j], caller prism order */
+    int32_t fallback;           /* with importance: samples that miss the threshold after the
+                                   first launch restart on the walked map */
 } orc_params;
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

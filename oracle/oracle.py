@@ -48,6 +48,7 @@ class _Params(C.Structure):
         ("adaptive_steps", C.c_uint32), ("sample_min", C.c_uint32),
         ("sample_max", C.c_uint32), ("seed", C.c_uint32), ("n_tot", C.c_double),
         ("tau", C.c_double), ("beta_cells", _f64p), ("threads", C.c_int32),
+        ("importance", _f64p), ("fallback", C.c_int32),
     ]
 
 
@@ -126,7 +127,8 @@ class OracleParams:
     def __init__(self, *, sigma_a, sigma_e, n_tot, tau, beta_cells, min_rays, max_rays=None,
                  max_sigma_a=None, max_sigma_e=None, mse_threshold=0.005, use_reflections=False,
                  max_reflections=1, defensive=0.05, max_repetitions=1, adaptive_steps=1,
-                 sample_min=0, sample_max=None, seed=5489, threads=0, n_samples=None):
+                 sample_min=0, sample_max=None, seed=5489, threads=0, n_samples=None, importance=None,
+                 fallback=False):
         self.sa = np.ascontiguousarray(np.atleast_1d(sigma_a), dtype=np.float64)
         self.se = np.ascontiguousarray(np.atleast_1d(sigma_e), dtype=np.float64)
         self.bc = np.ascontiguousarray(beta_cells, dtype=np.float64)
@@ -135,10 +137,12 @@ class OracleParams:
         mse_ = float(self.se.max()) if max_sigma_e is None else float(max_sigma_e)
         n_samples = self.bc.size if n_samples is None else n_samples
         smax = n_samples - 1 if sample_max is None else sample_max
+        self.imp = None if importance is None else np.ascontiguousarray(importance, dtype=np.float64)
         self.s = _Params(min_rays, max_rays, self.sa.size, _p(self.sa, C.c_double), _p(self.se, C.c_double),
                          msa, mse_, mse_threshold, int(bool(use_reflections)), max_reflections, defensive,
                          max_repetitions, adaptive_steps, sample_min, smax, seed, n_tot, tau,
-                         _p(self.bc, C.c_double), threads)
+                         _p(self.bc, C.c_double), threads,
+                         None if self.imp is None else _p(self.imp, C.c_double), int(bool(fallback)))
 
 
 def philox4x32_10(ctr, key):

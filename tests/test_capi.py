@@ -29,13 +29,15 @@ def test_library_exports_every_declared_symbol():
     assert set(declared) == set(_native.EXPORTED_SYMBOLS)
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.hase_abi_version() == 1
+    assert lib.hase_abi_version() == 2
     assert b"gfx950" in lib.hase_build_info()
 
 
 def test_struct_layouts_match_header(tmp_path):
     structs = {"hase_mesh_t": _native.Mesh, "hase_experiment_t": _native.Experiment,
-               "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats}
+               "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats,
+               "hase_train_t": _native.Train, "hase_proposal_t": _native.Proposal,
+               "hase_proposal_stats_t": _native.ProposalStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():
@@ -96,3 +98,31 @@ def test_null_arguments_are_rejected():
     lib.hase_context_destroy(None)
     with pytest.raises(_native.HaseError):
         _native.check(-1)
+
+
+def test_network_layout_constants():
+    lib = _native.load()
+    # W1 b1 (10 -> 128), three 128 x 128 layers, w5 b5
+    assert lib.hase_mlp_param_count() == 128 * 10 + 128 + 3 * (128 * 128 + 128) + 128 + 1
+    text = HEADER.read_text()
+    assert f"#define HASE_MLP_INPUTS {_native.MLP_INPUTS}" in text
+    assert f"#define HASE_MLP_HIDDEN {_native.MLP_HIDDEN}" in text
+
+
+def test_network_entry_points_reject_bad_arguments_on_the_host():
+    lib = _native.load()
+    p = np.zeros(lib.hase_mlp_param_count(), np.float32)
+    pp = p.ctypes.data_as(C.POINTER(C.c_float))
+    y = np.zeros(4, np.float32)
+    yp = y.ctypes.data_as(C.POINTER(C.c_float))
+    assert lib.hase_mlp_forward(pp, 4, None, yp, 0) == -1
+    assert lib.hase_mlp_forward_bf16(pp, 0, yp, yp, 0) == -1
+    assert lib.hase_mlp_loss_gradient(pp, 4, yp, None, None, None, None, 0) == -1
+    cfg = _native.Train(0, 16, 1e-3, 0.9, 0.999, 1e-8, 0.0, 0)
+    x = np.zeros((4, 10), np.float32)
+    assert lib.hase_mlp_train(pp, 4, x.ctypes.data_as(C.POINTER(C.c_float)), yp, None, C.byref(cfg), None, 0) == -1
+    assert b"epochs" in lib.hase_last_error()
+    assert lib.hase_context_set_proposal(None, None) == -1
+    assert lib.hase_context_transfer_map(None, 0, 1, yp, 4) == -1
+    assert lib.hase_context_proposal_logits(None, 0, 1, yp, 4, None, None) == -1
+    assert lib.hase_context_proposal_stats(None, None) == -1

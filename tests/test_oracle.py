@@ -11,6 +11,7 @@ import pytest
 
 from haseongpu_amd.geometry import GainMedium, Grid, MeshTopology
 from haseongpu_amd.laser import CrossSectionData
+from haseongpu_amd.workloads import plain_small
 from oracle import oracle as O
 from tests.helpers import load_compare_serial, load_true_cladding
 from tests.quadrature import box_point_flux
@@ -300,3 +301,25 @@ def test_cladding_absorption_lowers_flux_monotonically():
         topo, med, sp, ref = load_true_cladding(idx)
         means.append(ref["phiASE"].mean())
     assert means[0] > means[1] > means[2]
+
+
+def test_external_importance_reproduces_the_walked_run():
+    """The oracle's external-proposal input, fed its own walked map, gives the
+    walked run bit for bit (the learned-proposal parity tests feed it the
+    network's map through the same door)."""
+    w = plain_small(rays=2000)
+    om = O.OracleMesh(w.topology, w.medium)
+    kw = dict(sigma_a=[1.16e-21], sigma_e=[2.48e-20], n_tot=2.76e20, tau=9.5e-4,
+              beta_cells=w.medium.value("betaCells"), min_rays=2000, sample_min=100, sample_max=139, seed=3,
+              n_samples=w.samples)
+    base = O.calc_phi_ase(om, O.OracleParams(**kw))
+    imp = np.stack([O.importance(om, O.OracleParams(**kw), s) for s in range(100, 140)])
+    ext = O.calc_phi_ase(om, O.OracleParams(**kw, importance=imp))
+    np.testing.assert_array_equal(ext["phiAse"], base["phiAse"])
+    np.testing.assert_array_equal(ext["mse"], base["mse"])
+    # a flat (beta*V only) external proposal changes the estimate but not its mean
+    flat = O.calc_phi_ase(om, O.OracleParams(**kw, importance=np.zeros_like(imp)))
+    sl = slice(100, 140)
+    assert not np.array_equal(flat["phiAse"][sl], base["phiAse"][sl])
+    z = (flat["phiAse"][sl] - base["phiAse"][sl]) / (base["phiAse"][sl] * np.sqrt(flat["mse"][sl] + base["mse"][sl]))
+    assert abs(np.mean(z)) < 1.0 and np.all(np.abs(z) < 5)
