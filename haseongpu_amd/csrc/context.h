@@ -35,7 +35,7 @@ struct hase_context {
     hase::DevBuf<float> imp;
     hase::DevBuf<double> prefix, Q, s1, s2, bcells;
     hase::DevBuf<double2> partial;
-    hase::DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count;
+    hase::DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count, slot_sample;
     hase::DevBuf<int32_t> active_flag;
     hase::DevBuf<float> phi;
     hase::DevBuf<double> mse, se;

@@ -144,6 +144,7 @@ struct ImportanceParams {
     double* Q;                // [slots]
     unsigned long long* counters;
     const uint32_t* active;   // block -> slot (nullptr: block index is the slot)
+    const uint32_t* sample_of;// slot -> sample (nullptr: sample_base + slot)
     int transfer;             // write the transfer factor gain * reflectance instead of the importance
 };
 
@@ -310,7 +311,7 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     const MeshDev& m = p.m;
     const uint32_t T = m.T, P = T * m.layers;
     const size_t n = (size_t)p.R * P;
-    const uint32_t sample = p.sample_base + slot;
+    const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
     SampleGeom sp = sample_point(p.pts, m.np, sample);
     float* imp = p.imp + (size_t)slot * n;
     double* pre = p.prefix + (size_t)slot * n;
@@ -413,23 +414,11 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
                    p.Q + slot, red);
 }
 
-// Clears the pooled statistics of the listed slots (a proposal restart).
-__global__ void hase_reset_slots_kernel(const uint32_t* list, uint32_t count, double* s1, double* s2, uint32_t* N,
-                                        uint32_t* dropped)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t slot = list[i];
-    s1[slot] = 0.0;
-    s2[slot] = 0.0;
-    N[slot] = 0u;
-    dropped[slot] = 0u;
-}
-
 struct TraceParams {
     MeshDev m;
     const float2* pts;
     uint32_t sample_base;
+    const uint32_t* sample_of;// slot -> sample (nullptr: sample_base + slot)
     uint32_t R;
     uint32_t n_lambda;
     const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a)
@@ -465,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
     const MeshDev& m = p.m;
     const uint32_t T = m.T, P = T * m.layers;
     const size_t n = (size_t)p.R * P;
-    const uint32_t sample = p.sample_base + slot;
+    const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
     const SampleGeom sp = sample_point(p.pts, m.np, sample);
     const double* pre = p.prefix + (size_t)slot * n;
     const double Q = p.Q[slot];
@@ -560,8 +549,6 @@ struct FinalizeParams {
     uint32_t* active_count;
     double mse_threshold;
     double phi_scale;           // nTot / tau
-    const double* beta_cells;   // [slots] of this batch
-    double dndt_a, dndt_b;      // dndt = (beta * a - b) * phi
     float* phi;
     double* mse;
     double* se;
@@ -911,6 +898,202 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// One batch of resident samples: proposal (walked map or learned), then
+// adaptive trace launches until every slot converges or reaches its budget.
+// Slot i holds sample samples[i] (or b0 + i when samples is null).  With the
+// learned proposal and `deferred`, samples that miss the threshold after the
+// first launch are handed back (they restart on the walked map later, from
+// launch index 1, so their random streams stay distinct).
+struct BatchTimers {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> imp, trace, fin, inf, prop;
+};
+
+void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_t* samples, uint32_t slots,
+               bool learned, uint32_t launch0, std::vector<uint32_t>* deferred, BatchTimers& tm)
+{
+    const hase_compute_t& c = ctx->comp;
+    const hase_experiment_t& e = ctx->exp;
+    const uint32_t R = slices(ctx);
+    const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
+    const uint32_t rays_per_block = kBlock * rpt;
+    const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
+    const MeshDev m = mesh_view(ctx);
+    const Schedule sched{e.min_rays_per_sample, e.max_rays_per_sample, c.adaptive_steps, c.max_repetitions};
+    std::vector<uint32_t> iota(slots);
+    for (uint32_t i = 0; i < slots; ++i) iota[i] = i;
+    std::vector<int32_t> ones(slots, 1);
+    const uint32_t* sample_of = nullptr;
+    if (samples) {
+        HIP_CHECK(hipMemcpyAsync(ctx->slot_sample.p, samples, slots * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        sample_of = ctx->slot_sample.p;
+    }
+    HIP_CHECK(hipMemsetAsync(ctx->s1.p, 0, slots * sizeof(double), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->s2.p, 0, slots * sizeof(double), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->N.p, 0, slots * sizeof(uint32_t), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->dropped.p, 0, slots * sizeof(uint32_t), ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(ctx->active_flag.p, ones.data(), slots * sizeof(int32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(ctx->active_list.p, iota.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+
+    if (learned) {
+        hipEvent_t a0 = record(ctx);
+        launch_proposal_inference(ctx, b0, slots, ctx->logits.p);
+        hipEvent_t a1 = record(ctx);
+        LearnedParams lp;
+        lp.m = m;
+        lp.pts = ctx->pts.p;
+        lp.centroid = ctx->centroid.p;
+        lp.logits = ctx->logits.p;
+        lp.slots = slots;
+        lp.sample_base = b0;
+        lp.R = R;
+        lp.C = ctx->C;
+        lp.TG = ctx->TG;
+        lp.ct = ctx->cluster_t;
+        lp.cl = ctx->cluster_l;
+        lp.eps = e.defensive_fraction;
+        lp.Bsum = ctx->Bsum;
+        lp.prefix = ctx->prefix.p;
+        lp.Q = ctx->Q.p;
+        hipLaunchKernelGGL(hase_learned_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, lp);
+        HIP_CHECK(hipGetLastError());
+        tm.inf.emplace_back(a0, a1);
+        tm.prop.emplace_back(a1, record(ctx));
+    } else {
+        ImportanceParams ip;
+        ip.m = m;
+        ip.pts = ctx->pts.p;
+        ip.sample_base = b0;
+        ip.slots = slots;
+        ip.R = R;
+        ip.n_lambda = e.number_of_wavelengths;
+        ip.c1 = ctx->c1.p;
+        ip.c2 = ctx->c2.p;
+        ip.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+        ip.eps = e.defensive_fraction;
+        ip.Bsum = ctx->Bsum;
+        ip.imp = ctx->imp.p;
+        ip.prefix = ctx->prefix.p;
+        ip.Q = ctx->Q.p;
+        ip.counters = ctx->counters.p;
+        ip.active = nullptr;
+        ip.sample_of = sample_of;
+        ip.transfer = 0;
+        hipEvent_t i0 = record(ctx);
+        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+        HIP_CHECK(hipGetLastError());
+        tm.imp.emplace_back(i0, record(ctx));
+        ctx->stats.importance_launches++;
+    }
+
+    uint32_t active = slots;
+    uint32_t N = 0, k = 0, extra = 0, launch = launch0;
+    while (active > 0) {
+        const uint32_t target = sched.target(k);
+        uint32_t nrays;
+        if (target > N) {
+            nrays = target - N;
+        } else if (N >= sched.max_rays || target >= sched.max_rays) {
+            if (extra + 1u >= sched.reps) break;
+            extra++;
+            nrays = sched.max_rays;
+        } else {
+            k++;
+            continue;
+        }
+        const uint32_t chunks = (nrays + rays_per_block - 1) / rays_per_block;
+        TraceParams tp;
+        tp.m = m;
+        tp.pts = ctx->pts.p;
+        tp.sample_base = b0;
+        tp.sample_of = sample_of;
+        tp.R = R;
+        tp.n_lambda = e.number_of_wavelengths;
+        tp.c1 = ctx->c1.p;
+        tp.c2 = ctx->c2.p;
+        tp.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
+        tp.seed = c.rng_seed;
+        tp.launch = launch;
+        tp.nrays = nrays;
+        tp.chunks = chunks;
+        tp.rays_per_thread = rpt;
+        tp.active = ctx->active_list.p;
+        tp.prefix = ctx->prefix.p;
+        tp.Q = ctx->Q.p;
+        tp.partial = ctx->partial.p;
+        tp.partial_drop = ctx->partial_drop.p;
+        tp.max_chunks = max_chunks;
+        tp.counters = ctx->counters.p;
+        hipEvent_t t0 = record(ctx);
+        hipLaunchKernelGGL(hase_trace_kernel, dim3(active * chunks), dim3(kBlock), 0, ctx->stream, tp);
+        HIP_CHECK(hipGetLastError());
+        tm.trace.emplace_back(t0, record(ctx));
+        ctx->stats.trace_launches++;
+
+        HIP_CHECK(hipMemsetAsync(ctx->active_count.p, 0, sizeof(uint32_t), ctx->stream));
+        FinalizeParams fp;
+        fp.slots = slots;
+        fp.chunks = chunks;
+        fp.max_chunks = max_chunks;
+        fp.nrays = nrays;
+        fp.partial = ctx->partial.p;
+        fp.partial_drop = ctx->partial_drop.p;
+        fp.s1 = ctx->s1.p;
+        fp.s2 = ctx->s2.p;
+        fp.N = ctx->N.p;
+        fp.dropped = ctx->dropped.p;
+        fp.active_flag = ctx->active_flag.p;
+        fp.active_out = ctx->active_next.p;
+        fp.active_count = ctx->active_count.p;
+        fp.mse_threshold = e.mse_threshold;
+        fp.phi_scale = ctx->n_tot / ctx->tau;
+        fp.phi = ctx->phi.p;
+        fp.mse = ctx->mse.p;
+        fp.se = ctx->se.p;
+        hipEvent_t f0 = record(ctx);
+        hipLaunchKernelGGL(hase_finalize_kernel, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, fp);
+        HIP_CHECK(hipGetLastError());
+        tm.fin.emplace_back(f0, record(ctx));
+        HIP_CHECK(hipMemcpyAsync(&active, ctx->active_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        std::swap(ctx->active_list.p, ctx->active_next.p);
+        N += nrays;
+        launch++;
+        if (target < sched.max_rays) k++;
+        if (learned && deferred && launch == launch0 + 1u && active > 0u) {
+            // samples the learned proposal did not settle with the first budget: hand them back
+            std::vector<uint32_t> miss(active);
+            HIP_CHECK(hipMemcpy(miss.data(), ctx->active_list.p, active * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (uint32_t slot : miss) deferred->push_back(samples ? samples[slot] : b0 + slot);
+            ctx->pstats.fallback_samples += active;
+            break;
+        }
+    }
+    std::vector<float> h_phi(slots);
+    std::vector<double> h_mse(slots), h_se(slots);
+    std::vector<uint32_t> h_N(slots), h_drop(slots);
+    HIP_CHECK(hipMemcpyAsync(h_phi.data(), ctx->phi.p, slots * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(h_mse.data(), ctx->mse.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(h_se.data(), ctx->se.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(h_N.data(), ctx->N.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(h_drop.data(), ctx->dropped.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < slots; ++i) {
+        const uint32_t s = samples ? samples[i] : b0 + i;
+        const double phi = (double)h_phi[i];
+        res->phi_ase[s] = h_phi[i];
+        res->mse[s] = h_mse[i];
+        res->total_rays[s] = h_N[i];
+        res->dndt_ase[s] = (ctx->beta_cells[s] * (e.max_sigma_e + e.max_sigma_a) - e.max_sigma_a) * phi;
+        if (res->standard_error) res->standard_error[s] = h_se[i];
+        if (res->dropped_rays) res->dropped_rays[s] = h_drop[i];
+    }
+}
+
 void evaluate(hase_context* ctx, hase_result_t* res)
 {
     if (!res || !res->phi_ase || !res->mse || !res->total_rays || !res->dndt_ase)
@@ -961,6 +1144,7 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     ctx->active_list.alloc(batch);
     ctx->active_next.alloc(batch);
     ctx->active_count.alloc(1);
+    ctx->slot_sample.alloc(batch);
     ctx->partial.alloc((size_t)batch * max_chunks);
     ctx->partial_drop.alloc((size_t)batch * max_chunks);
     ctx->phi.alloc(batch);
@@ -968,201 +1152,27 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     ctx->se.alloc(batch);
     HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
 
-    const MeshDev m = mesh_view(ctx);
-    Schedule sched{e.min_rays_per_sample, e.max_rays_per_sample, c.adaptive_steps, c.max_repetitions};
-    std::vector<uint32_t> iota(batch);
-    for (uint32_t i = 0; i < batch; ++i) iota[i] = i;
-    std::vector<float> h_phi(batch);
-    std::vector<double> h_mse(batch), h_se(batch);
-    std::vector<uint32_t> h_N(batch), h_drop(batch);
-    std::vector<int32_t> ones(batch, 1);
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_imp, ev_trace, ev_fin, ev_inf, ev_prop;
+    BatchTimers tm;
     hipEvent_t ev_begin = record(ctx);
-
+    std::vector<uint32_t> deferred;
+    std::vector<uint32_t>* defer = (learned && ctx->proposal_fallback) ? &deferred : nullptr;
     for (uint32_t b0 = first; b0 <= last; b0 += batch) {
         const uint32_t slots = std::min(batch, last - b0 + 1u);
-        HIP_CHECK(hipMemsetAsync(ctx->s1.p, 0, slots * sizeof(double), ctx->stream));
-        HIP_CHECK(hipMemsetAsync(ctx->s2.p, 0, slots * sizeof(double), ctx->stream));
-        HIP_CHECK(hipMemsetAsync(ctx->N.p, 0, slots * sizeof(uint32_t), ctx->stream));
-        HIP_CHECK(hipMemsetAsync(ctx->dropped.p, 0, slots * sizeof(uint32_t), ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(ctx->active_flag.p, ones.data(), slots * sizeof(int32_t), hipMemcpyHostToDevice,
-                                 ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(ctx->active_list.p, iota.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                 ctx->stream));
-
-        ImportanceParams ip;
-        ip.m = m;
-        ip.pts = ctx->pts.p;
-        ip.sample_base = b0;
-        ip.slots = slots;
-        ip.R = R;
-        ip.n_lambda = e.number_of_wavelengths;
-        ip.c1 = ctx->c1.p;
-        ip.c2 = ctx->c2.p;
-        ip.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
-        ip.eps = e.defensive_fraction;
-        ip.Bsum = ctx->Bsum;
-        ip.imp = ctx->imp.p;
-        ip.prefix = ctx->prefix.p;
-        ip.Q = ctx->Q.p;
-        ip.counters = ctx->counters.p;
-        ip.active = nullptr;
-        ip.transfer = 0;
-        if (learned) {
-            hipEvent_t a0 = record(ctx);
-            launch_proposal_inference(ctx, b0, slots, ctx->logits.p);
-            hipEvent_t a1 = record(ctx);
-            LearnedParams lp;
-            lp.m = m;
-            lp.pts = ctx->pts.p;
-            lp.centroid = ctx->centroid.p;
-            lp.logits = ctx->logits.p;
-            lp.slots = slots;
-            lp.sample_base = b0;
-            lp.R = R;
-            lp.C = ctx->C;
-            lp.TG = ctx->TG;
-            lp.ct = ctx->cluster_t;
-            lp.cl = ctx->cluster_l;
-            lp.eps = e.defensive_fraction;
-            lp.Bsum = ctx->Bsum;
-            lp.prefix = ctx->prefix.p;
-            lp.Q = ctx->Q.p;
-            hipLaunchKernelGGL(hase_learned_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, lp);
-            HIP_CHECK(hipGetLastError());
-            ev_inf.emplace_back(a0, a1);
-            ev_prop.emplace_back(a1, record(ctx));
-        } else {
-            hipEvent_t i0 = record(ctx);
-            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
-            HIP_CHECK(hipGetLastError());
-            ev_imp.emplace_back(i0, record(ctx));
-            ctx->stats.importance_launches++;
-        }
-
-        uint32_t active = slots;
-        uint32_t N = 0, k = 0, extra = 0, launch = 0;
-        while (active > 0) {
-            uint32_t target = sched.target(k);
-            uint32_t nrays;
-            if (target > N) {
-                nrays = target - N;
-            } else if (N >= sched.max_rays || target >= sched.max_rays) {
-                if (extra + 1u >= sched.reps) break;
-                extra++;
-                nrays = sched.max_rays;
-            } else {
-                k++;
-                continue;
-            }
-            const uint32_t chunks = (nrays + rays_per_block - 1) / rays_per_block;
-            TraceParams tp;
-            tp.m = m;
-            tp.pts = ctx->pts.p;
-            tp.sample_base = b0;
-            tp.R = R;
-            tp.n_lambda = e.number_of_wavelengths;
-            tp.c1 = ctx->c1.p;
-            tp.c2 = ctx->c2.p;
-            tp.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
-            tp.seed = c.rng_seed;
-            tp.launch = launch;
-            tp.nrays = nrays;
-            tp.chunks = chunks;
-            tp.rays_per_thread = rpt;
-            tp.active = ctx->active_list.p;
-            tp.prefix = ctx->prefix.p;
-            tp.Q = ctx->Q.p;
-            tp.partial = ctx->partial.p;
-            tp.partial_drop = ctx->partial_drop.p;
-            tp.max_chunks = max_chunks;
-            tp.counters = ctx->counters.p;
-            hipEvent_t t0 = record(ctx);
-            hipLaunchKernelGGL(hase_trace_kernel, dim3(active * chunks), dim3(kBlock), 0, ctx->stream, tp);
-            HIP_CHECK(hipGetLastError());
-            ev_trace.emplace_back(t0, record(ctx));
-            ctx->stats.trace_launches++;
-
-            HIP_CHECK(hipMemsetAsync(ctx->active_count.p, 0, sizeof(uint32_t), ctx->stream));
-            FinalizeParams fp;
-            fp.slots = slots;
-            fp.chunks = chunks;
-            fp.max_chunks = max_chunks;
-            fp.nrays = nrays;
-            fp.partial = ctx->partial.p;
-            fp.partial_drop = ctx->partial_drop.p;
-            fp.s1 = ctx->s1.p;
-            fp.s2 = ctx->s2.p;
-            fp.N = ctx->N.p;
-            fp.dropped = ctx->dropped.p;
-            fp.active_flag = ctx->active_flag.p;
-            fp.active_out = ctx->active_next.p;
-            fp.active_count = ctx->active_count.p;
-            fp.mse_threshold = e.mse_threshold;
-            fp.phi_scale = ctx->n_tot / ctx->tau;
-            fp.beta_cells = ctx->bcells.p + b0;
-            fp.dndt_a = e.max_sigma_e + e.max_sigma_a;
-            fp.dndt_b = e.max_sigma_a;
-            fp.phi = ctx->phi.p;
-            fp.mse = ctx->mse.p;
-            fp.se = ctx->se.p;
-            hipEvent_t f0 = record(ctx);
-            hipLaunchKernelGGL(hase_finalize_kernel, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream, fp);
-            HIP_CHECK(hipGetLastError());
-            ev_fin.emplace_back(f0, record(ctx));
-            HIP_CHECK(hipMemcpyAsync(&active, ctx->active_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                     ctx->stream));
-            HIP_CHECK(hipStreamSynchronize(ctx->stream));
-            std::swap(ctx->active_list.p, ctx->active_next.p);
-            N += nrays;
-            launch++;
-            if (target < sched.max_rays) k++;
-            if (learned && ctx->proposal_fallback && launch == 1u && active > 0u) {
-                // samples the learned proposal did not settle with the first budget restart on the
-                // walked importance map; their learned rays are discarded
-                ImportanceParams fb = ip;
-                fb.active = ctx->active_list.p;
-                hipEvent_t i0 = record(ctx);
-                hipLaunchKernelGGL(hase_importance_kernel, dim3(active), dim3(kBlock), 0, ctx->stream, fb);
-                HIP_CHECK(hipGetLastError());
-                ev_imp.emplace_back(i0, record(ctx));
-                ctx->stats.importance_launches++;
-                hipLaunchKernelGGL(hase_reset_slots_kernel, dim3((active + 255u) / 256u), dim3(256), 0, ctx->stream,
-                                   (const uint32_t*)ctx->active_list.p, active, ctx->s1.p, ctx->s2.p, ctx->N.p,
-                                   ctx->dropped.p);
-                HIP_CHECK(hipGetLastError());
-                ctx->pstats.fallback_samples += active;
-                N = 0;
-                k = 0;
-                extra = 0;
-            }
-        }
-        HIP_CHECK(hipMemcpyAsync(h_phi.data(), ctx->phi.p, slots * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(h_mse.data(), ctx->mse.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(h_se.data(), ctx->se.p, slots * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(h_N.data(), ctx->N.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipMemcpyAsync(h_drop.data(), ctx->dropped.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        for (uint32_t i = 0; i < slots; ++i) {
-            uint32_t s = b0 + i;
-            double phi = (double)h_phi[i];
-            res->phi_ase[s] = h_phi[i];
-            res->mse[s] = h_mse[i];
-            res->total_rays[s] = h_N[i];
-            res->dndt_ase[s] = (ctx->beta_cells[s] * (e.max_sigma_e + e.max_sigma_a) - e.max_sigma_a) * phi;
-            if (res->standard_error) res->standard_error[s] = h_se[i];
-            if (res->dropped_rays) res->dropped_rays[s] = h_drop[i];
-        }
+        run_batch(ctx, res, b0, nullptr, slots, learned, 0u, defer, tm);
         if (last - b0 < batch) break;
+    }
+    // learned-proposal misses restart on the walked map, batched across the whole range
+    for (size_t d0 = 0; d0 < deferred.size(); d0 += batch) {
+        const uint32_t slots = (uint32_t)std::min<size_t>(batch, deferred.size() - d0);
+        run_batch(ctx, res, 0u, deferred.data() + d0, slots, false, 1u, nullptr, tm);
     }
     hipEvent_t ev_end = record(ctx);
     HIP_CHECK(hipEventSynchronize(ev_end));
-    for (auto& pr : ev_imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
-    for (auto& pr : ev_trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
-    for (auto& pr : ev_fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
-    for (auto& pr : ev_inf) ctx->pstats.inference_ms += elapsed(pr.first, pr.second);
-    for (auto& pr : ev_prop) ctx->pstats.proposal_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : tm.imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : tm.trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : tm.fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : tm.inf) ctx->pstats.inference_ms += elapsed(pr.first, pr.second);
+    for (auto& pr : tm.prop) ctx->pstats.proposal_ms += elapsed(pr.first, pr.second);
     ctx->stats.total_ms = elapsed(ev_begin, ev_end);
     unsigned long long cnt[4];
     HIP_CHECK(hipMemcpy(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
@@ -1321,6 +1331,7 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
         ip.Q = context->Q.p;
         ip.counters = context->counters.p;
         ip.active = nullptr;
+        ip.sample_of = nullptr;
         ip.transfer = 0;
         hipLaunchKernelGGL(hase_importance_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
         HIP_CHECK(hipGetLastError());
@@ -1408,6 +1419,7 @@ int hase_context_transfer_map(hase_context_t* context, uint32_t first, uint32_t 
             ip.Q = nullptr;
             ip.counters = context->counters.p;
             ip.active = nullptr;
+            ip.sample_of = nullptr;
             ip.transfer = 1;
             hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, context->stream, ip);
             HIP_CHECK(hipGetLastError());
