@@ -1,15 +1,22 @@
-"""Benchmark: adaptive importance-sampled ASE on the production disk, 1 GPU.
+"""Benchmark: adaptive importance-sampled ASE on the production disk, 1 GPU,
+learned proposal vs the walked (plain) importance map.
 
-One step = one full evaluation of the hot path (importance map of every
-sample point, adaptive rays until every sample's mse <= 0.005 or 1e7 rays,
-FP64 statistics) on the seeded production disk: radius 3 cm, ~2.1k points,
-~4.1k triangles, 20 levels over 0.7 cm, 64 wavelengths, first-order top /
-bottom reflections with per-triangle reflectivity ~U(0, 0.1), cladding ring
+One step = one full evaluation of the hot path (proposal for every sample
+point, adaptive rays until every sample's mse <= 0.005 or 1e7 rays, FP64
+statistics) on the seeded production disk: radius 3 cm, ~2.1k points, ~4.1k
+triangles, 20 levels over 0.7 cm, 64 wavelengths, first-order top / bottom
+reflections with per-triangle reflectivity ~U(0, 0.1), cladding ring
 (5.5 cm^-1), adaptive 1e5 -> 1e7 rays per sample.  The mesh is resident on
 the device before timing starts (persistent context).
 
-Prints one JSON line.  ``value`` is rays/s of the whole job;
-``wall_time_to_mse_s`` is the median warm time of a step.
+Both forms are timed: the plain form (walked importance map) and the learned
+form (network trained on the GPU, before timing, on seeded random pump
+fields of the same mesh - never on the benchmark's own beta field - and
+evaluated per prism cluster, with the walked map as fallback).
+
+Prints one JSON line.  ``value`` is rays/s of the whole job in the learned
+form; ``wall_time_to_mse_s`` is its median warm step time, next to the plain
+form's under ``plain``.
 """
 from __future__ import annotations
 
@@ -32,6 +39,9 @@ FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_STEP = 56
 FLOPS_PER_STEP = 40
 FLOPS_PER_RAY = 40
+# network rows: 2 flops per multiply-add, 10 -> 128 -> 128 -> 128 -> 128 -> 1
+MLP_FLOPS_PER_ROW = 2 * (10 * 128 + 3 * 128 * 128 + 128)
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -88,6 +98,55 @@ def cpu_baseline(w, spectra, seconds=15.0):
             "samples_per_s": done_samples / elapsed}
 
 
+def timed_steps(ctx, w, label, steps, warmup):
+    """Warmup then `steps` timed evaluations; returns (times, results, stats list)."""
+    t0 = time.perf_counter()
+    res = ctx.evaluate()
+    cold = time.perf_counter() - t0
+    log(f"{label}: cold step {cold:.2f}s")
+    for i in range(max(0, warmup - 1)):
+        ctx.evaluate()
+    times, stats, rays = [], [], 0
+    for i in range(steps):
+        t0 = time.perf_counter()
+        res = ctx.evaluate()
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        st["proposal"] = ctx.proposal_stats()
+        times.append(dt)
+        stats.append(st)
+        rays += int(res.totalRays.astype(np.int64).sum())
+        log(f"{label} step {i}: {dt:.3f}s rays {st['rays']} max mse {np.nanmax(res.mse):.3e} "
+            f"importance {st['importance_ms']:.0f}ms trace {st['trace_ms']:.0f}ms "
+            f"network {st['proposal']['inference_ms']:.0f}ms proposal {st['proposal']['proposal_ms']:.0f}ms "
+            f"fallback {st['proposal']['fallback_samples']}")
+    return times, res, stats, rays, cold
+
+
+def kernel_table(stats):
+    trace_ms = sum(s["trace_ms"] for s in stats)
+    imp_ms = sum(s["importance_ms"] for s in stats)
+    launches = sum(s["trace_launches"] for s in stats)
+    imp_launches = sum(s["importance_launches"] for s in stats)
+    ray_steps = sum(s["ray_steps"] for s in stats)
+    imp_steps = sum(s["importance_steps"] for s in stats)
+    k = {
+        "hase_trace_kernel": {"launches": launches, "avg_ms": trace_ms / max(launches, 1),
+                              "bytes_per_launch": ray_steps * BYTES_PER_STEP / max(launches, 1)},
+        "hase_importance_kernel": {"launches": imp_launches, "avg_ms": imp_ms / max(imp_launches, 1),
+                                   "bytes_per_launch": imp_steps * BYTES_PER_STEP / max(imp_launches, 1)},
+    }
+    inf_ms = sum(s["proposal"]["inference_ms"] for s in stats)
+    if inf_ms > 0:
+        rows = sum(s["proposal"]["evaluations"] for s in stats)
+        k["mlp_infer_kernel"] = {"rows": rows, "ms": inf_ms,
+                                 "tflops": rows * MLP_FLOPS_PER_ROW / (inf_ms * 1e-3) / 1e12,
+                                 "bf16_mfma_frac": rows * MLP_FLOPS_PER_ROW / (inf_ms * 1e-3) / 1e12
+                                 / BF16_PEAK_TFLOPS}
+        k["hase_learned_importance_kernel"] = {"ms": sum(s["proposal"]["proposal_ms"] for s in stats)}
+    return k, ray_steps, imp_steps, trace_ms, imp_ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +155,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--levels", type=int, default=20, help="override for quick local runs")
+    ap.add_argument("--fields", type=int, default=16, help="random pump fields the network trains on")
+    ap.add_argument("--clusters", default="16x4", help="triangles x layers per network evaluation")
+    ap.add_argument("--plain-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every sample point of a run is scheduled on a single device "
@@ -103,6 +165,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
+    from haseongpu_amd import proposal as prop
     from haseongpu_amd.workloads import production_disk
 
     w = production_disk(levels=args.levels)
@@ -111,45 +174,45 @@ def main():
         f"{topo.numberOfPrisms} prisms, {w.samples} samples")
     t0 = time.perf_counter()
     ctx, spectra = build_context(w)
-    upload_s = time.perf_counter() - t0
-    log(f"context created (mesh resident) in {upload_s:.2f}s")
+    log(f"context created (mesh resident) in {time.perf_counter() - t0:.2f}s")
 
-    t0 = time.perf_counter()
-    res = ctx.evaluate()
-    cold_s = time.perf_counter() - t0
-    log(f"cold step {cold_s:.2f}s, rays {int(res.totalRays.astype(np.int64).sum())}")
-    for i in range(max(0, args.warmup - 1)):
-        ctx.evaluate()
-        log(f"warmup {i + 2} done")
+    p_times, p_res, p_stats, p_rays, p_cold = timed_steps(ctx, w, "plain", args.steps, args.warmup)
+    p_wall = float(np.median(p_times))
+    plain = {"wall_time_to_mse_s": p_wall, "rays_per_s": p_rays / float(np.sum(p_times)),
+             "cold_first_call_s": p_cold, "max_mse": float(np.nanmax(p_res.mse)),
+             "all_samples_converged": bool(np.all(p_res.mse <= w.settings["mseThreshold"])),
+             "rays_per_step": p_rays / args.steps}
+    p_kernels, p_rs, p_is, p_tms, p_ims = kernel_table(p_stats)
+    plain["prism_steps_per_s"] = (p_rs + p_is) / float(np.sum(p_times))
+    plain["kernels"] = p_kernels
 
-    times, rays, stats = [], 0, []
-    for i in range(args.steps):
+    learned = None
+    times, res, stats, rays, cold = p_times, p_res, p_stats, p_rays, p_cold
+    if not args.plain_only:
+        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
         t0 = time.perf_counter()
-        res = ctx.evaluate()
-        dt = time.perf_counter() - t0
-        st = ctx.stats()
-        times.append(dt)
-        rays += int(res.totalRays.astype(np.int64).sum())
-        stats.append(st)
-        log(f"step {i}: {dt:.3f}s rays {st['rays']} max mse {np.nanmax(res.mse):.3e} "
-            f"importance {st['importance_ms']:.0f}ms trace {st['trace_ms']:.0f}ms")
+        pr, (X, _) = prop.train_on_workload(w, fields=args.fields, samples_per_field=48, rows_per_sample=1024,
+                                            seed=1, epochs=12, batch=4096, cluster_triangles=ct,
+                                            cluster_levels=cl)
+        train_s = time.perf_counter() - t0
+        log(f"network trained on {X.shape[0]} rows from {args.fields} random pump fields in {train_s:.1f}s "
+            f"(loss {pr.meta['loss_history'][-1]:.3f} of target variance {pr.meta['target_variance']:.3f})")
+        ctx.set_proposal(pr)
+        times, res, stats, rays, cold = timed_steps(ctx, w, "learned", args.steps, args.warmup)
+        z = (res.phiAse.astype(np.float64) - p_res.phiAse) / np.sqrt(res.standardError ** 2
+                                                                       + p_res.standardError ** 2)
+        learned = {"clusters": [ct, cl], "training_s": train_s, "training_rows": int(X.shape[0]),
+                   "training_fields": args.fields, "final_loss": pr.meta["loss_history"][-1],
+                   "target_variance": pr.meta["target_variance"],
+                   "fallback_samples": stats[-1]["proposal"]["fallback_samples"],
+                   "agreement_with_plain": {"mean_z": float(np.mean(z)),
+                                            "frac_abs_z_gt_3": float(np.mean(np.abs(z) > 3))}}
     total = float(np.sum(times))
-    converged = bool(np.all(res.mse <= w.settings["mseThreshold"]))
-
-    trace_ms = sum(s["trace_ms"] for s in stats)
-    imp_ms = sum(s["importance_ms"] for s in stats)
-    launches = sum(s["trace_launches"] for s in stats)
-    imp_launches = sum(s["importance_launches"] for s in stats)
-    ray_steps = sum(s["ray_steps"] for s in stats)
-    imp_steps = sum(s["importance_steps"] for s in stats)
+    kernels, ray_steps, imp_steps, trace_ms, imp_ms = kernel_table(stats)
     total_rays = sum(s["rays"] for s in stats)
-    kernels = {
-        "hase_trace_kernel": {"launches": launches, "avg_ms": trace_ms / max(launches, 1),
-                              "bytes_per_launch": ray_steps * BYTES_PER_STEP / max(launches, 1)},
-        "hase_importance_kernel": {"launches": imp_launches, "avg_ms": imp_ms / max(imp_launches, 1),
-                                   "bytes_per_launch": imp_steps * BYTES_PER_STEP / max(imp_launches, 1)},
-    }
-    dominant = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    wall = float(np.median(times))
+    dominant = max(("hase_trace_kernel", "hase_importance_kernel"),
+                   key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
     kd = kernels[dominant]
     achieved = kd["bytes_per_launch"] / (kd["avg_ms"] * 1e-3) / 1e9
     traffic = None
@@ -177,14 +240,19 @@ def main():
             "levels": int(topo.levels), "prisms": topo.numberOfPrisms, "sample_points": w.samples,
             "wavelengths": int(spectra[0].size), "reflection_order": w.settings.get("maxReflections", 1),
             "rays_per_sample": [w.settings["minRaysPerSample"], w.settings["maxRaysPerSample"]],
-            "mse_threshold": w.settings["mseThreshold"], "proposal": "plain importance map",
+            "mse_threshold": w.settings["mseThreshold"],
+            "proposal": "plain importance map" if learned is None else
+                        f"learned (4x128 SiLU, BF16 MFMA, clusters {args.clusters}) with walked-map fallback",
         },
-        "wall_time_to_mse_s": float(np.median(times)),
-        "cold_first_call_s": cold_s,
-        "all_samples_converged": converged,
+        "wall_time_to_mse_s": wall,
+        "cold_first_call_s": cold,
+        "all_samples_converged": bool(np.all(res.mse <= w.settings["mseThreshold"])),
+        "max_mse": float(np.nanmax(res.mse)),
+        "speedup_vs_plain": p_wall / wall,
         "prism_steps_per_s": (ray_steps + imp_steps) / total,
         "rays_per_step": rays / args.steps,
-        "learned_proposal": None,
+        "plain": plain,
+        "learned_proposal": learned,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": BYTES_PER_STEP, "fp32_valu_tflops": valu_tflops,
