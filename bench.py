@@ -48,7 +48,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def build_context(w):
+def build_context(w, **compute):
     from haseongpu_amd import _native
 
     s = w.settings
@@ -59,7 +59,7 @@ def build_context(w):
                                    mse_threshold=s["mseThreshold"], use_reflections=s["useReflections"],
                                    max_reflections=s.get("maxReflections", 1), defensive_fraction=0.05)
     comp = _native.make_compute(n_samples=w.samples, max_repetitions=s["repetitions"],
-                                adaptive_steps=s["adaptiveSteps"], rng_seed=s["rngSeed"])
+                                adaptive_steps=s["adaptiveSteps"], rng_seed=s["rngSeed"], **compute)
     return _native.Context(mesh, exp, comp), (sa, se, msa, mse_)
 
 
@@ -147,6 +147,49 @@ def kernel_table(stats):
     return k, ray_steps, imp_steps, trace_ms, imp_ms
 
 
+def traffic_pass(w, args):
+    """Kernels for the FETCH_SIZE / WRITE_SIZE counter runs: the learned
+    evaluation (its hase_trace_kernel launches are the bench line's), a plain
+    evaluation of the first level's samples (hase_importance_kernel), and a
+    1 GiB device fill + device-to-device copy (the runtime's fillBuffer and
+    copyBuffer kernels) whose known byte counts calibrate the counters."""
+    from haseongpu_amd import proposal as prop
+
+    ctx, _ = build_context(w)
+    if args.load_proposal:
+        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+        ctx.set_proposal(prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl))
+        ctx.evaluate()
+        learned = ctx.stats()
+        ctx.close()
+    else:
+        learned = None
+    ctx, _ = build_context(w, max_sample_range=w.topology.numberOfPoints - 1)
+    ctx.evaluate()
+    plain = ctx.stats()
+    ctx.close()
+    # the HIP runtime's own fill and device-to-device copy kernels over 1 GiB
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 28
+    nbytes = 4 * n
+    src, dst = C.c_void_p(), C.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    print(json.dumps({"learned": learned, "plain_first_level": plain, "fill_bytes": nbytes, "copy_bytes": nbytes}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +201,13 @@ def main():
     ap.add_argument("--fields", type=int, default=16, help="random pump fields the network trains on")
     ap.add_argument("--clusters", default="16x4", help="triangles x layers per network evaluation")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default queue launch shape")
+    ap.add_argument("--rounds", action="store_true", help="one trace launch per adaptive round (no work queue)")
+    ap.add_argument("--save-proposal", default=None, help="write the trained network (.npz)")
+    ap.add_argument("--load-proposal", default=None, help="use a saved network instead of training one")
+    ap.add_argument("--traffic-pass", action="store_true",
+                    help="counter pass: one learned evaluation, one plain evaluation of the first level's "
+                         "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every sample point of a run is scheduled on a single device "
@@ -169,12 +219,26 @@ def main():
     from haseongpu_amd.workloads import production_disk
 
     w = production_disk(levels=args.levels)
+    if args.traffic_pass:
+        traffic_pass(w, args)
+        return
     topo = w.topology
     log(f"workload: {topo.numberOfPoints} points, {topo.numberOfTriangles} triangles, {topo.levels} levels, "
         f"{topo.numberOfPrisms} prisms, {w.samples} samples")
     t0 = time.perf_counter()
     ctx, spectra = build_context(w)
     log(f"context created (mesh resident) in {time.perf_counter() - t0:.2f}s")
+    from haseongpu_amd import _native
+
+    tune_s = 0.0
+    if args.rounds:
+        ctx.set_tuning(mode=_native.TRACE_ROUNDS)
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        best = ctx.autotune(512)
+        tune_s = time.perf_counter() - t0
+        log(f"autotuned queue in {tune_s:.1f}s: {best}")
+    tuning = ctx.tuning()
 
     p_times, p_res, p_stats, p_rays, p_cold = timed_steps(ctx, w, "plain", args.steps, args.warmup)
     p_wall = float(np.median(p_times))
@@ -191,10 +255,16 @@ def main():
     if not args.plain_only:
         ct, cl = (int(v) for v in args.clusters.lower().split("x"))
         t0 = time.perf_counter()
-        pr, (X, _) = prop.train_on_workload(w, fields=args.fields, samples_per_field=48, rows_per_sample=1024,
-                                            seed=1, epochs=12, batch=4096, cluster_triangles=ct,
-                                            cluster_levels=cl)
+        if args.load_proposal:
+            pr = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
+            X = np.zeros((pr.meta.get("rows", 0), 10))
+        else:
+            pr, (X, _) = prop.train_on_workload(w, fields=args.fields, samples_per_field=48, rows_per_sample=1024,
+                                                seed=1, epochs=12, batch=4096, cluster_triangles=ct,
+                                                cluster_levels=cl)
         train_s = time.perf_counter() - t0
+        if args.save_proposal:
+            pr.save(args.save_proposal)
         log(f"network trained on {X.shape[0]} rows from {args.fields} random pump fields in {train_s:.1f}s "
             f"(loss {pr.meta['loss_history'][-1]:.3f} of target variance {pr.meta['target_variance']:.3f})")
         ctx.set_proposal(pr)
@@ -213,12 +283,14 @@ def main():
     wall = float(np.median(times))
     dominant = max(("hase_trace_kernel", "hase_importance_kernel"),
                    key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches"])
+    dominant_name = dominant if tuning["mode"] == _native.TRACE_ROUNDS or dominant != "hase_trace_kernel" \
+        else "hase_queue_trace_kernel"
     kd = kernels[dominant]
     achieved = kd["bytes_per_launch"] / (kd["avg_ms"] * 1e-3) / 1e9
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get(dominant)
+        traffic = json.loads(tf.read_text()).get(dominant_name)
     valu_tflops = ((ray_steps + imp_steps) * FLOPS_PER_STEP + total_rays * FLOPS_PER_RAY) / ((trace_ms + imp_ms) * 1e-3) / 1e12
     out = {
         "metric": METRIC,
@@ -246,6 +318,8 @@ def main():
         },
         "wall_time_to_mse_s": wall,
         "cold_first_call_s": cold,
+        "ray_scheduling": {"mode": "rounds" if tuning["mode"] == _native.TRACE_ROUNDS else "persistent queue",
+                           "tuning": tuning, "autotune_s": tune_s},
         "all_samples_converged": bool(np.all(res.mse <= w.settings["mseThreshold"])),
         "max_mse": float(np.nanmax(res.mse)),
         "speedup_vs_plain": p_wall / wall,
@@ -253,7 +327,7 @@ def main():
         "rays_per_step": rays / args.steps,
         "plain": plain,
         "learned_proposal": learned,
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dominant_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": BYTES_PER_STEP, "fp32_valu_tflops": valu_tflops,
                      "fp32_valu_frac": valu_tflops / FP32_PEAK_TFLOPS},

@@ -40,6 +40,9 @@ EXPORTED_SYMBOLS = (
     "hase_context_transfer_map",
     "hase_context_proposal_logits",
     "hase_context_proposal_stats",
+    "hase_context_set_tuning",
+    "hase_context_get_tuning",
+    "hase_context_autotune",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -174,6 +177,24 @@ class ProposalStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32),
+        ("block_size", C.c_uint32),
+        ("rays_per_item", C.c_uint32),
+        ("queue_chunk", C.c_uint32),
+        ("refill_lanes", C.c_uint32),
+        ("step_budget", C.c_uint32),
+        ("use_graph", C.c_uint32),
+        ("probe_ms", C.c_double),
+    ]
+
+    def asdict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+TRACE_QUEUE = 0
+TRACE_ROUNDS = 1
 MLP_INPUTS = 10
 MLP_HIDDEN = 128
 PROPOSAL_PLAIN = 0
@@ -213,6 +234,9 @@ def load():
     L.hase_context_transfer_map.argtypes = [ctx_p, C.c_uint32, C.c_uint32, _f32p, C.c_size_t]
     L.hase_context_proposal_logits.argtypes = [ctx_p, C.c_uint32, C.c_uint32, _f32p, C.c_size_t, _f32p, _u32p]
     L.hase_context_proposal_stats.argtypes = [ctx_p, C.POINTER(ProposalStats)]
+    L.hase_context_set_tuning.argtypes = [ctx_p, C.POINTER(Tuning)]
+    L.hase_context_get_tuning.argtypes = [ctx_p, C.POINTER(Tuning)]
+    L.hase_context_autotune.argtypes = [ctx_p, C.c_uint32, C.POINTER(Tuning)]
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
@@ -220,7 +244,8 @@ def load():
                  "hase_context_evaluate", "hase_context_stats", "hase_context_trace_rays",
                  "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
                  "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
-                 "hase_context_proposal_logits", "hase_context_proposal_stats"):
+                 "hase_context_proposal_logits", "hase_context_proposal_stats", "hase_context_set_tuning",
+                 "hase_context_get_tuning", "hase_context_autotune"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -402,6 +427,26 @@ class Context:
         check(load().hase_context_proposal_logits(self._h, int(first), int(count), _ptr(out, C.c_float), out.size,
                                                   _ptr(clusters, C.c_float), C.byref(n_cl)))
         return out.reshape(count, R, Cn), clusters.reshape(Cn, 4)
+
+    def tuning(self):
+        t = Tuning()
+        check(load().hase_context_get_tuning(self._h, C.byref(t)))
+        return t.asdict()
+
+    def set_tuning(self, **fields):
+        """Change fields of the ray-scheduling settings (see hase_tuning_t)."""
+        t = Tuning()
+        check(load().hase_context_get_tuning(self._h, C.byref(t)))
+        for key, value in fields.items():
+            setattr(t, key, value)
+        check(load().hase_context_set_tuning(self._h, C.byref(t)))
+        return t.asdict()
+
+    def autotune(self, probe_samples=0):
+        """Pick the queue kernel's launch shape by timing probes; returns the choice."""
+        t = Tuning()
+        check(load().hase_context_autotune(self._h, int(probe_samples), C.byref(t)))
+        return t.asdict()
 
     def proposal_stats(self):
         st = ProposalStats()

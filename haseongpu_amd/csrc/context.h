@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -59,8 +61,22 @@ struct hase_context {
     hase::DevBuf<float> logits;              // [slots * R * C]
     hase_proposal_stats_t pstats{};
 
+    // persistent work queue
+    hase_tuning_t tuning{HASE_TRACE_ROUNDS, 256u, 1024u, 2u, 16u, 8u, 1u, 0.0};
+    hase::DevBuf<uint32_t> q_flags, q_k, q_extra, q_launch, q_nrays, q_target;   // [slots]
+    hase::DevBuf<uint32_t> q_list, q_item_off;                                  // [slots], [slots + 1]
+    hase::DevBuf<uint32_t> q_scalars;                                           // n_active, head, slots
+    hase::DevBuf<uint32_t> q_targets;                                           // schedule targets by step
+    size_t q_items = 0;                                                         // partial capacity (items)
+    hipGraph_t q_graph = nullptr;
+    hipGraphExec_t q_exec = nullptr;
+    uint64_t q_key = 0;                                                         // generation the graph was built for
+    uint64_t q_generation = 1;                                                  // bumped when launch inputs may change
+
     ~hase_context()
     {
+        if (q_exec) (void)hipGraphExecDestroy(q_exec);
+        if (q_graph) (void)hipGraphDestroy(q_graph);
         for (auto e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -68,6 +84,32 @@ struct hase_context {
 
 
 namespace hase {
+
+// Adaptive ray schedule (include/core/types.hpp:333-354): cumulative target
+// of increase k, min_rays * (max_rays / min_rays)^(k / steps) rounded up.
+struct Schedule {
+    uint32_t min_rays, max_rays, steps, reps;
+    uint32_t target(uint32_t k) const
+    {
+        if (max_rays <= min_rays || steps == 0) return min_rays;
+        if (k >= steps) return max_rays;
+        const double growth = std::pow((double)max_rays / (double)min_rays, 1.0 / (double)steps);
+        double t = std::ceil((double)min_rays * std::pow(growth, (double)k));
+        t = std::min(std::max(t, (double)min_rays), (double)max_rays);
+        return (uint32_t)t;
+    }
+};
+
 MeshDev mesh_view(const hase_context* ctx);
 uint32_t slices(const hase_context* ctx);
+
+// events recorded on the context stream (destroyed at the next evaluation)
+hipEvent_t record(hase_context* ctx);
+float elapsed(hipEvent_t a, hipEvent_t b);
+
+// persistent work queue (queue.hip)
+void check_tuning(const hase_tuning_t& t);
+uint32_t queue_max_rounds(const Schedule& s);
+void queue_reserve(hase_context* ctx, uint32_t batch);
+float queue_rounds(hase_context* ctx, uint32_t base, bool mapped, uint32_t slots, uint32_t launch0, bool defer);
 }  // namespace hase

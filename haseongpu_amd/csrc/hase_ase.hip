@@ -35,50 +35,13 @@
 #include "context.h"
 #include "prism_walk.h"
 #include "proposal.h"
+#include "device_common.h"
 
 namespace {
 
 using namespace hase;
 
 // ------------------------------------------------------------------ device
-
-struct SampleGeom {
-    float px, py;
-    int plane;   // level of the sample point (its height in layer units)
-};
-
-__device__ __forceinline__ SampleGeom sample_point(const float2* pts, uint32_t np, uint32_t sample)
-{
-    uint32_t p = sample % np, lv = sample / np;
-    float2 xy = pts[p];
-    return SampleGeom{xy.x, xy.y, (int)lv};
-}
-
-// ---------------------------------------------------- wave/block reductions
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// inclusive scan within a 64-lane wavefront
-__device__ __forceinline__ double wave_inclusive_scan(double v, int lane)
-{
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        double o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-    }
-    return v;
-}
 
 // Defensive-mixture proposal and its FP64 inclusive prefix for one sample
 // (one work-group): q_e = (1 - eps) imp_e / I + eps * bv_j / (R * Bsum), with
@@ -612,18 +575,6 @@ __global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* star
 }
 
 // ------------------------------------------------------------------- host
-struct Schedule {
-    uint32_t min_rays, max_rays, steps, reps;
-    uint32_t target(uint32_t k) const
-    {
-        if (max_rays <= min_rays || steps == 0) return min_rays;
-        if (k >= steps) return max_rays;
-        double growth = std::pow((double)max_rays / (double)min_rays, 1.0 / (double)steps);
-        double t = std::ceil((double)min_rays * std::pow(growth, (double)k));
-        t = std::min(std::max(t, (double)min_rays), (double)max_rays);
-        return (uint32_t)t;
-    }
-};
 
 }  // namespace
 
@@ -882,61 +833,65 @@ uint32_t slices(const hase_context* ctx)
 
 namespace {
 
-hipEvent_t record(hase_context* ctx)
-{
-    hipEvent_t ev;
-    HIP_CHECK(hipEventCreate(&ev));
-    HIP_CHECK(hipEventRecord(ev, ctx->stream));
-    ctx->events.push_back(ev);
-    return ev;
-}
-
-float elapsed(hipEvent_t a, hipEvent_t b)
-{
-    float ms = 0.0f;
-    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
-    return ms;
-}
-
-// One batch of resident samples: proposal (walked map or learned), then
-// adaptive trace launches until every slot converges or reaches its budget.
-// Slot i holds sample samples[i] (or b0 + i when samples is null).  With the
-// learned proposal and `deferred`, samples that miss the threshold after the
-// first launch are handed back (they restart on the walked map later, from
-// launch index 1, so their random streams stay distinct).
 struct BatchTimers {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> imp, trace, fin, inf, prop;
+    double queue_ms = 0.0;
 };
 
-void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_t* samples, uint32_t slots,
-               bool learned, uint32_t launch0, std::vector<uint32_t>* deferred, BatchTimers& tm)
+// Device buffers for batches of up to `want` resident samples; returns the
+// batch size (the importance prefix and scratch of a batch stay within ~6 GiB
+// unless the caller fixed sample_batch).
+uint32_t reserve_batch(hase_context* ctx, uint32_t want)
 {
     const hase_compute_t& c = ctx->comp;
     const hase_experiment_t& e = ctx->exp;
     const uint32_t R = slices(ctx);
+    const size_t n_ent = (size_t)R * ctx->P;
+    const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
     const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
     const uint32_t rays_per_block = kBlock * rpt;
     const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
-    const MeshDev m = mesh_view(ctx);
-    const Schedule sched{e.min_rays_per_sample, e.max_rays_per_sample, c.adaptive_steps, c.max_repetitions};
-    std::vector<uint32_t> iota(slots);
-    for (uint32_t i = 0; i < slots; ++i) iota[i] = i;
-    std::vector<int32_t> ones(slots, 1);
-    const uint32_t* sample_of = nullptr;
-    if (samples) {
-        HIP_CHECK(hipMemcpyAsync(ctx->slot_sample.p, samples, slots * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                 ctx->stream));
-        sample_of = ctx->slot_sample.p;
+    uint32_t batch = c.sample_batch;
+    if (batch == 0) {
+        const uint32_t per_item = ctx->tuning.mode == HASE_TRACE_QUEUE
+                                      ? (e.max_rays_per_sample + ctx->tuning.rays_per_item - 1) / ctx->tuning.rays_per_item
+                                      : max_chunks;
+        size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + (learned ? (size_t)R * ctx->C * 4u : 0u);
+        size_t budget = (size_t)6 << 30;
+        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
-    HIP_CHECK(hipMemsetAsync(ctx->s1.p, 0, slots * sizeof(double), ctx->stream));
-    HIP_CHECK(hipMemsetAsync(ctx->s2.p, 0, slots * sizeof(double), ctx->stream));
-    HIP_CHECK(hipMemsetAsync(ctx->N.p, 0, slots * sizeof(uint32_t), ctx->stream));
-    HIP_CHECK(hipMemsetAsync(ctx->dropped.p, 0, slots * sizeof(uint32_t), ctx->stream));
-    HIP_CHECK(hipMemcpyAsync(ctx->active_flag.p, ones.data(), slots * sizeof(int32_t), hipMemcpyHostToDevice,
-                             ctx->stream));
-    HIP_CHECK(hipMemcpyAsync(ctx->active_list.p, iota.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice,
-                             ctx->stream));
+    batch = std::min(batch, want);
+    ctx->imp.alloc((size_t)batch * n_ent);
+    ctx->prefix.alloc((size_t)batch * n_ent);
+    if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
+    ctx->Q.alloc(batch);
+    ctx->s1.alloc(batch);
+    ctx->s2.alloc(batch);
+    ctx->N.alloc(batch);
+    ctx->dropped.alloc(batch);
+    ctx->active_flag.alloc(batch);
+    ctx->active_list.alloc(batch);
+    ctx->active_next.alloc(batch);
+    ctx->active_count.alloc(1);
+    ctx->slot_sample.alloc(batch);
+    ctx->partial.alloc((size_t)batch * max_chunks);
+    ctx->partial_drop.alloc((size_t)batch * max_chunks);
+    ctx->phi.alloc(batch);
+    ctx->mse.alloc(batch);
+    ctx->se.alloc(batch);
+    ctx->counters.alloc(4);
+    if (ctx->tuning.mode == HASE_TRACE_QUEUE) queue_reserve(ctx, batch);
+    return batch;
+}
 
+// Proposal prefix and Q of slots [0, slots): the walked importance map or the
+// learned proposal (which needs contiguous samples b0 + slot).
+void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, uint32_t slots, bool learned,
+                    BatchTimers& tm)
+{
+    const hase_experiment_t& e = ctx->exp;
+    const uint32_t R = slices(ctx);
+    const MeshDev m = mesh_view(ctx);
     if (learned) {
         hipEvent_t a0 = record(ctx);
         launch_proposal_inference(ctx, b0, slots, ctx->logits.p);
@@ -988,7 +943,54 @@ void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_
         ctx->stats.importance_launches++;
     }
 
-    uint32_t active = slots;
+}
+
+
+void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_t* samples, uint32_t slots,
+               bool learned, uint32_t launch0, std::vector<uint32_t>* deferred, BatchTimers& tm)
+{
+    const hase_compute_t& c = ctx->comp;
+    const hase_experiment_t& e = ctx->exp;
+    const uint32_t R = slices(ctx);
+    const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
+    const uint32_t rays_per_block = kBlock * rpt;
+    const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
+    const MeshDev m = mesh_view(ctx);
+    const Schedule sched{e.min_rays_per_sample, e.max_rays_per_sample, c.adaptive_steps, c.max_repetitions};
+    (void)R;
+    std::vector<uint32_t> iota(slots);
+    for (uint32_t i = 0; i < slots; ++i) iota[i] = i;
+    std::vector<int32_t> ones(slots, 1);
+    const uint32_t* sample_of = nullptr;
+    if (samples) {
+        HIP_CHECK(hipMemcpyAsync(ctx->slot_sample.p, samples, slots * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        sample_of = ctx->slot_sample.p;
+    }
+    HIP_CHECK(hipMemsetAsync(ctx->s1.p, 0, slots * sizeof(double), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->s2.p, 0, slots * sizeof(double), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->N.p, 0, slots * sizeof(uint32_t), ctx->stream));
+    HIP_CHECK(hipMemsetAsync(ctx->dropped.p, 0, slots * sizeof(uint32_t), ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(ctx->active_flag.p, ones.data(), slots * sizeof(int32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+    HIP_CHECK(hipMemcpyAsync(ctx->active_list.p, iota.data(), slots * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+
+    build_proposal(ctx, b0, sample_of, slots, learned, tm);
+    std::vector<uint32_t> flags;
+    if (ctx->tuning.mode == HASE_TRACE_QUEUE) {
+        tm.queue_ms += queue_rounds(ctx, b0, samples != nullptr, slots, launch0, learned && deferred);
+        if (learned && deferred) {
+            flags.resize(slots);
+            HIP_CHECK(hipMemcpy(flags.data(), ctx->q_flags.p, slots * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < slots; ++i)
+                if (flags[i] & 2u) {
+                    deferred->push_back(samples ? samples[i] : b0 + i);
+                    ctx->pstats.fallback_samples++;
+                }
+        }
+    }
+    uint32_t active = ctx->tuning.mode == HASE_TRACE_QUEUE ? 0u : slots;
     uint32_t N = 0, k = 0, extra = 0, launch = launch0;
     while (active > 0) {
         const uint32_t target = sched.target(k);
@@ -999,6 +1001,8 @@ void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_
             if (extra + 1u >= sched.reps) break;
             extra++;
             nrays = sched.max_rays;
+        } else if (k >= sched.steps) {
+            break;   // flat schedule (no increases): nothing more to add
         } else {
             k++;
             continue;
@@ -1100,9 +1104,7 @@ void evaluate(hase_context* ctx, hase_result_t* res)
         throw HaseError(HASE_E_INVALID, "result arrays phi_ase, mse, total_rays and dndt_ase are required");
     HIP_CHECK(hipSetDevice(ctx->device));
     const hase_compute_t& c = ctx->comp;
-    const hase_experiment_t& e = ctx->exp;
-    const uint32_t S = ctx->S, P = ctx->P, R = slices(ctx);
-    const size_t n_ent = (size_t)R * P;
+    const uint32_t S = ctx->S;
     for (uint32_t s = 0; s < S; ++s) {
         res->phi_ase[s] = 0.0f;
         res->mse[s] = 0.0;
@@ -1119,38 +1121,10 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     if (!(ctx->Bsum > 0.0)) return;   // no emitting volume: the estimate is zero
     const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
 
-    const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
-    const uint32_t rays_per_block = kBlock * rpt;
-    const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
     const uint32_t first = c.min_sample_range, last = c.max_sample_range;
-    const uint32_t total = last - first + 1u;
-    uint32_t batch = c.sample_batch;
-    if (batch == 0) {
-        // keep the importance prefix (8 B) and scratch (4 B) of a batch within ~6 GiB
-        size_t per_sample = n_ent * 12u + (size_t)max_chunks * 12u + 64u + (learned ? (size_t)R * ctx->C * 4u : 0u);
-        size_t budget = (size_t)6 << 30;
-        batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
-    }
-    batch = std::min(batch, total);
-    ctx->imp.alloc((size_t)batch * n_ent);
-    ctx->prefix.alloc((size_t)batch * n_ent);
-    if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
-    ctx->Q.alloc(batch);
-    ctx->s1.alloc(batch);
-    ctx->s2.alloc(batch);
-    ctx->N.alloc(batch);
-    ctx->dropped.alloc(batch);
-    ctx->active_flag.alloc(batch);
-    ctx->active_list.alloc(batch);
-    ctx->active_next.alloc(batch);
-    ctx->active_count.alloc(1);
-    ctx->slot_sample.alloc(batch);
-    ctx->partial.alloc((size_t)batch * max_chunks);
-    ctx->partial_drop.alloc((size_t)batch * max_chunks);
-    ctx->phi.alloc(batch);
-    ctx->mse.alloc(batch);
-    ctx->se.alloc(batch);
+    const uint32_t batch = reserve_batch(ctx, last - first + 1u);
     HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    ctx->q_generation++;
 
     BatchTimers tm;
     hipEvent_t ev_begin = record(ctx);
@@ -1171,6 +1145,7 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     for (auto& pr : tm.imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
     for (auto& pr : tm.trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
     for (auto& pr : tm.fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
+    ctx->stats.trace_ms += tm.queue_ms;
     for (auto& pr : tm.inf) ctx->pstats.inference_ms += elapsed(pr.first, pr.second);
     for (auto& pr : tm.prop) ctx->pstats.proposal_ms += elapsed(pr.first, pr.second);
     ctx->stats.total_ms = elapsed(ev_begin, ev_end);
@@ -1186,11 +1161,96 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     ctx->stats.dropped_rays = dropped;
 }
 
+// Autotuner of the persistent work queue: coordinate descent over the
+// kernel's launch shape on one adaptive batch of the first probe samples,
+// each setting timed (best of two) on the same proposal.
+void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
+{
+    HIP_CHECK(hipSetDevice(ctx->device));
+    hase_tuning_t cur = ctx->tuning;
+    cur.mode = HASE_TRACE_QUEUE;
+    cur.probe_ms = 0.0;
+    if (!(ctx->Bsum > 0.0)) {
+        ctx->tuning = cur;
+        *best = cur;
+        return;
+    }
+    static const uint32_t blocks[] = {128u, 256u, 512u};
+    static const uint32_t items[] = {256u, 512u, 1024u, 2048u};
+    static const uint32_t chunks[] = {1u, 2u, 4u};
+    static const uint32_t refills[] = {8u, 16u, 32u};
+    static const uint32_t budgets[] = {4u, 8u, 16u};
+    const uint32_t first = ctx->comp.min_sample_range, last = ctx->comp.max_sample_range;
+    const uint32_t want = std::min(probe ? probe : 512u, last - first + 1u);
+    const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
+    ctx->tuning = cur;
+    ctx->tuning.rays_per_item = items[0];   // largest item count any candidate needs
+    const uint32_t slots = reserve_batch(ctx, want);
+    HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    BatchTimers tm;
+    build_proposal(ctx, first, nullptr, slots, learned, tm);
+    auto time_it = [&](const hase_tuning_t& t) {
+        ctx->tuning = t;
+        queue_reserve(ctx, slots);
+        ctx->q_generation++;
+        float ms = std::numeric_limits<float>::infinity();
+        for (int rep = 0; rep < 2; ++rep) ms = std::min(ms, queue_rounds(ctx, first, false, slots, 0u, false));
+        return (double)ms;
+    };
+    double best_ms = time_it(cur);
+    for (int pass = 0; pass < 2; ++pass) {
+        auto sweep = [&](const uint32_t* vals, size_t n, uint32_t hase_tuning_t::*field) {
+            for (size_t i = 0; i < n; ++i) {
+                if (cur.*field == vals[i]) continue;
+                hase_tuning_t t = cur;
+                t.*field = vals[i];
+                const double ms = time_it(t);
+                if (ms < 0.98 * best_ms) {
+                    best_ms = ms;
+                    cur = t;
+                }
+            }
+        };
+        sweep(blocks, 3, &hase_tuning_t::block_size);
+        sweep(items, 4, &hase_tuning_t::rays_per_item);
+        sweep(chunks, 3, &hase_tuning_t::queue_chunk);
+        sweep(refills, 3, &hase_tuning_t::refill_lanes);
+        sweep(budgets, 3, &hase_tuning_t::step_budget);
+    }
+    cur.probe_ms = best_ms;
+    ctx->tuning = cur;
+    ctx->q_generation++;
+    *best = cur;
+}
+
 }  // namespace
 
 namespace hase {
 thread_local std::string g_last_error;
+
+hipEvent_t record(hase_context* ctx)
+{
+    hipEvent_t ev;
+    HIP_CHECK(hipEventCreate(&ev));
+    HIP_CHECK(hipEventRecord(ev, ctx->stream));
+    ctx->events.push_back(ev);
+    return ev;
 }
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// One batch of resident samples: proposal (walked map or learned), then
+// adaptive trace launches until every slot converges or reaches its budget.
+// Slot i holds sample samples[i] (or b0 + i when samples is null).  With the
+// learned proposal and `deferred`, samples that miss the threshold after the
+// first launch are handed back (they restart on the walked map later, from
+// launch index 1, so their random streams stay distinct).
+}  // namespace hase
 
 extern "C" {
 
@@ -1476,6 +1536,32 @@ int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_sta
     return guarded([&] {
         if (!context || !stats) throw HaseError(HASE_E_INVALID, "context and stats must not be NULL");
         *stats = context->pstats;
+    });
+}
+
+int hase_context_set_tuning(hase_context_t* context, const hase_tuning_t* tuning)
+{
+    return guarded([&] {
+        if (!context || !tuning) throw HaseError(HASE_E_INVALID, "context and tuning must not be NULL");
+        check_tuning(*tuning);
+        context->tuning = *tuning;
+        context->q_generation++;
+    });
+}
+
+int hase_context_get_tuning(const hase_context_t* context, hase_tuning_t* tuning)
+{
+    return guarded([&] {
+        if (!context || !tuning) throw HaseError(HASE_E_INVALID, "context and tuning must not be NULL");
+        *tuning = context->tuning;
+    });
+}
+
+int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_tuning_t* best)
+{
+    return guarded([&] {
+        if (!context || !best) throw HaseError(HASE_E_INVALID, "context and best must not be NULL");
+        autotune(context, probe_samples, best);
     });
 }
 

@@ -103,41 +103,79 @@ HASE_HD inline float fast_rcp(float x)
 // the nearest one whose outward normal faces the ray (entry edge excluded);
 // an exit edge with an end point at the end's (x, y) meets the ray exactly
 // at the end; reflections are the copy boundaries of the unfolded z axis.
+//
+// The walk is resumable: walk_begin sets up a WalkState and walk_advance
+// takes up to `budget` prism steps, so a lane can hand a finished ray back
+// and start another while its neighbours are still walking.
+struct WalkState {
+    float ax, ay, za;         // start (x, y in cm; z in layer units)
+    float ex, ey;             // end (x, y)
+    float vx, vy, D, kz, tol, sin2;
+    int zdir, end_plane, end_on_plane;
+    uint32_t nrefl, first;
+    uint32_t t;               // current triangle
+    int u, copy, r, entry;    // unfolded layer, mirror copy, layer in copy, entry edge
+    float s;                  // distance walked
+    uint32_t applied;         // surface factors applied
+    float beta_len, gain_len, clad_len, refl;
+    uint32_t steps;
+    int status;               // 0 ok, 1 left the mesh, 2 step limit
+    int done;
+};
+
+HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float za, uint32_t t, uint32_t layer,
+                               float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+{
+    w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
+    w.steps = 0; w.status = 0; w.done = 0;
+    w.ax = ax; w.ay = ay; w.za = za; w.ex = ex; w.ey = ey;
+    w.nrefl = nrefl; w.first = first;
+    const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
+    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+    w.D = D;
+    if (!(D > 0.0f)) {
+        w.done = 1;
+        return;
+    }
+    const float inv = 1.0f / D;
+    w.vx = dx * inv;
+    w.vy = dy * inv;
+    const float vz = dz * inv;
+    w.sin2 = fmaxf(0.0f, 1.0f - vz * vz);
+    w.tol = 2e-5f * (1.0f + D);
+    // parameter per layer unit of height, and the end's plane index
+    w.kz = dzeta != 0.0f ? D / dzeta : 0.0f;
+    w.zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
+    w.end_plane = (int)floorf(ze + 0.5f);
+    w.end_on_plane = (float)w.end_plane == ze;
+    w.t = t;
+    w.u = (int)layer;
+    w.copy = 0;
+    w.r = (int)layer;
+    w.entry = -1;
+    w.s = 0.0f;
+    w.applied = 0;
+}
+
 // kReflect = false is the direct walk (start and end inside the slab), which
 // needs neither the mirror folding nor surface factors.
 template <bool kReflect>
-HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget)
 {
-    WalkOut w;
-    w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
-    w.steps = 0; w.status = 0;
-    const float h = m.h;
-    const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
-    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
-    w.length = D;
-    if (!(D > 0.0f)) return w;
-    const float inv = 1.0f / D;
-    const float vx = dx * inv, vy = dy * inv, vz = dz * inv;
-    const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
-    const float tol = 2e-5f * (1.0f + D);
-    // parameter per layer unit of height, and the end's plane index
-    const float kz = dzeta != 0.0f ? D / dzeta : 0.0f;
-    const int zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
-    const int end_plane = (int)floorf(ze + 0.5f);
-    const bool end_on_plane = (float)end_plane == ze;
     const int layers = (int)m.layers;
-    int u = (int)layer;    // unfolded layer index
-    int copy = 0, r = (int)layer;
-    int entry = -1;
-    float s = 0.0f;
-    uint32_t applied = 0;
-    for (uint32_t step = 0; step < kMaxSteps; ++step) {
+    for (uint32_t it = 0; it < budget && !w.done; ++it) {
+        if (w.steps >= kMaxSteps) {
+            w.status = 2;
+            w.done = 1;
+            return;
+        }
         w.steps++;
-        const int level = kReflect ? ((copy & 1) ? (layers - 1 - r) : r) : u;
+        const uint32_t t = w.t;
+        const int level = kReflect ? ((w.copy & 1) ? (layers - 1 - w.r) : w.r) : w.u;
         // the triangle record and this prism's beta are independent loads
         const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
         const float beta = m.beta[t + (uint32_t)level * m.T];
+        const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy, D = w.D;
         const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
         const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
         const float X2 = bits_to_float(q1.x) - ax, Y2 = bits_to_float(q1.y) - ay;
@@ -149,9 +187,9 @@ HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za
         float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
         float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
         float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
-        s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
-        s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
-        s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
+        s0 = (w.entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
+        s1 = (w.entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
+        s2 = (w.entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
         int bestk = 0;
         float best = s0;
         if (s1 < best) { best = s1; bestk = 1; }
@@ -164,71 +202,97 @@ HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za
             const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
             const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
             const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
-            const bool touches = (bits_to_float(axb) == ex && bits_to_float(ayb) == ey) ||
-                                 (bits_to_float(bxb) == ex && bits_to_float(byb) == ey);
+            const bool touches = (bits_to_float(axb) == w.ex && bits_to_float(ayb) == w.ey) ||
+                                 (bits_to_float(bxb) == w.ex && bits_to_float(byb) == w.ey);
             best = (touches && best < INFINITY) ? D : best;
         }
         // next level plane along the ray (exactly D for the end's plane)
-        const int plane = u + (zdir > 0 ? 1 : 0);
-        float sz = zdir != 0 ? ((float)plane - za) * kz : INFINITY;
-        sz = (end_on_plane && plane == end_plane && zdir != 0) ? D : sz;
+        const int plane = w.u + (w.zdir > 0 ? 1 : 0);
+        float sz = w.zdir != 0 ? ((float)plane - w.za) * w.kz : INFINITY;
+        sz = (w.end_on_plane && plane == w.end_plane && w.zdir != 0) ? D : sz;
         float snext = fminf(fminf(best, sz), D);
-        snext = fmaxf(snext, s);
-        const float seg = snext - s;
+        snext = fmaxf(snext, w.s);
+        const float seg = snext - w.s;
         const uint32_t clad = q2.y;
         w.clad_len += clad ? seg : 0.0f;
         w.gain_len += clad ? 0.0f : seg;
         w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
-        s = snext;
+        w.s = snext;
         const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
         const bool zstep = sz <= best;
-        bool done = s >= D;
+        bool done = w.s >= D;
         if (!done && !zstep && nw == 0xFFFFFFFFu) {
             // leaving the mesh sideways: only rounding short of the end (a sample
             // point on the rim) counts as arrival
-            if (D - s <= tol) done = true;
-            else { w.status = 1; return w; }
+            if (D - w.s <= w.tol) done = true;
+            else {
+                w.status = 1;
+                w.done = 1;
+                return;
+            }
         }
         if (done) {
             if (kReflect) {
                 // an end point on a reflecting plane takes its last factor here
-                while (applied < nrefl) {
-                    uint32_t pl = (applied & 1u) ? 1u - first : first;
-                    w.refl *= surface_factor(m, t, pl, sin2);
-                    applied++;
+                while (w.applied < w.nrefl) {
+                    uint32_t pl = (w.applied & 1u) ? 1u - w.first : w.first;
+                    w.refl *= surface_factor(m, t, pl, w.sin2);
+                    w.applied++;
                 }
             }
-            return w;
+            w.done = 1;
+            return;
         }
         if (kReflect) {
             if (zstep) {
-                if (zdir > 0) {
-                    if (++r == layers) {
-                        if (applied < nrefl) {
-                            w.refl *= surface_factor(m, t, (copy & 1) ? 0u : 1u, sin2);
-                            applied++;
+                if (w.zdir > 0) {
+                    if (++w.r == layers) {
+                        if (w.applied < w.nrefl) {
+                            w.refl *= surface_factor(m, t, (w.copy & 1) ? 0u : 1u, w.sin2);
+                            w.applied++;
                         }
-                        r = 0;
-                        copy++;
+                        w.r = 0;
+                        w.copy++;
                     }
                 } else {
-                    if (--r < 0) {
-                        if (applied < nrefl) {
-                            w.refl *= surface_factor(m, t, (copy & 1) ? 1u : 0u, sin2);
-                            applied++;
+                    if (--w.r < 0) {
+                        if (w.applied < w.nrefl) {
+                            w.refl *= surface_factor(m, t, (w.copy & 1) ? 1u : 0u, w.sin2);
+                            w.applied++;
                         }
-                        r = layers - 1;
-                        copy--;
+                        w.r = layers - 1;
+                        w.copy--;
                     }
                 }
             }
         }
-        u = zstep ? u + zdir : u;
-        t = zstep ? t : (nw >> 2);
-        entry = zstep ? entry : (int)(nw & 3u);
+        w.u = zstep ? w.u + w.zdir : w.u;
+        w.t = zstep ? t : (nw >> 2);
+        w.entry = zstep ? w.entry : (int)(nw & 3u);
     }
-    w.status = 2;
+}
+
+HASE_HD inline WalkOut walk_result(const WalkState& s)
+{
+    WalkOut w;
+    w.beta_len = s.beta_len;
+    w.gain_len = s.gain_len;
+    w.clad_len = s.clad_len;
+    w.refl = s.refl;
+    w.length = s.D;
+    w.steps = s.steps;
+    w.status = s.status;
     return w;
+}
+
+template <bool kReflect>
+HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
+                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+{
+    WalkState s;
+    walk_begin(s, m.h, ax, ay, za, t, layer, ex, ey, ze, nrefl, first);
+    walk_advance<kReflect>(m, s, kMaxSteps + 1u);
+    return walk_result(s);
 }
 
 HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,

@@ -289,6 +289,39 @@ int hase_context_proposal_logits(hase_context_t* context, uint32_t first, uint32
 
 int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_stats_t* stats);
 
+/* ------------------------------------------------------------------------
+ * Ray scheduling.  No counterpart in the reference, whose adaptive loop is a
+ * host loop of launches (include/core/forwardPhiAseEvaluator.hpp:114-178);
+ * here the default runs each adaptive round of a batch of samples as one
+ * persistent work-queue kernel: every wavefront takes queue items (a run of
+ * rays_per_item rays of one sample) from a device cursor, its lanes walk
+ * those rays and refill with new ones when refill_lanes of them are idle,
+ * and the per-sample schedule lives on the device, so the rounds of a batch
+ * replay as one captured hipGraph with a single host synchronisation.
+ * ---------------------------------------------------------------------- */
+#define HASE_TRACE_QUEUE 0
+#define HASE_TRACE_ROUNDS 1
+
+typedef struct hase_tuning {
+    uint32_t mode;            /* HASE_TRACE_QUEUE (default) or HASE_TRACE_ROUNDS (one launch per round) */
+    uint32_t block_size;      /* queue kernel threads per work-group: 64..1024, multiple of 64 */
+    uint32_t rays_per_item;   /* rays per queue item, multiple of 64 */
+    uint32_t queue_chunk;     /* items a wavefront takes per queue access, >= 1 */
+    uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64 */
+    uint32_t step_budget;     /* prism steps between refill checks, >= 1 */
+    uint32_t use_graph;       /* 1: replay the rounds of a batch as a captured hipGraph */
+    double probe_ms;          /* hase_context_autotune: device time of the chosen setting's probe */
+} hase_tuning_t;
+
+int hase_context_set_tuning(hase_context_t* context, const hase_tuning_t* tuning);
+int hase_context_get_tuning(const hase_context_t* context, hase_tuning_t* tuning);
+
+/* Autotuner: times one adaptive round of the first probe_samples samples of
+ * the context's range (0: 512) under candidate settings (coordinate descent
+ * over block_size, rays_per_item, queue_chunk, refill_lanes, step_budget),
+ * keeps the fastest on the context and returns it. */
+int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_tuning_t* best);
+
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);
 

@@ -550,7 +550,8 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
                     if (extra + 1 >= reps) break;
                     extra++;
                     nr = prm->max_rays;
-                } else { k++; continue; }
+                } else if (k >= prm->adaptive_steps) break;   /* flat schedule: nothing to add */
+                else { k++; continue; }
                 orc_trace_launch(m, prm, s, launch, nr, prefix, order, &s1, &s2, &drop);
                 N += nr;
                 launch++;

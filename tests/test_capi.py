@@ -37,7 +37,7 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"hase_mesh_t": _native.Mesh, "hase_experiment_t": _native.Experiment,
                "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats,
                "hase_train_t": _native.Train, "hase_proposal_t": _native.Proposal,
-               "hase_proposal_stats_t": _native.ProposalStats}
+               "hase_proposal_stats_t": _native.ProposalStats, "hase_tuning_t": _native.Tuning}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():
@@ -126,3 +126,13 @@ def test_network_entry_points_reject_bad_arguments_on_the_host():
     assert lib.hase_context_transfer_map(None, 0, 1, yp, 4) == -1
     assert lib.hase_context_proposal_logits(None, 0, 1, yp, 4, None, None) == -1
     assert lib.hase_context_proposal_stats(None, None) == -1
+
+
+def test_tuning_entry_points_reject_bad_arguments_on_the_host():
+    lib = _native.load()
+    t = _native.Tuning(0, 256, 1024, 2, 16, 8, 1, 0.0)
+    assert lib.hase_context_set_tuning(None, C.byref(t)) == -1
+    assert lib.hase_context_get_tuning(None, C.byref(t)) == -1
+    assert lib.hase_context_autotune(None, 0, C.byref(t)) == -1
+    text = HEADER.read_text()
+    assert "#define HASE_TRACE_QUEUE 0" in text and "#define HASE_TRACE_ROUNDS 1" in text

@@ -1,0 +1,114 @@
+"""GPU tests of the persistent work-queue ray scheduling: it traces the same
+rays as one launch per adaptive round (identical ray counts, phiAse equal up
+to the FP64 summation order, i.e. a few FP32 ulps), with and without the
+captured hipGraph, through escalation, repetitions, flat schedules and the
+learned proposal's deferral; the autotuner returns a valid setting that
+leaves the results unchanged."""
+import numpy as np
+import pytest
+
+from haseongpu_amd import _native
+from haseongpu_amd.workloads import plain_small, production_disk
+from oracle import oracle as O
+from tests.test_gpu_parity import _context, _oracle
+from tests.test_gpu_proposal import _proposal_for, _random_net
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    assert np.array_equal(a.totalRays, b.totalRays)
+    assert np.array_equal(a.droppedRays, b.droppedRays)
+    np.testing.assert_allclose(a.phiAse, b.phiAse, rtol=3e-7, atol=0)
+    ok = np.isfinite(a.mse) & (a.mse < 1e300)
+    np.testing.assert_allclose(a.mse[ok], b.mse[ok], rtol=1e-9)
+
+
+def _modes(ctx):
+    out = {}
+    for name, kw in (("rounds", dict(mode=_native.TRACE_ROUNDS)),
+                     ("queue", dict(mode=_native.TRACE_QUEUE, use_graph=0)),
+                     ("graph", dict(mode=_native.TRACE_QUEUE, use_graph=1))):
+        ctx.set_tuning(**kw)
+        out[name] = ctx.evaluate()
+    return out
+
+
+@pytest.mark.parametrize("case", ["plain", "reflect"])
+def test_queue_traces_the_same_rays_as_rounds(case):
+    if case == "plain":
+        w = plain_small()
+        ctx, _ = _context(w, min_rays=2000, max_rays=64000, mse_threshold=2e-4, adaptive_steps=5,
+                          max_repetitions=2, rng_seed=9)
+    else:
+        w = production_disk(levels=6, n_points=400, wavelengths=8)
+        ctx, _ = _context(w, min_rays=3000, max_rays=30000, mse_threshold=3e-4, adaptive_steps=3,
+                          use_reflections=True, rng_seed=4)
+    with ctx:
+        r = _modes(ctx)
+        st = ctx.stats()
+    assert len(np.unique(r["rounds"].totalRays)) > 2       # the schedule escalated
+    _same(r["rounds"], r["queue"])
+    _same(r["rounds"], r["graph"])
+    assert st["rays"] == int(r["graph"].totalRays.astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("shape", [dict(block_size=64, rays_per_item=64, queue_chunk=1, refill_lanes=1, step_budget=1),
+                                   dict(block_size=1024, rays_per_item=4096, queue_chunk=7, refill_lanes=64,
+                                        step_budget=10001)])
+def test_queue_extreme_launch_shapes(shape):
+    w = plain_small()
+    ctx, _ = _context(w, min_rays=3000, max_rays=12000, mse_threshold=1e-4, adaptive_steps=2, rng_seed=3)
+    with ctx:
+        ctx.set_tuning(mode=_native.TRACE_ROUNDS)
+        ref = ctx.evaluate()
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1, **shape)
+        got = ctx.evaluate()
+    _same(ref, got)
+
+
+def test_flat_schedule_stops_after_the_minimum():
+    """adaptive_steps = 0 with max > min: the target never grows, so every
+    sample stops at min_rays (types.hpp:338-341), in both paths and the oracle."""
+    w = plain_small()
+    ctx, spectra = _context(w, min_rays=1500, max_rays=9000, mse_threshold=1e-9, adaptive_steps=0, rng_seed=2,
+                            max_sample_range=299)
+    with ctx:
+        r = _modes(ctx)
+    for res in r.values():
+        assert np.all(res.totalRays[:300] == 1500) and np.all(res.totalRays[300:] == 0)
+    om, op = _oracle(w, spectra, min_rays=1500, max_rays=9000, mse_threshold=1e-9, adaptive_steps=0, seed=2,
+                     sample_max=299)
+    ref = O.calc_phi_ase(om, op)
+    assert np.all(ref["totalRays"][:300] == 1500)
+    rel = np.abs(r["graph"].phiAse[:300] / ref["phiAse"][:300] - 1.0)
+    assert np.median(rel) <= 1e-5 and np.quantile(rel, 0.99) <= 1e-4
+
+
+def test_learned_deferral_in_the_queue_matches_rounds():
+    w = production_disk(levels=6, n_points=400, wavelengths=8)
+    ctx, _ = _context(w, min_rays=4096, max_rays=16384, adaptive_steps=2, use_reflections=True,
+                      mse_threshold=2e-3, rng_seed=8)
+    with ctx:
+        ctx.set_proposal(_proposal_for(w, _random_net(8, 0.5), 4, 2, fallback=True))
+        ctx.set_tuning(mode=_native.TRACE_ROUNDS)
+        a = ctx.evaluate()
+        fa = ctx.proposal_stats()["fallback_samples"]
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1)
+        b = ctx.evaluate()
+        fb = ctx.proposal_stats()["fallback_samples"]
+    assert fa == fb and fa > 0
+    _same(a, b)
+
+
+def test_autotune_picks_a_valid_setting_and_keeps_results():
+    w = production_disk(levels=6, n_points=400, wavelengths=8)
+    ctx, _ = _context(w, min_rays=4096, use_reflections=True, rng_seed=1)
+    with ctx:
+        ref = ctx.evaluate()
+        best = ctx.autotune(256)
+        assert best["mode"] == _native.TRACE_QUEUE and best["probe_ms"] > 0
+        assert best["block_size"] in (128, 256, 512) and best["rays_per_item"] in (256, 512, 1024, 2048)
+        assert ctx.tuning() == best
+        got = ctx.evaluate()
+    _same(ref, got)

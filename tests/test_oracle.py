@@ -323,3 +323,17 @@ def test_external_importance_reproduces_the_walked_run():
     assert not np.array_equal(flat["phiAse"][sl], base["phiAse"][sl])
     z = (flat["phiAse"][sl] - base["phiAse"][sl]) / (base["phiAse"][sl] * np.sqrt(flat["mse"][sl] + base["mse"][sl]))
     assert abs(np.mean(z)) < 1.0 and np.all(np.abs(z) < 5)
+
+
+def test_flat_schedule_terminates():
+    """adaptive_steps = 0 with max_rays > min_rays: the target never grows
+    (include/core/types.hpp:338-341), so a sample that misses the threshold
+    stops at min_rays instead of looping."""
+    w = plain_small(rays=500)
+    om = O.OracleMesh(w.topology, w.medium)
+    op = O.OracleParams(sigma_a=[1.16e-21], sigma_e=[2.48e-20], n_tot=2.76e20, tau=9.5e-4,
+                        beta_cells=w.medium.value("betaCells"), min_rays=500, max_rays=5000, adaptive_steps=0,
+                        mse_threshold=1e-12, sample_min=0, sample_max=20, n_samples=w.samples)
+    r = O.calc_phi_ase(om, op)
+    assert np.all(r["totalRays"][:21] == 500)
+    assert O.adaptive_target(500, 5000, 0, 7) == 500
