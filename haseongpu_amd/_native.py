@@ -43,6 +43,10 @@ EXPORTED_SYMBOLS = (
     "hase_context_set_tuning",
     "hase_context_get_tuning",
     "hase_context_autotune",
+    "hase_batch_create",
+    "hase_batch_evaluate",
+    "hase_batch_stats",
+    "hase_batch_destroy",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -237,6 +241,11 @@ def load():
     L.hase_context_set_tuning.argtypes = [ctx_p, C.POINTER(Tuning)]
     L.hase_context_get_tuning.argtypes = [ctx_p, C.POINTER(Tuning)]
     L.hase_context_autotune.argtypes = [ctx_p, C.c_uint32, C.POINTER(Tuning)]
+    L.hase_batch_create.argtypes = [C.c_uint32, C.POINTER(ctx_p), C.POINTER(ctx_p)]
+    L.hase_batch_evaluate.argtypes = [ctx_p, C.POINTER(C.POINTER(ResultC))]
+    L.hase_batch_stats.argtypes = [ctx_p, C.POINTER(Stats)]
+    L.hase_batch_destroy.argtypes = [ctx_p]
+    L.hase_batch_destroy.restype = None
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
@@ -245,7 +254,8 @@ def load():
                  "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
                  "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
                  "hase_context_proposal_logits", "hase_context_proposal_stats", "hase_context_set_tuning",
-                 "hase_context_get_tuning", "hase_context_autotune"):
+                 "hase_context_get_tuning", "hase_context_autotune", "hase_batch_create",
+                 "hase_batch_evaluate", "hase_batch_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -459,6 +469,47 @@ class Context:
         out = np.zeros(R * P, dtype=np.float32)
         check(load().hase_context_importance(self._h, int(sample), _ptr(out, C.c_float), out.size))
         return out
+
+
+class Batch:
+    """Independent problems (contexts) evaluated together: their samples share
+    one persistent work-queue kernel per adaptive round."""
+
+    def __init__(self, contexts):
+        L = load()
+        self._contexts = list(contexts)
+        arr = (C.c_void_p * len(self._contexts))(*[c._h for c in self._contexts])
+        handle = C.c_void_p()
+        check(L.hase_batch_create(len(self._contexts), arr, C.byref(handle)))
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().hase_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def evaluate(self):
+        res = [ResultArrays(c.n_samples) for c in self._contexts]
+        arr = (C.POINTER(ResultC) * len(res))(*[C.pointer(r.struct) for r in res])
+        check(load().hase_batch_evaluate(self._h, arr))
+        return res
+
+    def stats(self):
+        st = Stats()
+        check(load().hase_batch_stats(self._h, C.byref(st)))
+        return st.asdict()
 
 
 def mlp_param_count():

@@ -11,6 +11,7 @@
 #include "../../include/hase_ase.h"
 #include "common.h"
 #include "prism_walk.h"
+#include "queue.h"
 
 struct hase_context {
     int device = 0;
@@ -67,16 +68,14 @@ struct hase_context {
     hase::DevBuf<uint32_t> q_list, q_item_off;                                  // [slots], [slots + 1]
     hase::DevBuf<uint32_t> q_scalars;                                           // n_active, head, slots
     hase::DevBuf<uint32_t> q_targets;                                           // schedule targets by step
-    size_t q_items = 0;                                                         // partial capacity (items)
-    hipGraph_t q_graph = nullptr;
-    hipGraphExec_t q_exec = nullptr;
-    uint64_t q_key = 0;                                                         // generation the graph was built for
+    hase::DevBuf<hase::ProblemDev> q_problems;                                  // this context as a queue problem
+    hase::DevBuf<uint32_t> q_slot_problem;                                      // zeros
+    hase::GraphCache q_graph;
     uint64_t q_generation = 1;                                                  // bumped when launch inputs may change
 
     ~hase_context()
     {
-        if (q_exec) (void)hipGraphExecDestroy(q_exec);
-        if (q_graph) (void)hipGraphDestroy(q_graph);
+        q_graph.reset();
         for (auto e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -110,6 +109,8 @@ float elapsed(hipEvent_t a, hipEvent_t b);
 // persistent work queue (queue.hip)
 void check_tuning(const hase_tuning_t& t);
 uint32_t queue_max_rounds(const Schedule& s);
+void queue_targets(hase_context* ctx);
 void queue_reserve(hase_context* ctx, uint32_t batch);
+QueueState queue_state(hase_context* ctx);
 float queue_rounds(hase_context* ctx, uint32_t base, bool mapped, uint32_t slots, uint32_t launch0, bool defer);
 }  // namespace hase

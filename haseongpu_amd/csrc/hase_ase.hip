@@ -841,7 +841,7 @@ struct BatchTimers {
 // Device buffers for batches of up to `want` resident samples; returns the
 // batch size (the importance prefix and scratch of a batch stay within ~6 GiB
 // unless the caller fixed sample_batch).
-uint32_t reserve_batch(hase_context* ctx, uint32_t want)
+uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t)6 << 30, bool proposal_only = false)
 {
     const hase_compute_t& c = ctx->comp;
     const hase_experiment_t& e = ctx->exp;
@@ -857,7 +857,6 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want)
                                       ? (e.max_rays_per_sample + ctx->tuning.rays_per_item - 1) / ctx->tuning.rays_per_item
                                       : max_chunks;
         size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + (learned ? (size_t)R * ctx->C * 4u : 0u);
-        size_t budget = (size_t)6 << 30;
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
     batch = std::min(batch, want);
@@ -865,6 +864,12 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want)
     ctx->prefix.alloc((size_t)batch * n_ent);
     if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
     ctx->Q.alloc(batch);
+    ctx->slot_sample.alloc(batch);
+    ctx->counters.alloc(4);
+    if (proposal_only) {
+        queue_targets(ctx);
+        return batch;
+    }
     ctx->s1.alloc(batch);
     ctx->s2.alloc(batch);
     ctx->N.alloc(batch);
@@ -1159,6 +1164,234 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     for (uint32_t s = first; s <= last; ++s)
         if (res->dropped_rays) dropped += res->dropped_rays[s];
     ctx->stats.dropped_rays = dropped;
+}
+
+
+// ------------------------------------------------------------------ batches
+}  // namespace
+
+struct hase_batch {
+    std::vector<hase_context*> ctx;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hase::DevBuf<hase::ProblemDev> problems;
+    hase::DevBuf<uint32_t> slot_problem, flags, k, extra, launch, nrays, target, list, item_off, scalars, N, dropped;
+    hase::DevBuf<double> s1, s2, mse, se;
+    hase::DevBuf<float> phi;
+    hase::DevBuf<double2> partial;
+    hase::DevBuf<uint32_t> partial_drop;
+    hase::DevBuf<unsigned long long> counters;
+    hase::GraphCache graph;
+    uint64_t generation = 1;
+    hase_stats_t stats{};
+    ~hase_batch()
+    {
+        graph.reset();
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+void batch_evaluate(hase_batch* b, hase_result_t* const* results)
+{
+    if (!results) throw HaseError(HASE_E_INVALID, "results must not be NULL");
+    const size_t n = b->ctx.size();
+    for (size_t p = 0; p < n; ++p) {
+        hase_result_t* r = results[p];
+        if (!r || !r->phi_ase || !r->mse || !r->total_rays || !r->dndt_ase)
+            throw HaseError(HASE_E_INVALID, "every result needs phi_ase, mse, total_rays and dndt_ase");
+    }
+    HIP_CHECK(hipSetDevice(b->device));
+    hase_tuning_t tu = b->ctx[0]->tuning;
+    tu.mode = HASE_TRACE_QUEUE;
+    check_tuning(tu);
+    b->stats = hase_stats_t{};
+    b->generation++;
+    const size_t budget = std::min<size_t>((size_t)6 << 30, std::max<size_t>((size_t)512 << 20, ((size_t)24 << 30) / n));
+    std::vector<uint32_t> batch(n, 0), next(n), last(n);
+    std::vector<std::vector<uint32_t>> deferred(n);
+    std::vector<BatchTimers> tm(n);
+    for (size_t p = 0; p < n; ++p) {
+        hase_context* ctx = b->ctx[p];
+        hase_result_t* r = results[p];
+        for (uint32_t s = 0; s < ctx->S; ++s) {
+            r->phi_ase[s] = 0.0f;
+            r->mse[s] = 0.0;
+            r->total_rays[s] = 0;
+            r->dndt_ase[s] = 0.0;
+            if (r->standard_error) r->standard_error[s] = 0.0;
+            if (r->dropped_rays) r->dropped_rays[s] = 0;
+        }
+        for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+        ctx->events.clear();
+        ctx->stats = hase_stats_t{};
+        ctx->pstats = hase_proposal_stats_t{};
+        ctx->pstats.clusters = ctx->C;
+        next[p] = ctx->comp.min_sample_range;
+        last[p] = ctx->comp.max_sample_range;
+        if (!(ctx->Bsum > 0.0)) {   // no emitting volume: zeros
+            next[p] = last[p] + 1u;
+            continue;
+        }
+        batch[p] = reserve_batch(ctx, last[p] - next[p] + 1u, budget, true);
+        HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    }
+    b->counters.alloc(4);
+    HIP_CHECK(hipMemsetAsync(b->counters.p, 0, 4 * sizeof(unsigned long long), b->stream));
+    double queue_ms = 0.0;
+    for (;;) {
+        // one wave: the next resident batch of every problem with samples left
+        std::vector<uint32_t> part, slots_of, slot0_of;
+        uint32_t total = 0;
+        size_t items = 0;
+        for (size_t p = 0; p < n; ++p) {
+            if (next[p] > last[p]) continue;
+            const uint32_t sl = std::min(batch[p], last[p] - next[p] + 1u);
+            part.push_back((uint32_t)p);
+            slots_of.push_back(sl);
+            slot0_of.push_back(total);
+            total += sl;
+            items += (size_t)sl * queue_items_per_slot(b->ctx[p], tu);
+        }
+        if (part.empty()) break;
+        for (size_t w = 0; w < part.size(); ++w) {
+            hase_context* ctx = b->ctx[part[w]];
+            build_proposal(ctx, next[part[w]], nullptr, slots_of[w], ctx->proposal_kind == HASE_PROPOSAL_LEARNED,
+                           tm[part[w]]);
+        }
+        for (size_t w = 0; w < part.size(); ++w) HIP_CHECK(hipStreamSynchronize(b->ctx[part[w]]->stream));
+        for (auto* buf : {&b->slot_problem, &b->flags, &b->k, &b->extra, &b->launch, &b->nrays, &b->target, &b->list,
+                          &b->N, &b->dropped})
+            buf->alloc(total);
+        b->item_off.alloc((size_t)total + 1u);
+        b->scalars.alloc(kQueueScalars);
+        b->s1.alloc(total);
+        b->s2.alloc(total);
+        b->mse.alloc(total);
+        b->se.alloc(total);
+        b->phi.alloc(total);
+        b->partial.alloc(items);
+        b->partial_drop.alloc(items);
+        b->problems.alloc(part.size());
+        std::vector<ProblemDev> table(part.size());
+        std::vector<uint32_t> owner(total);
+        uint32_t rounds = 0;
+        for (size_t w = 0; w < part.size(); ++w) {
+            hase_context* ctx = b->ctx[part[w]];
+            const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
+            table[w] = queue_problem(ctx, slot0_of[w], next[part[w]], false, 0u, learned && ctx->proposal_fallback);
+            for (uint32_t i = 0; i < slots_of[w]; ++i) owner[slot0_of[w] + i] = (uint32_t)w;
+            const Schedule sc{ctx->exp.min_rays_per_sample, ctx->exp.max_rays_per_sample, ctx->comp.adaptive_steps,
+                              ctx->comp.max_repetitions};
+            rounds = std::max(rounds, queue_max_rounds(sc));
+        }
+        HIP_CHECK(hipMemcpyAsync(b->problems.p, table.data(), table.size() * sizeof(ProblemDev),
+                                 hipMemcpyHostToDevice, b->stream));
+        HIP_CHECK(hipMemcpyAsync(b->slot_problem.p, owner.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 b->stream));
+        QueueState qs;
+        std::memset(&qs, 0, sizeof(qs));
+        qs.problems = b->problems.p;
+        qs.slot_problem = b->slot_problem.p;
+        qs.flags = b->flags.p;
+        qs.k = b->k.p;
+        qs.extra = b->extra.p;
+        qs.launch = b->launch.p;
+        qs.nrays = b->nrays.p;
+        qs.target = b->target.p;
+        qs.list = b->list.p;
+        qs.item_off = b->item_off.p;
+        qs.scalars = b->scalars.p;
+        qs.s1 = b->s1.p;
+        qs.s2 = b->s2.p;
+        qs.N = b->N.p;
+        qs.dropped = b->dropped.p;
+        qs.phi = b->phi.p;
+        qs.mse = b->mse.p;
+        qs.se = b->se.p;
+        qs.partial = b->partial.p;
+        qs.partial_drop = b->partial_drop.p;
+        qs.counters = b->counters.p;
+        qs.rays_per_item = tu.rays_per_item;
+        qs.queue_chunk = tu.queue_chunk;
+        qs.refill = tu.refill_lanes;
+        qs.budget = tu.step_budget;
+        b->generation++;   // buffers may have moved: capture again
+        queue_ms += queue_run(b->stream, b->device, tu, qs, total, rounds, b->graph, b->generation);
+        b->stats.trace_launches += rounds;
+        std::vector<float> h_phi(total);
+        std::vector<double> h_mse(total), h_se(total);
+        std::vector<uint32_t> h_N(total), h_drop(total), h_flags(total);
+        HIP_CHECK(hipMemcpyAsync(h_phi.data(), b->phi.p, total * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+        HIP_CHECK(hipMemcpyAsync(h_mse.data(), b->mse.p, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIP_CHECK(hipMemcpyAsync(h_se.data(), b->se.p, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+        HIP_CHECK(hipMemcpyAsync(h_N.data(), b->N.p, total * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+        HIP_CHECK(hipMemcpyAsync(h_drop.data(), b->dropped.p, total * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 b->stream));
+        HIP_CHECK(hipMemcpyAsync(h_flags.data(), b->flags.p, total * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 b->stream));
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+        for (size_t w = 0; w < part.size(); ++w) {
+            const uint32_t p = part[w];
+            hase_context* ctx = b->ctx[p];
+            const hase_experiment_t& e = ctx->exp;
+            hase_result_t* r = results[p];
+            for (uint32_t i = 0; i < slots_of[w]; ++i) {
+                const uint32_t g = slot0_of[w] + i, s = next[p] + i;
+                r->phi_ase[s] = h_phi[g];
+                r->mse[s] = h_mse[g];
+                r->total_rays[s] = h_N[g];
+                r->dndt_ase[s] = (ctx->beta_cells[s] * (e.max_sigma_e + e.max_sigma_a) - e.max_sigma_a) * (double)h_phi[g];
+                if (r->standard_error) r->standard_error[s] = h_se[g];
+                if (r->dropped_rays) r->dropped_rays[s] = h_drop[g];
+                if (h_flags[g] & kQueueDeferred) {
+                    deferred[p].push_back(s);
+                    ctx->pstats.fallback_samples++;
+                }
+            }
+            next[p] += slots_of[w];
+        }
+    }
+    // learned-proposal misses restart on each problem's walked map
+    for (size_t p = 0; p < n; ++p) {
+        if (deferred[p].empty()) continue;
+        hase_context* ctx = b->ctx[p];
+        const uint32_t bs = reserve_batch(ctx, (uint32_t)deferred[p].size(), budget);
+        for (size_t d0 = 0; d0 < deferred[p].size(); d0 += bs) {
+            const uint32_t sl = (uint32_t)std::min<size_t>(bs, deferred[p].size() - d0);
+            run_batch(ctx, results[p], 0u, deferred[p].data() + d0, sl, false, 1u, nullptr, tm[p]);
+        }
+    }
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    HIP_CHECK(hipMemcpy(cnt, b->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    b->stats.rays = cnt[0];
+    b->stats.ray_steps = cnt[1];
+    b->stats.trace_ms = queue_ms;
+    for (size_t p = 0; p < n; ++p) {
+        hase_context* ctx = b->ctx[p];
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (auto& pr : tm[p].imp) b->stats.importance_ms += elapsed(pr.first, pr.second);
+        for (auto& pr : tm[p].trace) b->stats.trace_ms += elapsed(pr.first, pr.second);
+        for (auto& pr : tm[p].inf) ctx->pstats.inference_ms += elapsed(pr.first, pr.second);
+        for (auto& pr : tm[p].prop) ctx->pstats.proposal_ms += elapsed(pr.first, pr.second);
+        b->stats.importance_ms += ctx->pstats.inference_ms + ctx->pstats.proposal_ms;
+        b->stats.trace_ms += tm[p].queue_ms;
+        unsigned long long c2[4];
+        HIP_CHECK(hipMemcpy(c2, ctx->counters.p, sizeof(c2), hipMemcpyDeviceToHost));
+        b->stats.rays += c2[0];
+        b->stats.ray_steps += c2[1];
+        b->stats.importance_walks += c2[2];
+        b->stats.importance_steps += c2[3];
+        b->stats.importance_launches += ctx->stats.importance_launches;
+        uint64_t dr = 0;
+        if (results[p]->dropped_rays)
+            for (uint32_t s = ctx->comp.min_sample_range; s <= ctx->comp.max_sample_range; ++s) dr += results[p]->dropped_rays[s];
+        b->stats.dropped_rays += dr;
+    }
 }
 
 // Autotuner of the persistent work queue: coordinate descent over the
@@ -1563,6 +1796,52 @@ int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_
         if (!context || !best) throw HaseError(HASE_E_INVALID, "context and best must not be NULL");
         autotune(context, probe_samples, best);
     });
+}
+
+int hase_batch_create(uint32_t n_problems, hase_context_t* const* contexts, hase_batch_t** out_batch)
+{
+    return guarded([&] {
+        if (!out_batch) throw HaseError(HASE_E_INVALID, "out_batch must not be NULL");
+        *out_batch = nullptr;
+        if (n_problems == 0 || !contexts) throw HaseError(HASE_E_INVALID, "a batch needs at least one context");
+        auto b = std::make_unique<hase_batch>();
+        for (uint32_t i = 0; i < n_problems; ++i) {
+            if (!contexts[i]) throw HaseError(HASE_E_INVALID, "contexts must not be NULL");
+            if (contexts[i]->device != contexts[0]->device)
+                throw HaseError(HASE_E_INVALID, "every context of a batch must be on one device");
+            for (uint32_t j = 0; j < i; ++j)
+                if (contexts[j] == contexts[i]) throw HaseError(HASE_E_INVALID, "a context appears twice in the batch");
+            b->ctx.push_back(contexts[i]);
+        }
+        b->device = contexts[0]->device;
+        HIP_CHECK(hipSetDevice(b->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        *out_batch = b.release();
+    });
+}
+
+int hase_batch_evaluate(hase_batch_t* batch, hase_result_t* const* results)
+{
+    return guarded([&] {
+        if (!batch) throw HaseError(HASE_E_INVALID, "batch must not be NULL");
+        batch_evaluate(batch, results);
+    });
+}
+
+int hase_batch_stats(const hase_batch_t* batch, hase_stats_t* stats)
+{
+    return guarded([&] {
+        if (!batch || !stats) throw HaseError(HASE_E_INVALID, "batch and stats must not be NULL");
+        *stats = batch->stats;
+    });
+}
+
+void hase_batch_destroy(hase_batch_t* batch)
+{
+    if (batch) {
+        (void)hipSetDevice(batch->device);
+        delete batch;
+    }
 }
 
 const char* hase_last_error(void) { return hase::g_last_error.c_str(); }

@@ -5,6 +5,10 @@
   beta ~ U(0, 0.3) from numpy's default_rng(seed), one wavelength
   (sigma_a 1.16e-21, sigma_e 2.48e-20 cm^2), nTot 2.76e20 cm^-3,
   tau 9.5e-4 s, no reflections, no cladding, fixed rays per sample.
+* ``batch_problems`` - independent seeded Delaunay disks for batched runs:
+  triangles log-uniform in a range (200-8000 by default), 5-40 levels, beta
+  from 1-4 Gaussian pump spots (widths 0.3-1.5 cm), 1-16 Gaussian-bump
+  wavelengths, adaptive rays 1e4 -> 1e7 with mse threshold 0.005.
 * ``production_disk`` - disk of radius 3 cm, ~2k seeded points (jittered
   hexagonal lattice plus a boundary ring) Delaunay-triangulated into ~4k
   triangles, 20 levels over 0.7 cm, beta = 0.3 exp(-r^2 / (2 * 1.2^2)) +
@@ -114,3 +118,36 @@ def production_disk(seed=2, levels=20, n_points=2060, wavelengths=64, min_rays=1
                     repetitions=1, adaptiveSteps=4, useReflections=True, maxReflections=max_reflections,
                     monochromatic=False, rngSeed=seed)
     return Workload("production_disk", topo, medium, gaussian_spectra(wavelengths), settings)
+
+
+def batch_problems(n=64, seed=0, triangles=(200, 8000), levels=(5, 40), wavelengths=(1, 16), min_rays=10_000,
+                   max_rays=10_000_000, mse_threshold=0.005, adaptive_steps=4):
+    """`n` independent seeded problems of the batching workload."""
+    from .proposal import random_pump_beta
+
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng([seed, i])
+        T = float(np.exp(rng.uniform(np.log(triangles[0]), np.log(triangles[1]))))
+        L = int(rng.integers(levels[0], levels[1] + 1))
+        W = int(rng.integers(wavelengths[0], wavelengths[1] + 1))
+        radius = float(rng.uniform(1.5, 3.0))
+        pts = disk_points(radius, max(16, int(round(T / 2 + 1))), seed=int(rng.integers(1 << 30)))
+        topo = MeshTopology.fromPoints(pts, numberOfLevels=L, thickness=float(rng.uniform(0.5, 1.0)) / (L - 1))
+        beta = random_pump_beta(topo, rng, widths=(0.3, 1.5))
+        pr = np.hypot(topo.points[:, 0], topo.points[:, 1])
+        medium = GainMedium(topo).withPhysicalProperties(
+            betaVolume=beta,
+            betaCells=np.tile(np.interp(pr, [0, radius], [float(beta.max()), 0.0]), L),
+            claddingCellTypes=np.zeros(topo.numberOfTriangles, np.uint32),
+            claddingNumber=1,
+            claddingAbsorption=0.0,
+            nTot=2.76e20,
+            crystalTFluo=9.5e-4,
+            refractiveIndices=[1.83, 1.0, 1.83, 1.0],
+        )
+        settings = dict(minRaysPerSample=min_rays, maxRaysPerSample=max_rays, mseThreshold=mse_threshold,
+                        repetitions=1, adaptiveSteps=adaptive_steps, useReflections=False, monochromatic=False,
+                        rngSeed=seed * 1000 + i)
+        out.append(Workload(f"batch_problem_{i}", topo, medium, gaussian_spectra(W), settings))
+    return out

@@ -304,7 +304,7 @@ int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_sta
 
 typedef struct hase_tuning {
     uint32_t mode;            /* HASE_TRACE_QUEUE (default) or HASE_TRACE_ROUNDS (one launch per round) */
-    uint32_t block_size;      /* queue kernel threads per work-group: 64..1024, multiple of 64 */
+    uint32_t block_size;      /* queue kernel threads per work-group: 64..512, multiple of 64 */
     uint32_t rays_per_item;   /* rays per queue item, multiple of 64 */
     uint32_t queue_chunk;     /* items a wavefront takes per queue access, >= 1 */
     uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64 */
@@ -321,6 +321,26 @@ int hase_context_get_tuning(const hase_context_t* context, hase_tuning_t* tuning
  * over block_size, rays_per_item, queue_chunk, refill_lanes, step_budget),
  * keeps the fastest on the context and returns it. */
 int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_tuning_t* best);
+
+/* Batch of independent problems: the resident samples of every context run
+ * their adaptive rounds together in one persistent work-queue kernel per
+ * round (uneven budgets, different meshes, spectra and schedules), so small
+ * problems no longer leave the GPU idle.  Each context keeps its proposal
+ * (walked or learned) and its outputs; ray scheduling follows the first
+ * context's tuning.  Contexts must be on one device, stay owned by the
+ * caller and outlive the batch.  No counterpart in the reference (one
+ * problem per calcPhiASE run, include/core/simulation.hpp:42-333). */
+typedef struct hase_batch hase_batch_t;
+
+int hase_batch_create(uint32_t n_problems, hase_context_t* const* contexts, hase_batch_t** out_batch);
+
+/* results[i] receives context i's outputs, as hase_context_evaluate would. */
+int hase_batch_evaluate(hase_batch_t* batch, hase_result_t* const* results);
+
+/* Summed counters and device times of the last batch evaluation. */
+int hase_batch_stats(const hase_batch_t* batch, hase_stats_t* stats);
+
+void hase_batch_destroy(hase_batch_t* batch);
 
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);

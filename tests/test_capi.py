@@ -136,3 +136,13 @@ def test_tuning_entry_points_reject_bad_arguments_on_the_host():
     assert lib.hase_context_autotune(None, 0, C.byref(t)) == -1
     text = HEADER.read_text()
     assert "#define HASE_TRACE_QUEUE 0" in text and "#define HASE_TRACE_ROUNDS 1" in text
+
+
+def test_batch_entry_points_reject_bad_arguments_on_the_host():
+    lib = _native.load()
+    out = C.c_void_p()
+    assert lib.hase_batch_create(0, None, C.byref(out)) == -1
+    assert lib.hase_batch_create(1, None, None) == -1
+    assert lib.hase_batch_evaluate(None, None) == -1
+    assert lib.hase_batch_stats(None, None) == -1
+    lib.hase_batch_destroy(None)

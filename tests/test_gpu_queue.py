@@ -54,7 +54,7 @@ def test_queue_traces_the_same_rays_as_rounds(case):
 
 
 @pytest.mark.parametrize("shape", [dict(block_size=64, rays_per_item=64, queue_chunk=1, refill_lanes=1, step_budget=1),
-                                   dict(block_size=1024, rays_per_item=4096, queue_chunk=7, refill_lanes=64,
+                                   dict(block_size=512, rays_per_item=4096, queue_chunk=7, refill_lanes=64,
                                         step_budget=10001)])
 def test_queue_extreme_launch_shapes(shape):
     w = plain_small()
