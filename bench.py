@@ -147,6 +147,60 @@ def kernel_table(stats):
     return k, ray_steps, imp_steps, trace_ms, imp_ms
 
 
+def batch_bench(args):
+    """Independent seeded Delaunay disks (log-uniform triangles, 1-4 pump
+    spots, 1-16 wavelengths, adaptive rays, mse 0.005): every problem alone
+    (its own adaptive rounds, one after another) against all problems in one
+    persistent work queue per round (hase_batch).  Same rays either way."""
+    from haseongpu_amd import _native
+    from haseongpu_amd.simulation import native_problem
+    from haseongpu_amd.workloads import batch_problems
+
+    tri = tuple(int(v) for v in args.batch_triangles.split(","))
+    lev = tuple(int(v) for v in args.batch_levels.split(","))
+    rays = tuple(int(v) for v in args.batch_rays.split(","))
+    ws = batch_problems(args.batch, seed=0, triangles=tri, levels=lev, min_rays=rays[0], max_rays=rays[1])
+    ctxs = []
+    for w in ws:
+        mesh, exp, comp, _ = native_problem(w)
+        ctxs.append(_native.Context(mesh, exp, comp))
+    tuning = ctxs[0].autotune(512)
+    for c in ctxs[1:]:
+        c.set_tuning(**{k: v for k, v in tuning.items() if k != "probe_ms"})
+    samples = sum(w.samples for w in ws)
+    log(f"{len(ws)} problems, {samples} samples, {sum(w.topology.numberOfPrisms for w in ws)} prisms; tuning {tuning}")
+
+    def alone():
+        t0 = time.perf_counter()
+        out = [c.evaluate() for c in ctxs]
+        return time.perf_counter() - t0, out
+
+    batch = _native.Batch(ctxs)
+
+    def together():
+        t0 = time.perf_counter()
+        out = batch.evaluate()
+        return time.perf_counter() - t0, out
+
+    alone()
+    together()
+    ta, ra = alone()
+    tb, rb = together()
+    st = batch.stats()
+    rays_total = sum(int(r.totalRays.astype(np.int64).sum()) for r in rb)
+    same = all(np.array_equal(a.totalRays, b.totalRays) for a, b in zip(ra, rb))
+    batch.close()
+    for c in ctxs:
+        c.close()
+    out = {"measurement": "batching", "problems": len(ws), "samples": samples,
+           "triangles": [int(w.topology.numberOfTriangles) for w in ws], "levels": [int(w.topology.levels) for w in ws],
+           "rays_per_sample": list(rays), "alone_s": ta, "batched_s": tb, "speedup": ta / tb,
+           "batched_rays_per_s": rays_total / tb, "rays": rays_total, "same_ray_counts": same,
+           "all_converged": bool(all(np.all(r.mse <= 0.005) for r in rb)), "importance_ms": st["importance_ms"],
+           "trace_ms": st["trace_ms"], "tuning": tuning}
+    print(json.dumps(out), flush=True)
+
+
 def traffic_pass(w, args):
     """Kernels for the FETCH_SIZE / WRITE_SIZE counter runs: the learned
     evaluation (its hase_trace_kernel launches are the bench line's), a plain
@@ -205,6 +259,12 @@ def main():
     ap.add_argument("--rounds", action="store_true", help="one trace launch per adaptive round (no work queue)")
     ap.add_argument("--save-proposal", default=None, help="write the trained network (.npz)")
     ap.add_argument("--load-proposal", default=None, help="use a saved network instead of training one")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="batching measurement instead of the bench line: N independent problems evaluated one "
+                         "after another and together in one work queue")
+    ap.add_argument("--batch-triangles", default="200,2000")
+    ap.add_argument("--batch-levels", default="5,20")
+    ap.add_argument("--batch-rays", default="10000,10000000")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
@@ -218,6 +278,9 @@ def main():
     from haseongpu_amd import proposal as prop
     from haseongpu_amd.workloads import production_disk
 
+    if args.batch:
+        batch_bench(args)
+        return
     w = production_disk(levels=args.levels)
     if args.traffic_pass:
         traffic_pass(w, args)

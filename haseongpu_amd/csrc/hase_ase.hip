@@ -48,6 +48,8 @@ using namespace hase;
 // I the work-group sum of the thread partials `local`.  value(e) returns
 // imp_e and must be readable by every thread (the entries of other threads
 // are read after a barrier).  red holds kBlock/64 + 2 doubles of LDS.
+constexpr int kScanPer = 8;
+
 template <class F>
 __device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t P, const float* bv, double eps,
                                                double Bsum, uint32_t R, F value, double* pre, double* Qout,
@@ -70,19 +72,36 @@ __device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t 
     const bool use_imp = I > 0.0 && isfinite(I);
     const double wI = use_imp ? (1.0 - eps) / I : 0.0;
     const double wB = (use_imp ? eps : 1.0) / (Bsum * (double)R);
-    for (size_t base = 0; base < n; base += kBlock) {
-        size_t e = base + tid;
-        double q = 0.0;
-        if (e < n) {
-            uint32_t j = (uint32_t)(e % P);
-            q = wI * (double)value(e) + wB * (double)bv[j];
+    // each thread takes kScanPer consecutive entries per pass: a serial scan
+    // in registers, then one wavefront scan and one barrier round per pass
+    for (size_t base = 0; base < n; base += (size_t)kBlock * kScanPer) {
+        const size_t e0 = base + (size_t)tid * kScanPer;
+        double v[kScanPer];
+        double run = 0.0;
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            const size_t e = e0 + (size_t)i;
+            double q = 0.0;
+            if (e < n) {
+                const uint32_t j = (uint32_t)(e % P);
+                q = wI * (double)value(e) + wB * (double)bv[j];
+            }
+            run += q;
+            v[i] = run;
         }
-        double inc = wave_inclusive_scan(q, lane);
+        const double inc = wave_inclusive_scan(run, lane);
+        double before = __shfl_up(inc, 1, 64);
+        if (lane == 0) before = 0.0;
         if (lane == 63) red[wid] = inc;
         __syncthreads();
         double off = *carry_s;
         for (int i = 0; i < wid; ++i) off += red[i];
-        if (e < n) pre[e] = off + inc;
+        const double excl = off + before;
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            const size_t e = e0 + (size_t)i;
+            if (e < n) pre[e] = excl + v[i];
+        }
         __syncthreads();
         if (tid == kBlock - 1) *carry_s = off + inc;
         __syncthreads();
