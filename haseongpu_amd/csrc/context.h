@@ -63,7 +63,7 @@ struct hase_context {
     hase_proposal_stats_t pstats{};
 
     // persistent work queue
-    hase_tuning_t tuning{HASE_TRACE_ROUNDS, 256u, 1024u, 2u, 16u, 8u, 1u, 0.0};
+    hase_tuning_t tuning{HASE_TRACE_QUEUE, 512u, 256u, 1u, 0u, 16u, 1u, 0.0};
     hase::DevBuf<uint32_t> q_flags, q_k, q_extra, q_launch, q_nrays, q_target;   // [slots]
     hase::DevBuf<uint32_t> q_list, q_item_off;                                  // [slots], [slots + 1]
     hase::DevBuf<uint32_t> q_scalars;                                           // n_active, head, slots

@@ -1430,7 +1430,7 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
     static const uint32_t blocks[] = {128u, 256u, 512u};
     static const uint32_t items[] = {256u, 512u, 1024u, 2048u};
     static const uint32_t chunks[] = {1u, 2u, 4u};
-    static const uint32_t refills[] = {8u, 16u, 32u};
+    static const uint32_t refills[] = {0u, 8u, 16u, 32u};
     static const uint32_t budgets[] = {4u, 8u, 16u};
     const uint32_t first = ctx->comp.min_sample_range, last = ctx->comp.max_sample_range;
     const uint32_t want = std::min(probe ? probe : 512u, last - first + 1u);
@@ -1466,7 +1466,7 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
         sweep(blocks, 3, &hase_tuning_t::block_size);
         sweep(items, 4, &hase_tuning_t::rays_per_item);
         sweep(chunks, 3, &hase_tuning_t::queue_chunk);
-        sweep(refills, 3, &hase_tuning_t::refill_lanes);
+        sweep(refills, 4, &hase_tuning_t::refill_lanes);
         sweep(budgets, 3, &hase_tuning_t::step_budget);
     }
     cur.probe_ms = best_ms;

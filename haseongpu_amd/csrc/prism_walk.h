@@ -121,12 +121,6 @@ struct WalkState {
     uint32_t steps;
     int status;               // 0 ok, 1 left the mesh, 2 step limit
     int done;
-    // exit of the current triangle (valid while `cached`): crossing parameter,
-    // neighbour word and cladding flag; a level-plane crossing keeps the
-    // triangle, the entry edge and the ray, so the edge test is not repeated
-    float best;
-    uint32_t nw, clad;
-    int cached;
 };
 
 HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float za, uint32_t t, uint32_t layer,
@@ -161,7 +155,6 @@ HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float 
     w.entry = -1;
     w.s = 0.0f;
     w.applied = 0;
-    w.cached = 0;
 }
 
 // kReflect = false is the direct walk (start and end inside the slab), which
@@ -179,47 +172,40 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         w.steps++;
         const uint32_t t = w.t;
         const int level = kReflect ? ((w.copy & 1) ? (layers - 1 - w.r) : w.r) : w.u;
+        // the triangle record and this prism's beta are independent loads
+        const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
         const float beta = m.beta[t + (uint32_t)level * m.T];
-        const float D = w.D;
-        if (!w.cached) {
-            const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
-            const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy;
-            const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
-            const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
-            const float X2 = bits_to_float(q1.x) - ax, Y2 = bits_to_float(q1.y) - ay;
-            // edge k joins vertex k and k+1; crossing parameter num / nv for nv > 0
-            const float ex0 = X1 - X0, ey0 = Y1 - Y0;
-            const float ex1 = X2 - X1, ey1 = Y2 - Y1;
-            const float ex2 = X0 - X2, ey2 = Y0 - Y2;
-            const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
-            float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
-            float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
-            float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
-            s0 = (w.entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
-            s1 = (w.entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
-            s2 = (w.entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-            int bestk = 0;
-            float best = s0;
-            if (s1 < best) { best = s1; bestk = 1; }
-            if (s2 < best) { best = s2; bestk = 2; }
-            // the winning edge through the end's (x, y) meets the ray exactly there;
-            // compared on the stored coordinates, which no contraction of the
-            // relative ones can perturb
-            {
-                const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
-                const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
-                const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
-                const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
-                const bool touches = (bits_to_float(axb) == w.ex && bits_to_float(ayb) == w.ey) ||
-                                     (bits_to_float(bxb) == w.ex && bits_to_float(byb) == w.ey);
-                best = (touches && best < INFINITY) ? D : best;
-            }
-            w.best = best;
-            w.nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
-            w.clad = q2.y;
-            w.cached = 1;
+        const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy, D = w.D;
+        const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
+        const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
+        const float X2 = bits_to_float(q1.x) - ax, Y2 = bits_to_float(q1.y) - ay;
+        // edge k joins vertex k and k+1; crossing parameter num / nv for nv > 0
+        const float ex0 = X1 - X0, ey0 = Y1 - Y0;
+        const float ex1 = X2 - X1, ey1 = Y2 - Y1;
+        const float ex2 = X0 - X2, ey2 = Y0 - Y2;
+        const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
+        float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
+        float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
+        float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
+        s0 = (w.entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
+        s1 = (w.entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
+        s2 = (w.entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
+        int bestk = 0;
+        float best = s0;
+        if (s1 < best) { best = s1; bestk = 1; }
+        if (s2 < best) { best = s2; bestk = 2; }
+        // the winning edge through the end's (x, y) meets the ray exactly there;
+        // compared on the stored coordinates, which no contraction of the
+        // relative ones can perturb
+        {
+            const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
+            const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
+            const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
+            const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
+            const bool touches = (bits_to_float(axb) == w.ex && bits_to_float(ayb) == w.ey) ||
+                                 (bits_to_float(bxb) == w.ex && bits_to_float(byb) == w.ey);
+            best = (touches && best < INFINITY) ? D : best;
         }
-        const float best = w.best;
         // next level plane along the ray (exactly D for the end's plane)
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
         float sz = w.zdir != 0 ? ((float)plane - w.za) * w.kz : INFINITY;
@@ -227,12 +213,12 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         float snext = fminf(fminf(best, sz), D);
         snext = fmaxf(snext, w.s);
         const float seg = snext - w.s;
-        const uint32_t clad = w.clad;
+        const uint32_t clad = q2.y;
         w.clad_len += clad ? seg : 0.0f;
         w.gain_len += clad ? 0.0f : seg;
         w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
         w.s = snext;
-        const uint32_t nw = w.nw;
+        const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
         const bool zstep = sz <= best;
         bool done = w.s >= D;
         if (!done && !zstep && nw == 0xFFFFFFFFu) {
@@ -283,7 +269,6 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         w.u = zstep ? w.u + w.zdir : w.u;
         w.t = zstep ? t : (nw >> 2);
         w.entry = zstep ? w.entry : (int)(nw & 3u);
-        w.cached = zstep ? 1 : 0;
     }
 }
 

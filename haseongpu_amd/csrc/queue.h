@@ -85,8 +85,10 @@ ProblemDev queue_problem(const hase_context* ctx, uint32_t slot0, uint32_t base,
 size_t queue_items_per_slot(const hase_context* ctx, const hase_tuning_t& tu);
 
 // Runs `rounds` adaptive rounds over `slots` global slots on `stream`
-// (the state is reset first); returns the device time in ms.
+// (the state is reset first); returns the device time in ms.  `single`
+// (a batch of one problem) passes the problem as a kernel argument instead
+// of the device table.
 float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const QueueState& qs, uint32_t slots,
-                uint32_t rounds, GraphCache& gc, uint64_t generation);
+                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single = nullptr);
 
 }  // namespace hase

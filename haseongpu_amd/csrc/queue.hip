@@ -153,7 +153,16 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
     walk_begin(lr.w, m.h, ax, ay, az, t, l, c.sp.px, c.sp.py, ez, nr, fp);
 }
 
-__global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q)
+// kRefill: lanes hand finished rays back and start the item's next rays when
+// `refill` of them are idle (resumable walk); otherwise each lane walks rays
+// lane, lane + 64, ... of the item to the end (fewer live registers, higher
+// occupancy).
+//
+// kTable: per-problem constants from the device table (batches of several
+// problems); otherwise from the kernel argument `one` (a single problem), so
+// they sit in scalar registers.
+template <bool kRefill, bool kTable>
+__global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q, ProblemDev one)
 {
     const uint32_t n_active = q.scalars[kNActive];
     if (n_active == 0u) return;
@@ -164,7 +173,8 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
     for (;;) {
         uint32_t first = 0;
         if (lane == 0) first = atomicAdd(q.scalars + kHead, q.queue_chunk);
-        first = __shfl(first, 0, 64);
+        // wavefront-uniform from here on: item, slot and problem live in scalar registers
+        first = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(first, 0, 64));
         if (first >= total) break;
         const uint32_t last = min(total, first + q.queue_chunk);
         for (uint32_t item = first; item < last; ++item) {
@@ -175,7 +185,7 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                 if (q.item_off[mid] <= item) lo = mid; else hi = mid;
             }
             const uint32_t slot = q.list[lo];
-            const ProblemDev& pd = q.problems[q.slot_problem[slot]];
+            const ProblemDev& pd = kTable ? q.problems[q.slot_problem[slot]] : one;
             const MeshDev m = pd.m;
             const uint32_t local = slot - pd.slot0;
             const uint32_t nrays = q.nrays[slot];
@@ -194,47 +204,60 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
 
             double s1 = 0.0, s2 = 0.0;
             uint32_t drops = 0;
-            LaneRay lr;
-            bool have = (uint32_t)lane < cnt;
-            if (have) start_ray(m, c, c.r0 + (uint32_t)lane, lr);
-            uint32_t cursor = 64u;   // next ray of the item not yet started (wavefront-uniform)
-            for (;;) {
-                if (have && !lr.w.done) {
-                    if (lr.w.nrefl) walk_advance<true>(m, lr.w, q.budget);
-                    else walk_advance<false>(m, lr.w, q.budget);
-                }
-                if (have && lr.w.done) {
-                    const WalkOut w = walk_result(lr.w);
-                    steps += w.steps;
-                    rays++;
-                    bool ok = w.status == 0 && w.length > 0.0f;
-                    if (ok) {
-                        const float g = spectral_gain(pd.c1, pd.c2, pd.n_lambda, pd.inv_lambda, m.clad_alpha, w);
-                        const float f = g * w.refl * m.bv[lr.j] / (4.0f * kPi * w.length * w.length);
-                        const double X = (double)f * (c.Q / lr.q);
-                        if (isfinite(X)) {
-                            s1 += X;
-                            s2 += X * X;
-                        } else {
-                            ok = false;
-                        }
+            auto finish = [&](const LaneRay& lr) {
+                const WalkOut w = walk_result(lr.w);
+                steps += w.steps;
+                rays++;
+                bool ok = w.status == 0 && w.length > 0.0f;
+                if (ok) {
+                    const float g = spectral_gain(pd.c1, pd.c2, pd.n_lambda, pd.inv_lambda, m.clad_alpha, w);
+                    const float f = g * w.refl * m.bv[lr.j] / (4.0f * kPi * w.length * w.length);
+                    const double X = (double)f * (c.Q / lr.q);
+                    if (isfinite(X)) {
+                        s1 += X;
+                        s2 += X * X;
+                    } else {
+                        ok = false;
                     }
-                    drops += ok ? 0u : 1u;
-                    have = false;
                 }
-                const unsigned long long idle = __ballot(!have);
-                const uint32_t nidle = (uint32_t)__popcll(idle);
-                if (cursor >= cnt) {
-                    if (nidle == 64u) break;
-                } else if (nidle >= q.refill || nidle == 64u) {
-                    if (!have) {
-                        const uint32_t at = cursor + (uint32_t)__popcll(idle & below);
-                        if (at < cnt) {
-                            start_ray(m, c, c.r0 + at, lr);
-                            have = true;
-                        }
+                drops += ok ? 0u : 1u;
+            };
+            if (!kRefill) {
+                for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
+                    LaneRay lr;
+                    start_ray(m, c, c.r0 + i, lr);
+                    if (lr.w.nrefl) walk_advance<true>(m, lr.w, kMaxSteps + 1u);
+                    else walk_advance<false>(m, lr.w, kMaxSteps + 1u);
+                    finish(lr);
+                }
+            } else {
+                LaneRay lr;
+                bool have = (uint32_t)lane < cnt;
+                if (have) start_ray(m, c, c.r0 + (uint32_t)lane, lr);
+                uint32_t cursor = 64u;   // next ray of the item not yet started (wavefront-uniform)
+                for (;;) {
+                    if (have && !lr.w.done) {
+                        if (lr.w.nrefl) walk_advance<true>(m, lr.w, q.budget);
+                        else walk_advance<false>(m, lr.w, q.budget);
                     }
-                    cursor += nidle;
+                    if (have && lr.w.done) {
+                        finish(lr);
+                        have = false;
+                    }
+                    const unsigned long long idle = __ballot(!have);
+                    const uint32_t nidle = (uint32_t)__popcll(idle);
+                    if (cursor >= cnt) {
+                        if (nidle == 64u) break;
+                    } else if (nidle >= q.refill || nidle == 64u) {
+                        if (!have) {
+                            const uint32_t at = cursor + (uint32_t)__popcll(idle & below);
+                            if (at < cnt) {
+                                start_ray(m, c, c.r0 + at, lr);
+                                have = true;
+                            }
+                        }
+                        cursor += nidle;
+                    }
                 }
             }
             s1 = wave_sum(s1);
@@ -326,7 +349,7 @@ void check_tuning(const hase_tuning_t& t)
     if (t.rays_per_item < 64 || t.rays_per_item % 64 || t.rays_per_item > (1u << 20))
         throw HaseError(HASE_E_INVALID, "rays_per_item must be a multiple of 64 up to 2^20");
     if (t.queue_chunk < 1 || t.queue_chunk > 1024) throw HaseError(HASE_E_INVALID, "queue_chunk must be in [1, 1024]");
-    if (t.refill_lanes < 1 || t.refill_lanes > 64) throw HaseError(HASE_E_INVALID, "refill_lanes must be in [1, 64]");
+    if (t.refill_lanes > 64) throw HaseError(HASE_E_INVALID, "refill_lanes must be in [0, 64]");
     if (t.step_budget < 1 || t.step_budget > kMaxSteps + 1u)
         throw HaseError(HASE_E_INVALID, "step_budget must be in [1, 10001]");
 }
@@ -422,11 +445,18 @@ ProblemDev queue_problem(const hase_context* ctx, uint32_t slot0, uint32_t base,
 }
 
 float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const QueueState& qs, uint32_t slots,
-                uint32_t rounds, GraphCache& gc, uint64_t generation)
+                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single)
 {
     int cus = 0, per = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hase_queue_trace_kernel, (int)tu.block_size, 0));
+    const bool refill = tu.refill_lanes != 0u;
+    const bool table = single == nullptr;
+    auto kernel = refill ? (table ? hase_queue_trace_kernel<true, true> : hase_queue_trace_kernel<true, false>)
+                         : (table ? hase_queue_trace_kernel<false, true> : hase_queue_trace_kernel<false, false>);
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)tu.block_size, 0));
+    ProblemDev one;
+    std::memset(&one, 0, sizeof(one));
+    if (single) one = *single;
     const int grid = std::max(1, cus) * std::max(1, per);
     const uint32_t fin_blocks = std::max(1u, (slots + 255u) / 256u);
     hipEvent_t t0, t1;
@@ -440,13 +470,24 @@ float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const Q
     HIP_CHECK(hipGetLastError());
     auto launch_round = [&]() {
         hipLaunchKernelGGL(hase_queue_plan_kernel, dim3(1), dim3(kPlanBlock), 0, stream, qs);
-        hipLaunchKernelGGL(hase_queue_trace_kernel, dim3(grid), dim3(tu.block_size), 0, stream, qs);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(tu.block_size), 0, stream, qs, one);
         hipLaunchKernelGGL(hase_queue_finalize_kernel, dim3(fin_blocks), dim3(256), 0, stream, qs);
     };
     HIP_CHECK(hipEventRecord(t0, stream));
     if (tu.use_graph) {
-        // kernel arguments are fixed within a generation (slot count included)
-        const uint64_t key = generation * 1000003ull + slots;
+        // kernel arguments are fixed within a generation, up to the slot count and the
+        // single problem's constants (its sample base changes from batch to batch)
+        uint64_t key = 1469598103934665603ull;
+        auto mix = [&key](const void* data, size_t n) {
+            const unsigned char* b = (const unsigned char*)data;
+            for (size_t i = 0; i < n; ++i) {
+                key ^= b[i];
+                key *= 1099511628211ull;
+            }
+        };
+        mix(&generation, sizeof(generation));
+        mix(&slots, sizeof(slots));
+        mix(&one, sizeof(one));
         if (!gc.exec || gc.key != key) {
             gc.reset();
             HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -514,7 +555,7 @@ float queue_rounds(hase_context* ctx, uint32_t base, bool mapped, uint32_t slots
     const Schedule sc{ctx->exp.min_rays_per_sample, ctx->exp.max_rays_per_sample, ctx->comp.adaptive_steps,
                       ctx->comp.max_repetitions};
     const float ms = queue_run(ctx->stream, ctx->device, ctx->tuning, queue_state(ctx), slots, queue_max_rounds(sc),
-                               ctx->q_graph, ctx->q_generation);
+                               ctx->q_graph, ctx->q_generation, &pd);
     ctx->stats.trace_launches += queue_max_rounds(sc);
     return ms;
 }

@@ -295,7 +295,8 @@ int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_sta
  * here the default runs each adaptive round of a batch of samples as one
  * persistent work-queue kernel: every wavefront takes queue items (a run of
  * rays_per_item rays of one sample) from a device cursor, its lanes walk
- * those rays and refill with new ones when refill_lanes of them are idle,
+ * those rays (optionally refilling with new ones when refill_lanes of them
+ * are idle),
  * and the per-sample schedule lives on the device, so the rounds of a batch
  * replay as one captured hipGraph with a single host synchronisation.
  * ---------------------------------------------------------------------- */
@@ -307,7 +308,7 @@ typedef struct hase_tuning {
     uint32_t block_size;      /* queue kernel threads per work-group: 64..512, multiple of 64 */
     uint32_t rays_per_item;   /* rays per queue item, multiple of 64 */
     uint32_t queue_chunk;     /* items a wavefront takes per queue access, >= 1 */
-    uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64 */
+    uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64; 0: each lane walks its rays to the end */
     uint32_t step_budget;     /* prism steps between refill checks, >= 1 */
     uint32_t use_graph;       /* 1: replay the rounds of a batch as a captured hipGraph */
     double probe_ms;          /* hase_context_autotune: device time of the chosen setting's probe */

@@ -39,7 +39,8 @@ def test_batch_equals_each_problem_alone():
     for a, b in zip(alone, together):
         _same(a, b)
     assert ctxs[-1].proposal_stats()["fallback_samples"] == fallback_alone > 0
-    assert st["rays"] == sum(int(r.totalRays.astype(np.int64).sum()) for r in together)
+    # traced rays include the first launch of the samples that restarted on the walked map
+    assert st["rays"] == sum(int(r.totalRays.astype(np.int64).sum()) for r in together) + fallback_alone * 4096
     assert any(len(np.unique(r.totalRays)) > 1 for r in together)   # uneven budgets
     for c in ctxs:
         c.close()
