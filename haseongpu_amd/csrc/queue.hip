@@ -122,7 +122,13 @@ struct ItemCtx {
     uint32_t sample, launch, r0, seed;
 };
 
-__device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, LaneRay& lr)
+// Start point, end image and proposal probability of ray `ray` of an item.
+struct RayStart {
+    float ax, ay, az, ez;
+    uint32_t t, l, nr, fp;
+};
+
+__device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, double& q, uint32_t& jout)
 {
     const uint32_t T = m.T, P = T * m.layers;
     const double target = ((double)ray + c.U) * c.scale;
@@ -133,10 +139,10 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
         if (c.pre[mid] > target) hi = mid; else lo = mid + 1;
     }
     const size_t e = lo < c.n ? lo : c.n - 1;
-    lr.q = c.pre[e] - (e ? c.pre[e - 1] : 0.0);
+    q = c.pre[e] - (e ? c.pre[e - 1] : 0.0);
     const uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
     const uint32_t t = j % T, l = j / T;
-    lr.j = j;
+    jout = j;
     uint32_t rnd[4];
     philox4x32_10(ray, c.sample, c.launch, kTagRay, c.seed, kKeyTag, rnd);
     float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
@@ -145,18 +151,24 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
     const float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
     const float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
     const float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
-    const float ax = x0 + u1 * (x1 - x0) + u2 * (x2 - x0);
-    const float ay = y0 + u1 * (y1 - y0) + u2 * (y2 - y0);
-    const float az = (float)l + u3;   // layer units
-    const uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
-    const float ez = (float)image_plane(c.sp.plane, (int)m.layers, nr, fp);
-    walk_begin(lr.w, m.h, ax, ay, az, t, l, c.sp.px, c.sp.py, ez, nr, fp);
+    RayStart r;
+    r.ax = x0 + u1 * (x1 - x0) + u2 * (x2 - x0);
+    r.ay = y0 + u1 * (y1 - y0) + u2 * (y2 - y0);
+    r.az = (float)l + u3;   // layer units
+    r.t = t;
+    r.l = l;
+    r.nr = (k + 1u) >> 1;
+    r.fp = k & 1u;
+    r.ez = (float)image_plane(c.sp.plane, (int)m.layers, r.nr, r.fp);
+    return r;
 }
 
-// kRefill: lanes hand finished rays back and start the item's next rays when
-// `refill` of them are idle (resumable walk); otherwise each lane walks rays
-// lane, lane + 64, ... of the item to the end (fewer live registers, higher
-// occupancy).
+__device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, LaneRay& lr)
+{
+    const RayStart r = draw_ray(m, c, ray, lr.q, lr.j);
+    walk_begin(lr.w, m.h, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp);
+}
+
 //
 // kTable: per-problem constants from the device table (batches of several
 // problems); otherwise from the kernel argument `one` (a single problem), so
@@ -204,15 +216,14 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
 
             double s1 = 0.0, s2 = 0.0;
             uint32_t drops = 0;
-            auto finish = [&](const LaneRay& lr) {
-                const WalkOut w = walk_result(lr.w);
+            auto finish = [&](const WalkOut& w, double qprob, uint32_t j) {
                 steps += w.steps;
                 rays++;
                 bool ok = w.status == 0 && w.length > 0.0f;
                 if (ok) {
                     const float g = spectral_gain(pd.c1, pd.c2, pd.n_lambda, pd.inv_lambda, m.clad_alpha, w);
-                    const float f = g * w.refl * m.bv[lr.j] / (4.0f * kPi * w.length * w.length);
-                    const double X = (double)f * (c.Q / lr.q);
+                    const float f = g * w.refl * m.bv[j] / (4.0f * kPi * w.length * w.length);
+                    const double X = (double)f * (c.Q / qprob);
                     if (isfinite(X)) {
                         s1 += X;
                         s2 += X * X;
@@ -224,11 +235,11 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
             };
             if (!kRefill) {
                 for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
-                    LaneRay lr;
-                    start_ray(m, c, c.r0 + i, lr);
-                    if (lr.w.nrefl) walk_advance<true>(m, lr.w, kMaxSteps + 1u);
-                    else walk_advance<false>(m, lr.w, kMaxSteps + 1u);
-                    finish(lr);
+                    double qprob;
+                    uint32_t j;
+                    const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j);
+                    const WalkOut w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp);
+                    finish(w, qprob, j);
                 }
             } else {
                 LaneRay lr;
@@ -241,7 +252,7 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                         else walk_advance<false>(m, lr.w, q.budget);
                     }
                     if (have && lr.w.done) {
-                        finish(lr);
+                        finish(walk_result(lr.w), lr.q, lr.j);
                         have = false;
                     }
                     const unsigned long long idle = __ballot(!have);
