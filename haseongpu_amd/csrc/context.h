@@ -59,6 +59,7 @@ struct hase_context {
     hase::DevBuf<float> mlp_vec;             // FP32 biases and output weights
     std::vector<float4> clusters_host;       // [C] (x, y, level, beta) representatives
     hase::DevBuf<float4> clusters;
+    hase::DevBuf<uint32_t> cl_tri, cl_lev;          // cluster column per triangle, cluster row start per layer
     hase::DevBuf<float> logits;              // [slots * R * C]
     hase_proposal_stats_t pstats{};
 

@@ -50,10 +50,30 @@ using namespace hase;
 // are read after a barrier).  red holds kBlock/64 + 2 doubles of LDS.
 constexpr int kScanPer = 8;
 
+// (slice k, prism j = t + l*T) of a proposal entry e = k*P + j, advanced one
+// entry at a time so the scans need no per-entry integer division.
+struct EntryIndex {
+    uint32_t k, j, t, l;
+    __device__ __forceinline__ void init(uint32_t e, uint32_t P, uint32_t T)
+    {
+        k = e / P;
+        j = e - k * P;
+        l = j / T;
+        t = j - l * T;
+    }
+    __device__ __forceinline__ void next(uint32_t P, uint32_t T)
+    {
+        ++j;
+        ++t;
+        if (t == T) { t = 0; ++l; }
+        if (j == P) { j = 0; t = 0; l = 0; ++k; }
+    }
+};
+
 template <class F>
-__device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t P, const float* bv, double eps,
-                                               double Bsum, uint32_t R, F value, double* pre, double* Qout,
-                                               double* red)
+__device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_t P, uint32_t T, const float* bv,
+                                               double eps, double Bsum, uint32_t R, F value, double* pre,
+                                               double* Qout, double* red)
 {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double* carry_s = red + kBlock / 64;
@@ -74,20 +94,20 @@ __device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t 
     const double wB = (use_imp ? eps : 1.0) / (Bsum * (double)R);
     // each thread takes kScanPer consecutive entries per pass: a serial scan
     // in registers, then one wavefront scan and one barrier round per pass
-    for (size_t base = 0; base < n; base += (size_t)kBlock * kScanPer) {
-        const size_t e0 = base + (size_t)tid * kScanPer;
+    for (uint32_t base = 0; base < n; base += (uint32_t)kBlock * kScanPer) {
+        const uint32_t e0 = base + (uint32_t)tid * kScanPer;
         double v[kScanPer];
         double run = 0.0;
+        EntryIndex x;
+        x.init(e0 < n ? e0 : 0u, P, T);
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
-            const size_t e = e0 + (size_t)i;
+            const uint32_t e = e0 + (uint32_t)i;
             double q = 0.0;
-            if (e < n) {
-                const uint32_t j = (uint32_t)(e % P);
-                q = wI * (double)value(e) + wB * (double)bv[j];
-            }
+            if (e < n) q = wI * (double)value(x, e) + wB * (double)bv[x.j];
             run += q;
             v[i] = run;
+            x.next(P, T);
         }
         const double inc = wave_inclusive_scan(run, lane);
         double before = __shfl_up(inc, 1, 64);
@@ -99,7 +119,7 @@ __device__ __forceinline__ void mixture_prefix(double local, size_t n, uint32_t 
         const double excl = off + before;
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
-            const size_t e = e0 + (size_t)i;
+            const uint32_t e = e0 + (uint32_t)i;
             if (e < n) pre[e] = excl + v[i];
         }
         __syncthreads();
@@ -343,7 +363,8 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         atomicAdd(&p.counters[3], ss);
     }
     if (p.transfer) return;
-    mixture_prefix(local, n, P, m.bv, p.eps, p.Bsum, p.R, [&](size_t e) { return imp[e]; }, pre, p.Q + slot, red);
+    mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
+                   [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red);
 }
 
 // Learned proposal for one sample per work-group: the network's log transfer
@@ -356,6 +377,8 @@ struct LearnedParams {
     const float2* pts;
     const float2* centroid;   // [T] device order
     const float* logits;      // [slots * R * C]
+    const uint32_t* cl_tri;   // [T] cluster column of each device triangle
+    const uint32_t* cl_lev;   // [layers] first cluster of each layer's cluster row
     uint32_t slots, sample_base, R, C, TG, ct, cl;
     double eps, Bsum;
     double* prefix;
@@ -363,19 +386,15 @@ struct LearnedParams {
 };
 
 __device__ __forceinline__ float learned_importance(const LearnedParams& p, const SampleGeom& sp, const float* lg,
-                                                    size_t e)
+                                                    const EntryIndex& x, float zi)
 {
     const MeshDev& m = p.m;
-    const uint32_t P = m.T * m.layers;
-    const uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
-    const uint32_t t = j % m.T, l = j / m.T;
-    const uint32_t c = (l / p.cl) * p.TG + t / p.ct;
-    const float y = fminf(lg[(size_t)k * p.C + c], 80.0f);
-    const float2 cxy = p.centroid[t];
-    const float zi = (float)image_plane(sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
-    const float dx = cxy.x - sp.px, dy = cxy.y - sp.py, dz = ((float)l + 0.5f - zi) * m.h;
+    const uint32_t c = p.cl_lev[x.l] + p.cl_tri[x.t];
+    const float y = fminf(lg[(size_t)x.k * p.C + c], 80.0f);
+    const float2 cxy = p.centroid[x.t];
+    const float dx = cxy.x - sp.px, dy = cxy.y - sp.py, dz = ((float)x.l + 0.5f - zi) * m.h;
     const float d2 = dx * dx + dy * dy + dz * dz;
-    float v = m.bv[j] * __expf(y) / (4.0f * kPi * d2);
+    float v = m.bv[x.j] * __expf(y) / (4.0f * kPi * d2);
     return (v >= 0.0f && !isinf(v)) ? v : 0.0f;
 }
 
@@ -384,16 +403,37 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
     const uint32_t slot = blockIdx.x;
     if (slot >= p.slots) return;
     const MeshDev& m = p.m;
-    const uint32_t P = m.T * m.layers;
-    const size_t n = (size_t)p.R * P;
+    const uint32_t T = m.T, P = T * m.layers;
+    const uint32_t n = p.R * P;
     const SampleGeom sp = sample_point(p.pts, m.np, p.sample_base + slot);
     const float* lg = p.logits + (size_t)slot * p.R * p.C;
+    // image plane of each slice (R <= 2 * reflection cap + 1)
+    auto plane_of = [&](uint32_t k) {
+        return (float)image_plane(sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
+    };
     __shared__ double red[kBlock / 64 + 2];
     double local = 0.0;
-    for (size_t e = threadIdx.x; e < n; e += kBlock) local += (double)learned_importance(p, sp, lg, e);
-    mixture_prefix(local, n, P, m.bv, p.eps, p.Bsum, p.R,
-                   [&](size_t e) { return learned_importance(p, sp, lg, e); }, p.prefix + (size_t)slot * n,
-                   p.Q + slot, red);
+    for (uint32_t base = 0; base < n; base += (uint32_t)kBlock * kScanPer) {
+        const uint32_t e0 = base + threadIdx.x * kScanPer;
+        if (e0 >= n) continue;
+        EntryIndex x;
+        x.init(e0, P, T);
+        float zi = plane_of(x.k);
+        uint32_t kc = x.k;
+        for (int i = 0; i < kScanPer && e0 + (uint32_t)i < n; ++i) {
+            if (x.k != kc) { kc = x.k; zi = plane_of(kc); }
+            local += (double)learned_importance(p, sp, lg, x, zi);
+            x.next(P, T);
+        }
+    }
+    uint32_t kc = 0xFFFFFFFFu;
+    float zi = 0.0f;
+    mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R,
+                   [&](const EntryIndex& x, uint32_t) {
+                       if (x.k != kc) { kc = x.k; zi = plane_of(kc); }
+                       return learned_importance(p, sp, lg, x, zi);
+                   },
+                   p.prefix + (size_t)slot * n, p.Q + slot, red);
 }
 
 struct TraceParams {
@@ -866,6 +906,8 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t
     const hase_experiment_t& e = ctx->exp;
     const uint32_t R = slices(ctx);
     const size_t n_ent = (size_t)R * ctx->P;
+    if (n_ent >= ((size_t)1 << 32))
+        throw HaseError(HASE_E_INVALID, "reflection slices times prisms must stay below 2^32");
     const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
     const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
     const uint32_t rays_per_block = kBlock * rpt;
@@ -925,6 +967,8 @@ void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, u
         lp.pts = ctx->pts.p;
         lp.centroid = ctx->centroid.p;
         lp.logits = ctx->logits.p;
+        lp.cl_tri = ctx->cl_tri.p;
+        lp.cl_lev = ctx->cl_lev.p;
         lp.slots = slots;
         lp.sample_base = b0;
         lp.R = R;

@@ -570,6 +570,11 @@ void build_clusters(hase_context* ctx)
                 make_float4((float)(x / a), (float)(y / a), (float)(lev / nl), (float)(bvs / (a * nl)));
         }
     ctx->clusters.upload(ctx->clusters_host.data(), ctx->C, ctx->stream);
+    std::vector<uint32_t> ctri(T), clev(layers);
+    for (uint32_t t = 0; t < T; ++t) ctri[t] = t / ct;
+    for (uint32_t l = 0; l < layers; ++l) clev[l] = (l / cl) * ctx->TG;
+    ctx->cl_tri.upload(ctri.data(), T, ctx->stream);
+    ctx->cl_lev.upload(clev.data(), layers, ctx->stream);
     ctx->pstats.clusters = ctx->C;
 }
 

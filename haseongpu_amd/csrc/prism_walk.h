@@ -202,9 +202,10 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
             const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
             const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
             const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
-            const bool touches = (bits_to_float(axb) == w.ex && bits_to_float(ayb) == w.ey) ||
-                                 (bits_to_float(bxb) == w.ex && bits_to_float(byb) == w.ey);
-            best = (touches && best < INFINITY) ? D : best;
+            // bitwise, not short-circuit: no branches (and exec-mask traffic) per step
+            const int touches = ((int)(bits_to_float(axb) == w.ex) & (int)(bits_to_float(ayb) == w.ey)) |
+                                ((int)(bits_to_float(bxb) == w.ex) & (int)(bits_to_float(byb) == w.ey));
+            best = (touches & (int)(best < INFINITY)) ? D : best;
         }
         // next level plane along the ray (exactly D for the end's plane)
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
