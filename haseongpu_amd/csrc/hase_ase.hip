@@ -50,8 +50,7 @@ using namespace hase;
 // are read after a barrier).  red holds kBlock/64 + 2 doubles of LDS.
 constexpr int kScanPer = 8;
 
-// (slice k, prism j = t + l*T) of a proposal entry e = k*P + j, advanced one
-// entry at a time so the scans need no per-entry integer division.
+// (slice k, prism j = t + l*T) of a proposal entry e = k*P + j.
 struct EntryIndex {
     uint32_t k, j, t, l;
     __device__ __forceinline__ void init(uint32_t e, uint32_t P, uint32_t T)
@@ -61,19 +60,24 @@ struct EntryIndex {
         l = j / T;
         t = j - l * T;
     }
-    __device__ __forceinline__ void next(uint32_t P, uint32_t T)
-    {
-        ++j;
-        ++t;
-        if (t == T) { t = 0; ++l; }
-        if (j == P) { j = 0; t = 0; l = 0; ++k; }
-    }
 };
 
+// LDS for one pass of mixture_prefix: kBlock * kScanPer doubles, one pad
+// slot per kScanPer so a thread's consecutive run is bank-conflict free.
+constexpr int kScanLds = kBlock * kScanPer + kBlock * kScanPer / kScanPer;
+__device__ __forceinline__ int scan_slot(int i) { return i + i / kScanPer; }
+
+// Defensive-mixture proposal and its FP64 inclusive prefix for one sample
+// (one work-group): q_e = (1 - eps) imp_e / I + eps * bv_j / (R * Bsum), with
+// I the work-group sum of the thread partials `local`.  value(x, e) returns
+// imp_e.  Each pass covers kBlock * kScanPer entries: they are computed and
+// stored in coalesced order (entry base + i*kBlock + tid), while the scan
+// runs over kScanPer consecutive entries per thread, transposed through LDS.
+// red holds kBlock/64 + 2 doubles, buf kScanLds doubles of LDS.
 template <class F>
 __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_t P, uint32_t T, const float* bv,
                                                double eps, double Bsum, uint32_t R, F value, double* pre,
-                                               double* Qout, double* red)
+                                               double* Qout, double* red, double* buf)
 {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double* carry_s = red + kBlock / 64;
@@ -92,22 +96,25 @@ __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_
     const bool use_imp = I > 0.0 && isfinite(I);
     const double wI = use_imp ? (1.0 - eps) / I : 0.0;
     const double wB = (use_imp ? eps : 1.0) / (Bsum * (double)R);
-    // each thread takes kScanPer consecutive entries per pass: a serial scan
-    // in registers, then one wavefront scan and one barrier round per pass
     for (uint32_t base = 0; base < n; base += (uint32_t)kBlock * kScanPer) {
-        const uint32_t e0 = base + (uint32_t)tid * kScanPer;
-        double v[kScanPer];
-        double run = 0.0;
-        EntryIndex x;
-        x.init(e0 < n ? e0 : 0u, P, T);
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
-            const uint32_t e = e0 + (uint32_t)i;
+            const uint32_t e = base + (uint32_t)(i * kBlock + tid);
             double q = 0.0;
-            if (e < n) q = wI * (double)value(x, e) + wB * (double)bv[x.j];
-            run += q;
+            if (e < n) {
+                EntryIndex x;
+                x.init(e, P, T);
+                q = wI * (double)value(x, e) + wB * (double)bv[x.j];
+            }
+            buf[scan_slot(i * kBlock + tid)] = q;
+        }
+        __syncthreads();
+        double v[kScanPer];
+        double run = 0.0;
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            run += buf[scan_slot(tid * kScanPer + i)];
             v[i] = run;
-            x.next(P, T);
         }
         const double inc = wave_inclusive_scan(run, lane);
         double before = __shfl_up(inc, 1, 64);
@@ -118,11 +125,13 @@ __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_
         for (int i = 0; i < wid; ++i) off += red[i];
         const double excl = off + before;
 #pragma unroll
-        for (int i = 0; i < kScanPer; ++i) {
-            const uint32_t e = e0 + (uint32_t)i;
-            if (e < n) pre[e] = excl + v[i];
-        }
+        for (int i = 0; i < kScanPer; ++i) buf[scan_slot(tid * kScanPer + i)] = excl + v[i];
         __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kScanPer; ++i) {
+            const uint32_t e = base + (uint32_t)(i * kBlock + tid);
+            if (e < n) pre[e] = buf[scan_slot(i * kBlock + tid)];
+        }
         if (tid == kBlock - 1) *carry_s = off + inc;
         __syncthreads();
     }
@@ -319,6 +328,7 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     double* pre = p.prefix + (size_t)slot * n;
     const int tid = threadIdx.x, lane = tid & 63;
     __shared__ double red[kBlock / 64 + 2];
+    __shared__ double buf[kScanLds];
 
     double local = 0.0;
     unsigned long long walks = 0, steps = 0;
@@ -364,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     }
     if (p.transfer) return;
     mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
-                   [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red);
+                   [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf);
 }
 
 // Learned proposal for one sample per work-group: the network's log transfer
@@ -407,33 +417,20 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
     const uint32_t n = p.R * P;
     const SampleGeom sp = sample_point(p.pts, m.np, p.sample_base + slot);
     const float* lg = p.logits + (size_t)slot * p.R * p.C;
-    // image plane of each slice (R <= 2 * reflection cap + 1)
-    auto plane_of = [&](uint32_t k) {
-        return (float)image_plane(sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
+    auto value = [&](const EntryIndex& x, uint32_t) {
+        const float zi = (float)image_plane(sp.plane, (int)m.layers, (x.k + 1u) >> 1, x.k & 1u);
+        return learned_importance(p, sp, lg, x, zi);
     };
     __shared__ double red[kBlock / 64 + 2];
+    __shared__ double buf[kScanLds];
     double local = 0.0;
-    for (uint32_t base = 0; base < n; base += (uint32_t)kBlock * kScanPer) {
-        const uint32_t e0 = base + threadIdx.x * kScanPer;
-        if (e0 >= n) continue;
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
         EntryIndex x;
-        x.init(e0, P, T);
-        float zi = plane_of(x.k);
-        uint32_t kc = x.k;
-        for (int i = 0; i < kScanPer && e0 + (uint32_t)i < n; ++i) {
-            if (x.k != kc) { kc = x.k; zi = plane_of(kc); }
-            local += (double)learned_importance(p, sp, lg, x, zi);
-            x.next(P, T);
-        }
+        x.init(e, P, T);
+        local += (double)value(x, e);
     }
-    uint32_t kc = 0xFFFFFFFFu;
-    float zi = 0.0f;
-    mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R,
-                   [&](const EntryIndex& x, uint32_t) {
-                       if (x.k != kc) { kc = x.k; zi = plane_of(kc); }
-                       return learned_importance(p, sp, lg, x, zi);
-                   },
-                   p.prefix + (size_t)slot * n, p.Q + slot, red);
+    mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, value, p.prefix + (size_t)slot * n, p.Q + slot, red,
+                   buf);
 }
 
 struct TraceParams {
