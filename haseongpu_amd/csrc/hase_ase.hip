@@ -157,6 +157,7 @@ struct ImportanceParams {
     const uint32_t* active;   // block -> slot (nullptr: block index is the slot)
     const uint32_t* sample_of;// slot -> sample (nullptr: sample_base + slot)
     int transfer;             // write the transfer factor gain * reflectance instead of the importance
+    uint32_t parts;           // work-groups per sample; > 1: importance only, hase_prefix_kernel follows
 };
 
 // Levels whose centroid walks share one xy walk in the importance kernel.
@@ -317,7 +318,8 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
 
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
-    const uint32_t slot = p.active ? p.active[blockIdx.x] : blockIdx.x;
+    const uint32_t row = blockIdx.x / p.parts, part = blockIdx.x % p.parts;
+    const uint32_t slot = p.active ? p.active[row] : row;
     if (slot >= p.slots) return;
     const MeshDev& m = p.m;
     const uint32_t T = m.T, P = T * m.layers;
@@ -334,7 +336,10 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     unsigned long long walks = 0, steps = 0;
     const uint32_t groups = (m.layers + kLevelGroup - 1) / kLevelGroup;
     const uint32_t items = T * p.R * groups;
-    for (uint32_t it = tid; it < items; it += kBlock) {
+    // this work-group's share of the sample's column walks
+    const uint32_t it0 = (uint32_t)((uint64_t)items * part / p.parts);
+    const uint32_t it1 = (uint32_t)((uint64_t)items * (part + 1u) / p.parts);
+    for (uint32_t it = it0 + tid; it < it1; it += kBlock) {
         const uint32_t t0 = it % T, rest = it / T;
         const uint32_t gi = rest % groups, k = rest / groups;
         const int lv0 = (int)(gi * kLevelGroup);
@@ -372,9 +377,26 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         atomicAdd(&p.counters[2], ws);
         atomicAdd(&p.counters[3], ss);
     }
-    if (p.transfer) return;
+    if (p.transfer || p.parts > 1u) return;
     mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
                    [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf);
+}
+
+// Prefix of importance maps written by split work-groups (one per sample here).
+__global__ void __launch_bounds__(kBlock) hase_prefix_kernel(ImportanceParams p)
+{
+    const uint32_t slot = p.active ? p.active[blockIdx.x] : blockIdx.x;
+    if (slot >= p.slots) return;
+    const MeshDev& m = p.m;
+    const uint32_t T = m.T, P = T * m.layers;
+    const uint32_t n = p.R * P;
+    const float* imp = p.imp + (size_t)slot * n;
+    __shared__ double red[kBlock / 64 + 2];
+    __shared__ double buf[kScanLds];
+    double local = 0.0;
+    for (uint32_t e = threadIdx.x; e < n; e += kBlock) local += (double)imp[e];
+    mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, [&](const EntryIndex&, uint32_t e) { return imp[e]; },
+                   p.prefix + (size_t)slot * n, p.Q + slot, red, buf);
 }
 
 // Learned proposal for one sample per work-group: the network's log transfer
@@ -947,6 +969,17 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t
     return batch;
 }
 
+// Work-groups per sample for the walked importance map: enough to give a
+// small batch two work-groups per CU.
+uint32_t importance_parts(const hase_context* ctx, uint32_t slots)
+{
+    int cus = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const uint32_t want = 2u * (uint32_t)std::max(1, cus);
+    if (slots >= want) return 1u;
+    return std::min<uint32_t>(64u, (want + slots - 1u) / slots);
+}
+
 // Proposal prefix and Q of slots [0, slots): the walked importance map or the
 // learned proposal (which needs contiguous samples b0 + slot).
 void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, uint32_t slots, bool learned,
@@ -1001,9 +1034,15 @@ void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, u
         ip.active = nullptr;
         ip.sample_of = sample_of;
         ip.transfer = 0;
+        // a small batch would leave most of the chip idle: split each sample's walks
+        ip.parts = importance_parts(ctx, slots);
         hipEvent_t i0 = record(ctx);
-        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
         HIP_CHECK(hipGetLastError());
+        if (ip.parts > 1u) {
+            hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+            HIP_CHECK(hipGetLastError());
+        }
         tm.imp.emplace_back(i0, record(ctx));
         ctx->stats.importance_launches++;
     }
@@ -1686,7 +1725,10 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
         ip.active = nullptr;
         ip.sample_of = nullptr;
         ip.transfer = 0;
-        hipLaunchKernelGGL(hase_importance_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
+        ip.parts = importance_parts(context, 1u);
+        hipLaunchKernelGGL(hase_importance_kernel, dim3(ip.parts), dim3(kBlock), 0, context->stream, ip);
+        HIP_CHECK(hipGetLastError());
+        if (ip.parts > 1u) hipLaunchKernelGGL(hase_prefix_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
         HIP_CHECK(hipGetLastError());
         std::vector<float> dev(n_ent);
         HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, n_ent * sizeof(float), hipMemcpyDeviceToHost,
@@ -1774,7 +1816,8 @@ int hase_context_transfer_map(hase_context_t* context, uint32_t first, uint32_t 
             ip.active = nullptr;
             ip.sample_of = nullptr;
             ip.transfer = 1;
-            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots), dim3(kBlock), 0, context->stream, ip);
+            ip.parts = importance_parts(context, slots);
+            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, context->stream, ip);
             HIP_CHECK(hipGetLastError());
             HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, (size_t)slots * n_ent * sizeof(float),
                                      hipMemcpyDeviceToHost, context->stream));
