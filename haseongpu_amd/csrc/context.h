@@ -65,7 +65,7 @@ struct hase_context {
 
     // persistent work queue
     hase_tuning_t tuning{HASE_TRACE_QUEUE, 512u, 256u, 1u, 0u, 16u, 1u, 0.0};
-    hase::DevBuf<uint32_t> q_flags, q_k, q_extra, q_launch, q_nrays, q_target;   // [slots]
+    hase::DevBuf<uint32_t> q_flags, q_k, q_extra, q_launch, q_nrays, q_item_rays, q_target;   // [slots]
     hase::DevBuf<uint32_t> q_list, q_item_off;                                  // [slots], [slots + 1]
     hase::DevBuf<uint32_t> q_scalars;                                           // n_active, head, slots
     hase::DevBuf<uint32_t> q_targets;                                           // schedule targets by step

@@ -934,7 +934,7 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t
     uint32_t batch = c.sample_batch;
     if (batch == 0) {
         const uint32_t per_item = ctx->tuning.mode == HASE_TRACE_QUEUE
-                                      ? (e.max_rays_per_sample + ctx->tuning.rays_per_item - 1) / ctx->tuning.rays_per_item
+                                      ? (uint32_t)queue_items_per_slot(ctx, ctx->tuning)
                                       : max_chunks;
         size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + (learned ? (size_t)R * ctx->C * 4u : 0u);
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
@@ -1274,7 +1274,8 @@ struct hase_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hase::DevBuf<hase::ProblemDev> problems;
-    hase::DevBuf<uint32_t> slot_problem, flags, k, extra, launch, nrays, target, list, item_off, scalars, N, dropped;
+    hase::DevBuf<uint32_t> slot_problem, flags, k, extra, launch, nrays, item_rays, target, list, item_off, scalars, N,
+        dropped;
     hase::DevBuf<double> s1, s2, mse, se;
     hase::DevBuf<float> phi;
     hase::DevBuf<double2> partial;
@@ -1363,8 +1364,8 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
                            tm[part[w]]);
         }
         for (size_t w = 0; w < part.size(); ++w) HIP_CHECK(hipStreamSynchronize(b->ctx[part[w]]->stream));
-        for (auto* buf : {&b->slot_problem, &b->flags, &b->k, &b->extra, &b->launch, &b->nrays, &b->target, &b->list,
-                          &b->N, &b->dropped})
+        for (auto* buf : {&b->slot_problem, &b->flags, &b->k, &b->extra, &b->launch, &b->nrays, &b->item_rays,
+                          &b->target, &b->list, &b->N, &b->dropped})
             buf->alloc(total);
         b->item_off.alloc((size_t)total + 1u);
         b->scalars.alloc(kQueueScalars);
@@ -1401,6 +1402,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         qs.extra = b->extra.p;
         qs.launch = b->launch.p;
         qs.nrays = b->nrays.p;
+        qs.item_rays = b->item_rays.p;
         qs.target = b->target.p;
         qs.list = b->list.p;
         qs.item_off = b->item_off.p;

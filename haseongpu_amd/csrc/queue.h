@@ -42,6 +42,7 @@ struct QueueState {
     uint32_t* extra;
     uint32_t* launch;
     uint32_t* nrays;
+    uint32_t* item_rays;            // rays per queue item of each slot this round
     uint32_t* target;
     uint32_t* list;                 // [slots] listed slots of the round
     uint32_t* item_off;             // [slots + 1] first queue item of each listed slot
@@ -61,6 +62,9 @@ struct QueueState {
 
 constexpr uint32_t kQueueActive = 1u, kQueueDeferred = 2u;
 constexpr int kQueueScalars = 4;
+// Queue items of one slot in one round at most: larger rounds get longer
+// items, so the per-item partials stay at 16 KB per slot.
+constexpr uint32_t kQueueItemsPerSlot = 1024u;
 
 struct GraphCache {
     hipGraph_t graph = nullptr;

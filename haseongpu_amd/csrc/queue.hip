@@ -74,8 +74,12 @@ __global__ void __launch_bounds__(kPlanBlock) hase_queue_plan_kernel(QueueState 
             if (nr == 0u) q.flags[s] &= ~kQueueActive;
         }
         q.nrays[s] = nr;
+        // item length: the tuned one, or longer so the round fits kQueueItemsPerSlot items
+        uint32_t ir = max(q.rays_per_item, (nr + kQueueItemsPerSlot - 1u) / kQueueItemsPerSlot);
+        ir = (ir + 63u) & ~63u;
+        q.item_rays[s] = ir;
         if (nr) {
-            my_items += (nr + q.rays_per_item - 1u) / q.rays_per_item;
+            my_items += (nr + ir - 1u) / ir;
             my_slots++;
         }
     }
@@ -97,7 +101,7 @@ __global__ void __launch_bounds__(kPlanBlock) hase_queue_plan_kernel(QueueState 
         if (!nr) continue;
         q.list[at] = s;
         q.item_off[at] = item;
-        item += (nr + q.rays_per_item - 1u) / q.rays_per_item;
+        item += (nr + q.item_rays[s] - 1u) / q.item_rays[s];
         at++;
     }
     if (tid == kPlanBlock - 1) {
@@ -201,9 +205,10 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
             const MeshDev m = pd.m;
             const uint32_t local = slot - pd.slot0;
             const uint32_t nrays = q.nrays[slot];
+            const uint32_t ir = q.item_rays[slot];
             ItemCtx c;
-            c.r0 = (item - q.item_off[lo]) * q.rays_per_item;
-            const uint32_t cnt = min(q.rays_per_item, nrays - c.r0);
+            c.r0 = (item - q.item_off[lo]) * ir;
+            const uint32_t cnt = min(ir, nrays - c.r0);
             c.sample = pd.sample_of ? pd.sample_of[local] : pd.sample_base + local;
             c.launch = q.launch[slot];
             c.seed = pd.seed;
@@ -389,7 +394,8 @@ uint32_t queue_max_rounds(const Schedule& s)
 
 size_t queue_items_per_slot(const hase_context* ctx, const hase_tuning_t& tu)
 {
-    return (ctx->exp.max_rays_per_sample + tu.rays_per_item - 1) / tu.rays_per_item;
+    const size_t need = (ctx->exp.max_rays_per_sample + tu.rays_per_item - 1) / tu.rays_per_item;
+    return std::min<size_t>(need, kQueueItemsPerSlot);
 }
 
 void queue_targets(hase_context* ctx)
@@ -409,6 +415,7 @@ void queue_reserve(hase_context* ctx, uint32_t batch)
     ctx->q_extra.alloc(batch);
     ctx->q_launch.alloc(batch);
     ctx->q_nrays.alloc(batch);
+    ctx->q_item_rays.alloc(batch);
     ctx->q_target.alloc(batch);
     ctx->q_list.alloc(batch);
     ctx->q_item_off.alloc((size_t)batch + 1u);
@@ -534,6 +541,7 @@ QueueState queue_state(hase_context* ctx)
     qs.extra = ctx->q_extra.p;
     qs.launch = ctx->q_launch.p;
     qs.nrays = ctx->q_nrays.p;
+    qs.item_rays = ctx->q_item_rays.p;
     qs.target = ctx->q_target.p;
     qs.list = ctx->q_list.p;
     qs.item_off = ctx->q_item_off.p;
