@@ -47,6 +47,13 @@ EXPORTED_SYMBOLS = (
     "hase_batch_evaluate",
     "hase_batch_stats",
     "hase_batch_destroy",
+    "hase_simulation_create",
+    "hase_simulation_set_pump",
+    "hase_simulation_set_beta",
+    "hase_simulation_step",
+    "hase_simulation_state",
+    "hase_simulation_times",
+    "hase_simulation_destroy",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -197,6 +204,46 @@ class Tuning(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class RunControl(C.Structure):
+    _fields_ = [
+        ("time_step", C.c_double),
+        ("integrator", C.c_uint32),
+        ("implicit_iterations", C.c_uint32),
+        ("first_simulation_step", C.c_uint32),
+        ("ase_steps", C.c_uint32),
+        ("pump_steps", C.c_uint32),
+        ("pre_pump", C.c_uint32),
+    ]
+
+
+class TimeState(C.Structure):
+    _fields_ = [
+        ("step", C.c_uint32),
+        ("ase_evaluations", C.c_uint32),
+        ("time", C.c_double),
+        ("beta_volume", _f64p),
+        ("phi_ase", _f64p),
+        ("dndt_ase", _f64p),
+        ("dndt_pump", _f64p),
+        ("phi_samples", _f32p),
+        ("mse", _f64p),
+        ("total_rays", _u32p),
+        ("standard_error", _f64p),
+    ]
+
+
+# integrator codes of include/hase_ase.h, keyed by the reference's names
+# (include/core/simulationRunControl.hpp:65-74)
+INTEGRATORS = {
+    "explicit-euler": 0,
+    "heun": 1,
+    "midpoint": 2,
+    "runge-kutta-4": 3,
+    "frozen-phi-ase-runge-kutta-4": 4,
+    "implicit-euler": 5,
+    "exponential-euler": 6,
+}
+
 TRACE_QUEUE = 0
 TRACE_ROUNDS = 1
 MLP_INPUTS = 10
@@ -246,6 +293,14 @@ def load():
     L.hase_batch_stats.argtypes = [ctx_p, C.POINTER(Stats)]
     L.hase_batch_destroy.argtypes = [ctx_p]
     L.hase_batch_destroy.restype = None
+    L.hase_simulation_create.argtypes = [ctx_p, C.POINTER(RunControl), _f64p, _f64p, _f64p, C.POINTER(ctx_p)]
+    L.hase_simulation_set_pump.argtypes = [ctx_p, _f64p, _f64p]
+    L.hase_simulation_set_beta.argtypes = [ctx_p, _f64p]
+    L.hase_simulation_step.argtypes = [ctx_p, C.c_uint32]
+    L.hase_simulation_state.argtypes = [ctx_p, C.POINTER(TimeState)]
+    L.hase_simulation_times.argtypes = [ctx_p, _f64p, _f64p]
+    L.hase_simulation_destroy.argtypes = [ctx_p]
+    L.hase_simulation_destroy.restype = None
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
@@ -255,7 +310,9 @@ def load():
                  "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
                  "hase_context_proposal_logits", "hase_context_proposal_stats", "hase_context_set_tuning",
                  "hase_context_get_tuning", "hase_context_autotune", "hase_batch_create",
-                 "hase_batch_evaluate", "hase_batch_stats"):
+                 "hase_batch_evaluate", "hase_batch_stats", "hase_simulation_create", "hase_simulation_set_pump",
+                 "hase_simulation_set_beta", "hase_simulation_step", "hase_simulation_state",
+                 "hase_simulation_times"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -510,6 +567,87 @@ class Batch:
         st = Stats()
         check(load().hase_batch_stats(self._h, C.byref(st)))
         return st.asdict()
+
+
+class NativeSimulation:
+    """Time-stepped beta loop on a context (hase_simulation_*): beta per
+    prism stays on the device between steps."""
+
+    def __init__(self, context: Context, *, time_step, integrator="explicit-euler", implicit_iterations=8,
+                 first_simulation_step=0, ase_steps=0, pump_steps=0, pre_pump=False, beta_volume=None,
+                 pump_rate=None, pump_saturation=None):
+        L = load()
+        if isinstance(integrator, str):
+            if integrator not in INTEGRATORS:
+                raise ValueError(f"unsupported time integrator '{integrator}'")
+            integrator = INTEGRATORS[integrator]
+        self._ctx = context
+        self.n_samples = context.n_samples
+        self.n_prisms = context.prisms
+        self.run = RunControl(float(time_step), int(integrator), int(implicit_iterations),
+                              int(first_simulation_step), int(ase_steps), int(pump_steps), int(bool(pre_pump)))
+        beta = None if beta_volume is None else self._prism_array(beta_volume, "beta_volume")
+        rate = self._prism_array(np.zeros(self.n_prisms) if pump_rate is None else pump_rate, "pump_rate")
+        sat = None if pump_saturation is None else self._prism_array(pump_saturation, "pump_saturation")
+        handle = C.c_void_p()
+        check(L.hase_simulation_create(context._h, C.byref(self.run), _ptr(beta, C.c_double),
+                                       _ptr(rate, C.c_double), _ptr(sat, C.c_double), C.byref(handle)))
+        self._h = handle
+
+    def _prism_array(self, values, name):
+        arr = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1, order="F"))
+        if arr.size != self.n_prisms:
+            raise ValueError(f"{name} needs {self.n_prisms} values (one per prism), got {arr.size}")
+        return arr
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().hase_simulation_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_pump(self, pump_rate, pump_saturation=None):
+        rate = self._prism_array(pump_rate, "pump_rate")
+        sat = None if pump_saturation is None else self._prism_array(pump_saturation, "pump_saturation")
+        check(load().hase_simulation_set_pump(self._h, _ptr(rate, C.c_double), _ptr(sat, C.c_double)))
+
+    def set_beta(self, beta_volume):
+        beta = self._prism_array(beta_volume, "beta_volume")
+        check(load().hase_simulation_set_beta(self._h, _ptr(beta, C.c_double)))
+
+    def step(self, n_steps=1):
+        check(load().hase_simulation_step(self._h, int(n_steps)))
+
+    def state(self):
+        """dict of the state after the last completed step (prism arrays in
+        the caller's order j = t + l * T)."""
+        P, S = self.n_prisms, self.n_samples
+        out = {"beta_volume": np.zeros(P), "phi_ase": np.zeros(P), "dndt_ase": np.zeros(P),
+               "dndt_pump": np.zeros(P), "phi_samples": np.zeros(S, np.float32), "mse": np.zeros(S),
+               "total_rays": np.zeros(S, np.uint32), "standard_error": np.zeros(S)}
+        st = TimeState(0, 0, 0.0, _ptr(out["beta_volume"], C.c_double), _ptr(out["phi_ase"], C.c_double),
+                       _ptr(out["dndt_ase"], C.c_double), _ptr(out["dndt_pump"], C.c_double),
+                       _ptr(out["phi_samples"], C.c_float), _ptr(out["mse"], C.c_double),
+                       _ptr(out["total_rays"], C.c_uint32), _ptr(out["standard_error"], C.c_double))
+        check(load().hase_simulation_state(self._h, C.byref(st)))
+        out.update(step=int(st.step), ase_evaluations=int(st.ase_evaluations), time=float(st.time))
+        return out
+
+    def times(self):
+        a, b = C.c_double(), C.c_double()
+        check(load().hase_simulation_times(self._h, C.byref(a), C.byref(b)))
+        return {"ase_ms": a.value, "integrator_ms": b.value}
 
 
 def mlp_param_count():

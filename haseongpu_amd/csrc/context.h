@@ -28,6 +28,7 @@ struct hase_context {
     std::vector<uint32_t> user2dev;    // caller triangle -> device triangle
     std::vector<double> beta_host;     // [P]
     std::vector<double> beta_cells;    // [S]
+    std::vector<uint32_t> tri_vertex;  // [3T] vertex k of device triangle t at 3t + k
     double Bsum = 0.0;
     hase_experiment_t exp{};
     std::vector<double> sigma_a, sigma_e;
@@ -74,6 +75,12 @@ struct hase_context {
     hase::GraphCache q_graph;
     uint64_t q_generation = 1;                                                  // bumped when launch inputs may change
 
+    // device copy of the flux of every sample, kept when phi_keep is set
+    // (the time-stepped loop reads it without a host round trip)
+    bool phi_keep = false;
+    hase::DevBuf<float> phi_all;                                                // [S]
+    hase::DevBuf<double> tri_area_dev;                                          // [T] device order
+
     ~hase_context()
     {
         q_graph.reset();
@@ -106,6 +113,13 @@ uint32_t slices(const hase_context* ctx);
 // events recorded on the context stream (destroyed at the next evaluation)
 hipEvent_t record(hase_context* ctx);
 float elapsed(hipEvent_t a, hipEvent_t b);
+
+// evaluation of the context's sample range into host results (hase_ase.hip)
+void evaluate_context(hase_context* ctx, hase_result_t* res);
+// Replaces the walk's copies of beta (FP32 beta, beta * V, column-major beta,
+// the sum of beta * V) by those of `beta` [P], FP64 in device prism order,
+// already on the device; block_sums holds ceil(P / 256) doubles.
+void refresh_beta_device(hase_context* ctx, const double* beta, double* block_sums);
 
 // persistent work queue (queue.hip)
 void check_tuning(const hase_tuning_t& t);

@@ -652,6 +652,43 @@ __global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* star
     status[i] = w.status;
 }
 
+// Flux of the resident slots into the flux of every sample.
+__global__ void hase_phi_scatter_kernel(const float* __restrict__ phi, const uint32_t* __restrict__ sample_of,
+                                        uint32_t base, uint32_t slots, float* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < slots) out[sample_of ? sample_of[i] : base + i] = phi[i];
+}
+
+// beta [P] (FP64, device prism order) -> FP32 beta, beta * V, column-major
+// beta, and per-block sums of beta * V (fixed order: deterministic).
+__global__ void __launch_bounds__(256) hase_refresh_beta_kernel(const double* __restrict__ beta,
+                                                                const double* __restrict__ area, double h,
+                                                                uint32_t T, uint32_t layers, float* __restrict__ b32,
+                                                                float* __restrict__ bv, float* __restrict__ bcol,
+                                                                double* __restrict__ block_sums)
+{
+    __shared__ double red[256];
+    const uint32_t P = T * layers;
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    double w = 0.0;
+    if (j < P) {
+        const double v = beta[j];
+        const uint32_t t = j % T, l = j / T;
+        w = v > 0.0 ? v * area[t] * h : 0.0;
+        b32[j] = (float)v;
+        bv[j] = (float)w;
+        bcol[(size_t)t * layers + l] = (float)v;
+    }
+    red[threadIdx.x] = w;
+    __syncthreads();
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = red[0];
+}
+
 // ------------------------------------------------------------------- host
 
 }  // namespace
@@ -825,12 +862,14 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     ctx->tri_area.resize(T);
     ctx->clad_flag.resize(T);
     ctx->centroid_host.resize(T);
+    ctx->tri_vertex.resize(3u * (size_t)T);
     for (uint32_t td = 0; td < T; ++td) {
         const uint32_t t = ctx->dev2user[td];
         uint32_t v[3];
         uint32_t nw[3];
         for (int k = 0; k < 3; ++k) {
             v[k] = mesh->triangle_point_indices[t + (size_t)k * T];
+            ctx->tri_vertex[3u * td + k] = v[k];
             int32_t nb = mesh->triangle_neighbors[t + (size_t)k * T];
             int32_t fe = mesh->forbidden_edge[t + (size_t)k * T];
             if (nb >= 0) {
@@ -1180,6 +1219,11 @@ void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_
             break;
         }
     }
+    if (ctx->phi_keep) {
+        hipLaunchKernelGGL(hase_phi_scatter_kernel, dim3((slots + 255) / 256), dim3(256), 0, ctx->stream,
+                           (const float*)ctx->phi.p, sample_of, b0, slots, ctx->phi_all.p);
+        HIP_CHECK(hipGetLastError());
+    }
     std::vector<float> h_phi(slots);
     std::vector<double> h_mse(slots), h_se(slots);
     std::vector<uint32_t> h_N(slots), h_drop(slots);
@@ -1222,6 +1266,10 @@ void evaluate(hase_context* ctx, hase_result_t* res)
     ctx->stats = hase_stats_t{};
     ctx->pstats = hase_proposal_stats_t{};
     ctx->pstats.clusters = ctx->C;
+    if (ctx->phi_keep) {
+        ctx->phi_all.alloc(S);
+        HIP_CHECK(hipMemsetAsync(ctx->phi_all.p, 0, (size_t)S * sizeof(float), ctx->stream));
+    }
     if (!(ctx->Bsum > 0.0)) return;   // no emitting volume: the estimate is zero
     const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
 
@@ -1268,6 +1316,40 @@ void evaluate(hase_context* ctx, hase_result_t* res)
 
 // ------------------------------------------------------------------ batches
 }  // namespace
+
+namespace hase {
+
+void evaluate_context(hase_context* ctx, hase_result_t* res) { evaluate(ctx, res); }
+
+void refresh_beta_device(hase_context* ctx, const double* beta, double* block_sums)
+{
+    HIP_CHECK(hipSetDevice(ctx->device));
+    const uint32_t T = ctx->T, layers = ctx->L - 1, P = ctx->P;
+    static_assert(sizeof(double) == 8, "FP64");
+    if (ctx->tri_area_dev.n < T) ctx->tri_area_dev.upload(ctx->tri_area.data(), T, ctx->stream);
+    const uint32_t blocks = (P + 255u) / 256u;
+    hipLaunchKernelGGL(hase_refresh_beta_kernel, dim3(blocks), dim3(256), 0, ctx->stream, beta,
+                       (const double*)ctx->tri_area_dev.p, ctx->h, T, layers, ctx->beta.p, ctx->bv.p,
+                       ctx->beta_col.p, block_sums);
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> sums(blocks);
+    HIP_CHECK(hipMemcpyAsync(sums.data(), block_sums, blocks * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (ctx->proposal_kind == HASE_PROPOSAL_LEARNED) {
+        // the cluster representatives are built on the host from beta
+        std::vector<double> b(P);
+        HIP_CHECK(hipMemcpyAsync(b.data(), beta, (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (uint32_t j = 0; j < P; ++j) ctx->beta_host[ctx->dev2user[j % T] + (size_t)(j / T) * T] = b[j];
+        build_clusters(ctx);
+    }
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double B = 0.0;
+    for (double v : sums) B += v;
+    ctx->Bsum = B;
+    ctx->q_generation++;
+}
+
+}  // namespace hase
 
 struct hase_batch {
     std::vector<hase_context*> ctx;

@@ -21,7 +21,8 @@ from . import _native
 from .geometry import GainMedium
 from .laser import CrossSectionData
 
-__all__ = ["PhiASE", "Result", "native_problem"]
+__all__ = ["PhiASE", "Result", "native_problem", "Simulation", "TimeSteppedSimulation", "TimeStepState",
+           "SIMULATION_OUTPUT_FIELDS", "SIMULATION_CONTROL_FIELDS"]
 
 
 @dataclass
@@ -73,6 +74,7 @@ class PhiASE:
     raysPerThread: int = 0
     sampleBatch: int = 0
     proposal: object | None = None   # a proposal.LearnedProposal; None = walked importance map
+    ase_steps: int | None = None     # time-stepped runs: ASE in steps < ase_steps (None counts as 0)
 
     _result: Result | None = field(default=None, init=False, repr=False)
 
@@ -85,6 +87,11 @@ class PhiASE:
             self.crossSections = self.spectralProperties
         if self.spectralProperties is None:
             self.spectralProperties = self.crossSections
+        if self.ase_steps is not None:
+            if isinstance(self.ase_steps, bool) or not isinstance(self.ase_steps, (int, np.integer)):
+                raise TypeError("PhiASE.ase_steps must be an integer or None")
+            if self.ase_steps < 0:
+                raise ValueError("PhiASE.ase_steps must be non-negative")
 
     @staticmethod
     def _loadConfig(filename):
@@ -181,3 +188,377 @@ def native_problem(workload, device=0, **overrides):
     mesh, exp, comp = phi.nativeArguments(workload.medium, workload.spectra)
     log_sigma = float(np.log(exp.struct.max_sigma_e / exp.struct.max_sigma_a))
     return mesh, exp, comp, log_sigma
+
+
+# ---------------------------------------------------------------------------
+# Time-stepped simulation (pyInclude/simulation.py:494-1004)
+
+SIMULATION_OUTPUT_FIELDS = ("beta_volume", "phi_ase", "standard_error", "relative_standard_error", "total_rays",
+                            "dndt_ase", "dndt_pump")
+SIMULATION_CONTROL_FIELDS = ("beta_volume",)
+
+
+@dataclass
+class TimeStepState:
+    """Snapshot after a completed step (pyInclude/simulation.py:494-542).
+
+    Prism arrays have the shape of ``GainMedium`` betaVolume, (T, L-1);
+    ``standardError``, ``relativeStandardError`` and ``totalRays`` belong to
+    the sample points of the last ASE evaluation, shape (Np, L); fields that
+    were not selected are None."""
+
+    step: int
+    time: float
+    betaVolume: np.ndarray | None
+    phiAse: np.ndarray | None
+    dndtAse: np.ndarray | None
+    dndtPump: np.ndarray | None
+    aseResult: object | None
+    standardError: np.ndarray | None = None
+    relativeStandardError: np.ndarray | None = None
+    totalRays: np.ndarray | None = None
+    topology: object | None = None
+
+    @property
+    def beta_volume(self):
+        return self.betaVolume
+
+    @property
+    def phi_ase(self):
+        return self.phiAse
+
+    @property
+    def dndt_ase(self):
+        return self.dndtAse
+
+    @property
+    def dndt_pump(self):
+        return self.dndtPump
+
+    @property
+    def ase_result(self):
+        return self.aseResult
+
+
+class Simulation:
+    """Time-stepped beta simulation on the library's device loop.
+
+    Mirrors the reference's ``Simulation`` (pyInclude/simulation.py:545-1002):
+    the same constructor keywords, ``add_pump``, ``on_init`` / ``on_step`` /
+    ``before_step`` callbacks, ``step`` / ``run_until``, output steps and
+    fields, synchronized-debug control of ``beta_volume``.  The loop runs in
+    the HIP library (hase_simulation_*) with beta resident on the device;
+    Python is contacted only at output steps and at the steps where the set of
+    active pumps changes.  Pumps are ``time_integration.LocalPump`` objects
+    (linear in beta per prism).  The device loop persists across ``step``
+    calls; ``close()`` releases it.
+    """
+
+    def __init__(self, *, gain_medium, phi_ase, time_integrator, time_step_size, cross_sections=None,
+                 simulation_steps=None, max_time=None, pre_pump=False, report_timings=False,
+                 execution_mode="autonomous", output_steps=None, output_fields=None, control_fields=()):
+        self.gainMedium = gain_medium
+        self.phiASE = phi_ase
+        self.timeIntegrationSolver = time_integrator
+        self.timeStep = float(time_step_size)
+        self.crossSections = cross_sections
+        self.endTime = max_time
+        self.simulationSteps = None if simulation_steps is None else int(simulation_steps)
+        self.prePump = bool(pre_pump)
+        self.reportTimings = bool(report_timings)
+        self.executionMode = str(execution_mode)
+        self.outputSteps = None if output_steps is None else tuple(int(s) for s in output_steps)
+        self.outputFields = tuple(SIMULATION_OUTPUT_FIELDS if output_fields is None else
+                                  (str(f) for f in output_fields))
+        self.controlFields = tuple(str(f) for f in control_fields)
+        self._pumps = []
+        self._time = 0.0
+        self._step = 0
+        self._initialized = False
+        self._init_callbacks = []
+        self._before_step_callbacks = []
+        self._step_callbacks = []
+        self._lastState = None
+        self._ctx = None
+        self._loop = None
+        self._pump_key = None
+        self._validate()
+
+    # -- configuration (validation of pyInclude/simulation.py:630-673) --------
+    def _validate(self):
+        solver = self.timeIntegrationSolver
+        if solver is None:
+            raise ValueError("Simulation requires a time_integrator")
+        name = solver if isinstance(solver, str) else getattr(solver, "name", None)
+        if name not in _native.INTEGRATORS:
+            raise ValueError(f"unsupported time integrator '{name}'")
+        if not self.timeStep > 0.0:
+            raise ValueError("time_step_size must be positive")
+        if self.simulationSteps is not None and self.simulationSteps <= 0:
+            raise ValueError("simulation_steps must be positive")
+        if self.simulationSteps is not None and self.endTime is not None:
+            raise ValueError("configure either simulation_steps or max_time, not both")
+        if self.executionMode not in ("autonomous", "synchronized-debug"):
+            raise ValueError("execution_mode must be 'autonomous' or 'synchronized-debug'")
+        if self.outputSteps is not None:
+            if not self.outputSteps:
+                raise ValueError("output_steps must be omitted or contain at least one completed-step index")
+            if any(s <= 0 for s in self.outputSteps):
+                raise ValueError("output_steps must contain positive completed-step indices")
+            if tuple(sorted(set(self.outputSteps))) != self.outputSteps:
+                raise ValueError("output_steps must be strictly increasing and unique")
+        if not self.outputFields:
+            raise ValueError("output_fields must contain at least one field")
+        unknown = sorted(set(self.outputFields) - set(SIMULATION_OUTPUT_FIELDS))
+        if unknown:
+            raise ValueError(f"unsupported output_fields: {unknown}")
+        if len(set(self.outputFields)) != len(self.outputFields):
+            raise ValueError("output_fields must be unique")
+        if self.executionMode == "autonomous" and self.controlFields:
+            raise ValueError("control_fields require execution_mode='synchronized-debug'")
+        if self.executionMode == "synchronized-debug" and self.outputSteps is not None:
+            raise ValueError("synchronized-debug emits every completed step; output_steps must be omitted")
+        unknown = sorted(set(self.controlFields) - set(SIMULATION_CONTROL_FIELDS))
+        if unknown:
+            raise ValueError(f"unsupported control_fields: {unknown}")
+        if len(set(self.controlFields)) != len(self.controlFields):
+            raise ValueError("control_fields must be unique")
+        if self.crossSections is None:
+            self.crossSections = self.phiASE.crossSections
+        if self.phiASE.crossSections is None:
+            self.phiASE.crossSections = self.crossSections
+
+    @property
+    def integratorName(self):
+        solver = self.timeIntegrationSolver
+        return solver if isinstance(solver, str) else solver.name
+
+    def add_pump(self, pump, injection_method=None):
+        """Register a pump (a ``LocalPump``); returns ``self``."""
+        from .time_integration import LocalPump
+
+        if self._initialized:
+            raise RuntimeError("pumps must be added before the simulation is initialized")
+        if not isinstance(pump, LocalPump):
+            raise TypeError("pump must be a LocalPump")
+        self._pumps.append(pump)
+        return self
+
+    def on_step(self, callback, *args, **kwargs):
+        self._step_callbacks.append((callback, args, kwargs))
+        return self
+
+    def on_init(self, callback, *args, **kwargs):
+        self._init_callbacks.append((callback, args, kwargs))
+        return self
+
+    def before_step(self, callback, *args, **kwargs):
+        self._before_step_callbacks.append((callback, args, kwargs))
+        return self
+
+    # -- running ---------------------------------------------------------------
+    def _setup(self):
+        if self._loop is not None:
+            return
+        if self.crossSections is None:
+            raise ValueError("Simulation requires cross sections (cross_sections= or PhiASE.crossSections)")
+        mesh, exp, comp = self.phiASE.nativeArguments(self.gainMedium, self.crossSections)
+        if comp.min_sample_range != 0 or comp.max_sample_range + 1 != self.gainMedium.topology.numberOfSamples:
+            raise ValueError("a time-stepped simulation evaluates every sample point (no sample range)")
+        self._ctx = _native.Context(mesh, exp, comp)
+        if self.phiASE.proposal is not None:
+            self._ctx.set_proposal(self.phiASE.proposal)
+        iters = getattr(self.timeIntegrationSolver, "iterations", 8)
+        ase_steps = 0 if self.phiASE.ase_steps is None else int(self.phiASE.ase_steps)
+        P = self._ctx.prisms
+        self._loop = _native.NativeSimulation(
+            self._ctx, time_step=self.timeStep, integrator=self.integratorName, implicit_iterations=iters,
+            first_simulation_step=self._step, ase_steps=ase_steps, pump_steps=0xFFFFFFFF,
+            pre_pump=self.prePump, beta_volume=self.gainMedium.value("betaVolume"), pump_rate=np.zeros(P))
+
+    def _apply_pumps(self, step):
+        """Sum of the pumps active in simulation step ``step``."""
+        active = tuple(i for i, p in enumerate(self._pumps) if p.active(step))
+        if active == self._pump_key:
+            return
+        P = self._ctx.prisms
+        rate, sat = np.zeros(P), None
+        for i in active:
+            r, s = self._pumps[i].arrays(P)
+            rate = rate + r
+            if s is not None:
+                sat = s.copy() if sat is None else sat + s
+        self._loop.set_pump(rate, sat)
+        self._pump_key = active
+
+    def _segment_end(self, step, last):
+        """Last step (exclusive) before the active pump set may change."""
+        end = last
+        for p in self._pumps:
+            n = 0 if p.pump_steps is None else int(p.pump_steps)
+            if step < n < end:
+                end = n
+        return end
+
+    def _emit(self, completed_in_run, steps_in_run):
+        st = self._loop.state()
+        T, L = self.gainMedium.topology.numberOfTriangles, int(self.gainMedium.topology.levels)
+        Np = self.gainMedium.topology.numberOfPoints
+        want = set(self.outputFields)
+        prism = lambda a: a.reshape(T, L - 1, order="F").copy()      # noqa: E731
+        sample = lambda a: a.reshape(Np, L, order="F").copy()        # noqa: E731
+        rse = np.sqrt(np.where(st["mse"] >= 0, st["mse"], np.nan))
+        ase_result = Result(phiAse=st["phi_samples"], mse=st["mse"], totalRays=st["total_rays"],
+                            standardError=st["standard_error"], relativeStandardError=rse)
+        state = TimeStepState(
+            step=self._step, time=self._time,
+            betaVolume=prism(st["beta_volume"]) if "beta_volume" in want else None,
+            phiAse=prism(st["phi_ase"]) if "phi_ase" in want else None,
+            dndtAse=prism(st["dndt_ase"]) if "dndt_ase" in want else None,
+            dndtPump=prism(st["dndt_pump"]) if "dndt_pump" in want else None,
+            aseResult=ase_result,
+            standardError=sample(st["standard_error"]) if "standard_error" in want else None,
+            relativeStandardError=sample(rse) if "relative_standard_error" in want else None,
+            totalRays=sample(st["total_rays"]) if "total_rays" in want else None,
+            topology=self.gainMedium.topology)
+        if state.betaVolume is not None:
+            self.gainMedium.set("betaVolume", state.betaVolume)
+        self._lastState = state
+        for cb, args, kwargs in self._step_callbacks:
+            cb(state, *args, **kwargs)
+        if self.executionMode == "synchronized-debug" and completed_in_run < steps_in_run:
+            for cb, args, kwargs in self._before_step_callbacks:
+                cb(self, *args, **kwargs)
+            if "beta_volume" in self.controlFields:
+                self._loop.set_beta(self.gainMedium.value("betaVolume"))
+
+    def _should_output(self, step):
+        if self.executionMode == "synchronized-debug" or self.outputSteps is None:
+            return True
+        return step in self.outputSteps
+
+    def runSteps(self, steps):
+        """Run exactly ``steps`` time steps on the device and return ``self``."""
+        steps = int(steps)
+        if steps <= 0:
+            raise ValueError("steps must be positive")
+        if self._before_step_callbacks and self.executionMode != "synchronized-debug":
+            raise ValueError("before_step callbacks require execution_mode='synchronized-debug'")
+        if not self._initialized:
+            self._initialized = True
+            for cb, args, kwargs in self._init_callbacks:
+                cb(self, *args, **kwargs)
+        self._setup()
+        done = 0
+        while done < steps:
+            self._apply_pumps(self._step)
+            # run up to the next output step or pump change in one library call
+            end = self._step + (steps - done)
+            end = self._segment_end(self._step, end)
+            nxt = self._step + 1
+            while nxt < end and not self._should_output(nxt):
+                nxt += 1
+            n = min(nxt, end) - self._step
+            self._loop.step(n)
+            done += n
+            self._step += n
+            self._time += n * self.timeStep
+            if self._should_output(self._step):
+                self._emit(done, steps)
+        return self
+
+    def _derived_simulation_steps(self):
+        activity = [0 if self.phiASE.ase_steps is None else int(self.phiASE.ase_steps)]
+        activity += [0 if p.pump_steps is None else int(p.pump_steps) for p in self._pumps]
+        derived = max(activity, default=0)
+        if derived <= 0:
+            raise ValueError("simulation_steps is required when all pump_steps and ase_steps are disabled")
+        return derived
+
+    def step(self, nsteps=None):
+        """Advance ``nsteps`` time steps (PICMI call pattern)."""
+        if nsteps is None:
+            if self.simulationSteps is not None:
+                nsteps = self.simulationSteps
+            elif self.endTime is not None:
+                return self.run_until()
+            else:
+                nsteps = self._derived_simulation_steps()
+        if int(nsteps) <= 0:
+            raise ValueError("nsteps must be positive")
+        return self.runSteps(int(nsteps))
+
+    def run_until(self, max_time=None):
+        target = self.endTime if max_time is None else max_time
+        if target is None:
+            raise ValueError("run_until requires max_time or a configured max_time")
+        steps = 0
+        while self._time + steps * self.timeStep < float(target) - 0.5 * self.timeStep:
+            steps += 1
+        if steps:
+            self.step(steps)
+        return self
+
+    def times(self):
+        """Milliseconds spent in ASE evaluations and in the integrator kernels."""
+        return {"ase_ms": 0.0, "integrator_ms": 0.0} if self._loop is None else self._loop.times()
+
+    def close(self):
+        if self._loop is not None:
+            self._loop.close()
+            self._loop = None
+        if self._ctx is not None:
+            self._ctx.close()
+            self._ctx = None
+
+    def get_last_state(self):
+        return self.getLastState()
+
+    def getLastState(self):
+        if self._lastState is None:
+            raise RuntimeError("simulation has not completed a time step yet")
+        return self._lastState
+
+    getResults = getLastState
+
+    @property
+    def lastState(self):
+        return self.getLastState()
+
+    @property
+    def current_step(self):
+        return self._step
+
+    @property
+    def current_time(self):
+        return self._time
+
+    time = current_time
+    stepIndex = current_step
+
+    @property
+    def gain_medium(self):
+        return self.gainMedium
+
+    @property
+    def phi_ase(self):
+        return self.phiASE
+
+    @property
+    def time_integrator(self):
+        return self.timeIntegrationSolver
+
+    @property
+    def time_step_size(self):
+        return self.timeStep
+
+    @property
+    def simulation_steps(self):
+        return self.simulationSteps
+
+    @property
+    def pumps(self):
+        return tuple(self._pumps)
+
+
+TimeSteppedSimulation = Simulation

@@ -151,3 +151,18 @@ def batch_problems(n=64, seed=0, triangles=(200, 8000), levels=(5, 40), waveleng
                         rngSeed=seed * 1000 + i)
         out.append(Workload(f"batch_problem_{i}", topo, medium, gaussian_spectra(W), settings))
     return out
+
+
+def local_pump(topology, *, peak_rate=1500.0, width=1.2, absorption=1.5, ratio=1.21, pump_steps=None,
+               center=(0.0, 0.0)):
+    """Deterministic smooth pump for the time-stepped loop: a Gaussian spot of
+    1/e^2-type width ``width`` (cm) entering at z = 0 and absorbed along z
+    with coefficient ``absorption`` (cm^-1); rate = peak_rate * profile and
+    saturation = ratio * rate (ratio = 1 + sigma_e,p / sigma_a,p; 1.21 for
+    Yb:YAG at 940 nm)."""
+    from .time_integration import LocalPump
+
+    cent = topology.prismCentroids()
+    r2 = (cent[:, 0] - center[0]) ** 2 + (cent[:, 1] - center[1]) ** 2
+    rate = peak_rate * np.exp(-r2 / (2.0 * width * width)) * np.exp(-absorption * cent[:, 2])
+    return LocalPump(rate=rate, saturation=ratio * rate, pump_steps=pump_steps)

@@ -343,6 +343,92 @@ int hase_batch_stats(const hase_batch_t* batch, hase_stats_t* stats);
 
 void hase_batch_destroy(hase_batch_t* batch);
 
+/* ------------------------------------------------------------------------
+ * Time-stepped beta loop: the reference's CompiledSimulationRunner
+ * (include/core/timeSteppedSimulation.hpp:132-615) on the prism mesh.  The
+ * state is beta per prism, FP64, resident on the device next to the
+ * context; each stage of an integrator refreshes the context's walk copies
+ * of beta and its importance map from that state (no host round trip of
+ * beta), evaluates the ASE flux at the sample points, averages it over the
+ * six vertices of each prism (the legacy wedge mean of
+ * tests/data/laserPumpCladding/fixed_legacy_no_reflection_pump3_wedge_reference/generate_reference.py:138-141)
+ * and composes the derivative of include/kernels/derivativeComposition.hpp:27-60:
+ *   d beta/dt = pump - (beta (sigma_e + sigma_a) - sigma_a) phi - beta / tau,
+ * with sigma_e = max_sigma_e, sigma_a = sigma_a at the peak of sigma_e
+ * (timeSteppedSimulation.hpp:45-54) and tau = crystal_t_fluo.  Cladding
+ * prisms keep their beta.  The pump is local and linear in beta,
+ *   pump_j = pump_rate_j - pump_saturation_j * beta_j   [1/s],
+ * the form of a pump of fixed intensity on a two-level absorber; it stands
+ * in for the reference's ray-traced pump (include/kernels/generalPump.hpp),
+ * which is not part of this library.  Every ASE evaluation uses the
+ * context's settings and seed, as the reference's runner re-evaluates with
+ * one ExperimentParameters.
+ * ---------------------------------------------------------------------- */
+#define HASE_INTEGRATOR_EXPLICIT_EULER 0               /* "explicit-euler" */
+#define HASE_INTEGRATOR_HEUN 1                         /* "heun" */
+#define HASE_INTEGRATOR_MIDPOINT 2                     /* "midpoint" */
+#define HASE_INTEGRATOR_RUNGE_KUTTA_4 3                /* "runge-kutta-4" */
+#define HASE_INTEGRATOR_FROZEN_PHI_ASE_RUNGE_KUTTA_4 4 /* "frozen-phi-ase-runge-kutta-4": one ASE flux per step */
+#define HASE_INTEGRATOR_IMPLICIT_EULER 5               /* "implicit-euler": fixed-point iterations */
+#define HASE_INTEGRATOR_EXPONENTIAL_EULER 6            /* "exponential-euler": exact fluorescence decay */
+
+/* Run control (include/core/simulationRunControl.hpp:76-81, 115-127). */
+typedef struct hase_run_control {
+    double time_step;               /* seconds, > 0 */
+    uint32_t integrator;            /* HASE_INTEGRATOR_* */
+    uint32_t implicit_iterations;   /* implicit Euler iterations (0 counts as 1) */
+    uint32_t first_simulation_step; /* index of the first step (activity windows count from it) */
+    uint32_t ase_steps;             /* ASE is evaluated in steps < ase_steps */
+    uint32_t pump_steps;            /* the pump acts in steps < pump_steps */
+    uint32_t pre_pump;              /* 1: no ASE in step 0 */
+} hase_run_control_t;
+
+/* State after the last completed step (include/core/simulationSnapshot.hpp:18-26);
+ * every pointer may be NULL, each non-NULL one receives its array.  Prism
+ * arrays [P] are in the caller's prism order (j = t + l * T); sample arrays
+ * [S] come from the last ASE evaluation (zeros when ASE was off). */
+typedef struct hase_time_state {
+    uint32_t step;                  /* steps completed since creation */
+    uint32_t ase_evaluations;       /* ASE evaluations since creation */
+    double time;                    /* step * time_step, seconds */
+    double* beta_volume;            /* [P] */
+    double* phi_ase;                /* [P] six-vertex mean of the sample flux */
+    double* dndt_ase;               /* [P] (beta (sigma_e + sigma_a) - sigma_a) phi of the last stage */
+    double* dndt_pump;              /* [P] pump term of the last stage */
+    float* phi_samples;             /* [S] */
+    double* mse;                    /* [S] */
+    uint32_t* total_rays;           /* [S] */
+    double* standard_error;         /* [S] */
+} hase_time_state_t;
+
+typedef struct hase_simulation hase_simulation_t;
+
+/* Starts a loop on `context` (which must outlive it, cover all S samples and
+ * is not used by the caller while the loop exists) from beta_volume [P]
+ * (NULL: the context's current beta).  pump_rate [P] is required,
+ * pump_saturation [P] may be NULL (0).  Mirrors the construction and
+ * validation of timeSteppedSimulation.hpp:56-129, 140-175. */
+int hase_simulation_create(hase_context_t* context, const hase_run_control_t* run, const double* beta_volume,
+                           const double* pump_rate, const double* pump_saturation, hase_simulation_t** out_sim);
+
+/* Replaces the pump terms from the next stage on. */
+int hase_simulation_set_pump(hase_simulation_t* sim, const double* pump_rate, const double* pump_saturation);
+
+/* Overwrites the beta state (the beta_volume control field of the
+ * synchronized-debug mode, timeSteppedSimulation.hpp:228-236). */
+int hase_simulation_set_beta(hase_simulation_t* sim, const double* beta_volume);
+
+/* Advances n_steps time steps (timeSteppedSimulation.hpp:177-241). */
+int hase_simulation_step(hase_simulation_t* sim, uint32_t n_steps);
+
+/* Copies the state after the last completed step. */
+int hase_simulation_state(const hase_simulation_t* sim, hase_time_state_t* state);
+
+/* Device times of the loop so far: ASE evaluations and the integrator kernels. */
+int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* integrator_ms);
+
+void hase_simulation_destroy(hase_simulation_t* sim);
+
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);
 

@@ -16,6 +16,10 @@ expected outputs), converted to the prism layout this library consumes:
 * trueCladding.npz - legacy importance-sampled phiASE / mse / totalRays at
   2000 rays per sample for cladding absorption 0, 5.5 and 55 cm^-1
   (tests/data/laserPumpCladding/true_cladding_no_reflection_phiase_reference/).
+* legacyPump3Wedge.npz - the six-step pumped legacy time series without
+  reflections: phiASE at every wedge point after each step and the wedge
+  integrals the archive's metadata records
+  (tests/data/laserPumpCladding/fixed_legacy_no_reflection_pump3_wedge_reference/).
 """
 from __future__ import annotations
 
@@ -198,6 +202,34 @@ def main(argv=None):
         rngSeed=np.int64(m["parameters"]["rngSeed"]),
     )
     print(f"wrote trueCladding.npz: T={T} L={levels}")
+
+    wedge_dir = ref / "tests/data/laserPumpCladding/fixed_legacy_no_reflection_pump3_wedge_reference"
+    d = np.load(wedge_dir / "phiase_reference.npz")
+    m = json.loads(str(d["metadata"]))
+    pts = d["points"]
+    cells = d["cells"].astype(np.int64)
+    levels = int(np.unique(pts[:, 2]).size)
+    npts = pts.shape[0] // levels
+    T = cells.shape[0] // (levels - 1)
+    base = cells[:T, :3]
+    for lv in range(levels - 1):
+        assert np.all(cells[lv * T:(lv + 1) * T, :3] == base + lv * npts)
+        assert np.all(cells[lv * T:(lv + 1) * T, 3:] == base + (lv + 1) * npts)
+    thickness = float(m["parameters"]["thickness"])
+    assert np.allclose(pts[:, 2].reshape(levels, npts), (np.arange(levels) * thickness)[:, None])
+    np.savez_compressed(
+        HERE / "legacyPump3Wedge.npz",
+        points=pts[:npts, :2].copy(),
+        triangles=base.astype(np.uint32),
+        levels=np.int64(levels),
+        thickness=np.float64(thickness),
+        phiASE=d["phiASE"],
+        integrals=np.asarray(m["integrals"], dtype=np.float64),
+        meanPhi=np.asarray(m["meanPhi"], dtype=np.float64),
+        totalWedgeVolume=np.float64(m["totalWedgeVolume"]),
+        timeStepSeconds=np.float64(m["parameters"]["timeStepSeconds"]),
+    )
+    print(f"wrote legacyPump3Wedge.npz: T={T} L={levels}")
 
 
 if __name__ == "__main__":

@@ -37,7 +37,8 @@ def test_struct_layouts_match_header(tmp_path):
     structs = {"hase_mesh_t": _native.Mesh, "hase_experiment_t": _native.Experiment,
                "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats,
                "hase_train_t": _native.Train, "hase_proposal_t": _native.Proposal,
-               "hase_proposal_stats_t": _native.ProposalStats, "hase_tuning_t": _native.Tuning}
+               "hase_proposal_stats_t": _native.ProposalStats, "hase_tuning_t": _native.Tuning,
+               "hase_run_control_t": _native.RunControl, "hase_time_state_t": _native.TimeState}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():
@@ -146,3 +147,51 @@ def test_batch_entry_points_reject_bad_arguments_on_the_host():
     assert lib.hase_batch_evaluate(None, None) == -1
     assert lib.hase_batch_stats(None, None) == -1
     lib.hase_batch_destroy(None)
+
+
+def test_time_loop_entry_points_reject_bad_arguments_on_the_host():
+    lib = _native.load()
+    out = C.c_void_p()
+    run = _native.RunControl(1e-5, 0, 8, 0, 1, 1, 0)
+    rate = np.zeros(4)
+    rp = rate.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.hase_simulation_create(None, C.byref(run), None, rp, None, C.byref(out)) == -1
+    assert b"context" in lib.hase_last_error()
+    assert lib.hase_simulation_create(None, C.byref(run), None, rp, None, None) == -1
+    assert lib.hase_simulation_set_pump(None, rp, None) == -1
+    assert lib.hase_simulation_set_beta(None, rp) == -1
+    assert lib.hase_simulation_step(None, 1) == -1
+    assert lib.hase_simulation_state(None, None) == -1
+    assert lib.hase_simulation_times(None, None, None) == -1
+    lib.hase_simulation_destroy(None)
+    text = HEADER.read_text()
+    for name, code in _native.INTEGRATORS.items():
+        macro = "HASE_INTEGRATOR_" + name.upper().replace("-", "_")
+        assert f"#define {macro} {code}" in text, macro
+
+
+def test_time_loop_front_end_validation():
+    from haseongpu_amd import ExplicitEuler, LocalPump, PhiASE, Simulation
+    from haseongpu_amd.workloads import plain_small
+
+    w = plain_small(rays=1000)
+    phi = PhiASE(crossSections=w.spectra, **w.settings)
+    ok = dict(gain_medium=w.medium, phi_ase=phi, time_integrator=ExplicitEuler(), time_step_size=1e-5)
+    Simulation(**ok)
+    for bad, msg in [({"time_integrator": None}, "time_integrator"), ({"time_step_size": 0.0}, "positive"),
+                     ({"time_integrator": "leapfrog"}, "unsupported"),
+                     ({"simulation_steps": 2, "max_time": 1.0}, "either"),
+                     ({"output_steps": (2, 1)}, "increasing"), ({"output_fields": ("beta",)}, "unsupported"),
+                     ({"control_fields": ("beta_volume",)}, "synchronized-debug"),
+                     ({"execution_mode": "synchronized-debug", "output_steps": (1,)}, "output_steps")]:
+        with pytest.raises(ValueError, match=msg):
+            Simulation(**{**ok, **bad})
+    sim = Simulation(**ok)
+    with pytest.raises(TypeError):
+        sim.add_pump(object())
+    with pytest.raises(ValueError, match="simulation_steps is required"):
+        sim.step()
+    with pytest.raises(ValueError, match="one per prism"):
+        LocalPump(rate=np.zeros(3)).arrays(w.topology.numberOfPrisms)
+    with pytest.raises(TypeError):
+        PhiASE(crossSections=w.spectra, ase_steps=1.5)
