@@ -201,6 +201,76 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def time_bench(args):
+    """Time-stepped beta loop on the production disk: a saturable Gaussian
+    pump (peak 1500 /s, width 1.2 cm, absorbed along z) plus ASE, every ASE
+    evaluation adaptive to mse <= 0.005, beta resident on the device.  Times
+    ``--time-steps`` warm steps of the chosen integrator with the walked map
+    and with the learned proposal (trained as in the main benchmark)."""
+    from haseongpu_amd import _native
+    from haseongpu_amd import proposal as prop
+    from haseongpu_amd.workloads import local_pump, production_disk
+
+    w = production_disk(levels=args.levels)
+    topo = w.topology
+    pump = local_pump(topo, peak_rate=1500.0, width=1.2, absorption=1.0)
+    beta0 = np.asarray(w.medium.value("betaVolume"), dtype=np.float64)
+    n = args.time_steps
+    h = 2e-5
+    log(f"time-stepped: {args.integrator}, {n} steps of {h:g} s, {topo.numberOfPrisms} prisms, {w.samples} samples")
+    forms = {}
+    net = None
+    for form in (("plain",) if args.plain_only else ("plain", "learned")):
+        ctx, spectra = build_context(w)
+        best = ctx.autotune(512)
+        if form == "learned":
+            ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+            if args.load_proposal:
+                net = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
+            else:
+                net, _ = prop.train_on_workload(w, fields=args.fields, samples_per_field=48, rows_per_sample=1024,
+                                                seed=1, epochs=12, batch=4096, cluster_triangles=ct,
+                                                cluster_levels=cl)
+            ctx.set_proposal(net)
+        sim = _native.NativeSimulation(ctx, time_step=h, integrator=args.integrator, beta_volume=beta0,
+                                       pump_rate=pump.rate, pump_saturation=pump.saturation,
+                                       ase_steps=n + 1, pump_steps=n + 1)
+        t0 = time.perf_counter()
+        sim.step(1)                                   # warm step (graph capture, first allocations)
+        cold = time.perf_counter() - t0
+        t_before = sim.times()
+        e0 = sim.state()["ase_evaluations"]
+        step_s = []
+        for i in range(n):
+            t0 = time.perf_counter()
+            sim.step(1)
+            step_s.append(time.perf_counter() - t0)
+            st = sim.state()
+            log(f"{form} step {i + 1}: {step_s[-1]:.3f}s, max mse {np.nanmax(st['mse']):.2e}, "
+                f"beta max {st['beta_volume'].max():.4f}")
+        t_after = sim.times()
+        st = sim.state()
+        total = float(np.sum(step_s))
+        rays = t_after["rays"] - t_before["rays"]
+        forms[form] = {"s_per_step": total / n, "cold_first_step_s": cold,
+                       "ase_s_per_step": (t_after["ase_ms"] - t_before["ase_ms"]) * 1e-3 / n,
+                       "integrator_ms_per_step": (t_after["integrator_ms"] - t_before["integrator_ms"]) / n,
+                       "ase_evaluations_per_step": (st["ase_evaluations"] - e0) / n,
+                       "rays_per_s": rays / total, "max_mse_last": float(np.nanmax(st["mse"])),
+                       "tuning": best, "beta": st["beta_volume"]}
+        sim.close()
+        ctx.close()
+    out = {"measurement": "time-stepped", "workload": "production_disk + local pump", "integrator": args.integrator,
+           "steps": n, "time_step_s": h, "prisms": topo.numberOfPrisms, "sample_points": w.samples}
+    for form, r in forms.items():
+        beta = r.pop("beta")
+        out[form] = r
+        r["beta_change_max"] = float(np.abs(beta - beta0).max())
+    if "learned" in forms:
+        out["speedup_vs_plain"] = forms["plain"]["s_per_step"] / forms["learned"]["s_per_step"]
+    print(json.dumps(out), flush=True)
+
+
 def traffic_pass(w, args):
     """Kernels for the FETCH_SIZE / WRITE_SIZE counter runs: the learned
     evaluation (its hase_trace_kernel launches are the bench line's), a plain
@@ -265,6 +335,10 @@ def main():
     ap.add_argument("--batch-triangles", default="200,2000")
     ap.add_argument("--batch-levels", default="5,20")
     ap.add_argument("--batch-rays", default="10000,10000000")
+    ap.add_argument("--time-steps", type=int, default=0, metavar="N",
+                    help="separate measurement: N warm steps of the time-stepped beta loop (plain and learned)")
+    ap.add_argument("--integrator", default="frozen-phi-ase-runge-kutta-4",
+                    help="time integrator of --time-steps")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
@@ -280,6 +354,9 @@ def main():
 
     if args.batch:
         batch_bench(args)
+        return
+    if args.time_steps:
+        time_bench(args)
         return
     w = production_disk(levels=args.levels)
     if args.traffic_pass:

@@ -298,7 +298,7 @@ def load():
     L.hase_simulation_set_beta.argtypes = [ctx_p, _f64p]
     L.hase_simulation_step.argtypes = [ctx_p, C.c_uint32]
     L.hase_simulation_state.argtypes = [ctx_p, C.POINTER(TimeState)]
-    L.hase_simulation_times.argtypes = [ctx_p, _f64p, _f64p]
+    L.hase_simulation_times.argtypes = [ctx_p, _f64p, _f64p, C.POINTER(C.c_uint64)]
     L.hase_simulation_destroy.argtypes = [ctx_p]
     L.hase_simulation_destroy.restype = None
     L.hase_last_error.restype = C.c_char_p
@@ -645,9 +645,9 @@ class NativeSimulation:
         return out
 
     def times(self):
-        a, b = C.c_double(), C.c_double()
-        check(load().hase_simulation_times(self._h, C.byref(a), C.byref(b)))
-        return {"ase_ms": a.value, "integrator_ms": b.value}
+        a, b, r = C.c_double(), C.c_double(), C.c_uint64()
+        check(load().hase_simulation_times(self._h, C.byref(a), C.byref(b), C.byref(r)))
+        return {"ase_ms": a.value, "integrator_ms": b.value, "rays": int(r.value)}
 
 
 def mlp_param_count():

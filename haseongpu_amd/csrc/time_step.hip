@@ -143,7 +143,7 @@ struct hase_simulation {
     uint32_t P = 0, S = 0;
     uint32_t completed = 0;
     uint32_t ase_evaluations = 0;
-    double sigma_sum = 0.0, sigma_a = 0.0, tau = 1.0;
+    double sigma_a = 0.0, tau = 1.0;
     bool has_sat = false;
     DevBuf<double> beta, beta_next, stage, deriv, dndt_pump, dndt_ase, phi_prism, k1, k2, k3, k4;
     DevBuf<double> pump_rate, pump_sat, block_sums;
@@ -153,6 +153,7 @@ struct hase_simulation {
     std::vector<double> mse, dndt_s, se;
     std::vector<uint32_t> rays, dropped;
     double ase_ms = 0.0, integrator_ms = 0.0;
+    uint64_t rays_total = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~hase_simulation()
@@ -285,6 +286,7 @@ void evaluate_stage(hase_simulation* sim, const double* beta, bool pump_on, bool
                               sim->se.data(), sim->dropped.data()};
             hase::evaluate_context(ctx, &res);
             sim->ase_evaluations++;
+            sim->rays_total += ctx->stats.rays;
             sim->ase_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             mode = kFluxAverage;
         } else {
@@ -518,12 +520,13 @@ int hase_simulation_state(const hase_simulation_t* sim, hase_time_state_t* st)
     });
 }
 
-int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* integrator_ms)
+int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* integrator_ms, uint64_t* rays)
 {
     return hase::guarded([&] {
         if (!sim) throw HaseError(HASE_E_INVALID, "simulation must not be NULL");
         if (ase_ms) *ase_ms = sim->ase_ms;
         if (integrator_ms) *integrator_ms = sim->integrator_ms;
+        if (rays) *rays = sim->rays_total;
     });
 }
 

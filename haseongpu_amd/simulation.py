@@ -500,8 +500,8 @@ class Simulation:
         return self
 
     def times(self):
-        """Milliseconds spent in ASE evaluations and in the integrator kernels."""
-        return {"ase_ms": 0.0, "integrator_ms": 0.0} if self._loop is None else self._loop.times()
+        """Milliseconds in ASE evaluations and in the integrator kernels, rays traced."""
+        return {"ase_ms": 0.0, "integrator_ms": 0.0, "rays": 0} if self._loop is None else self._loop.times()
 
     def close(self):
         if self._loop is not None:

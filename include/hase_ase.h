@@ -424,8 +424,10 @@ int hase_simulation_step(hase_simulation_t* sim, uint32_t n_steps);
 /* Copies the state after the last completed step. */
 int hase_simulation_state(const hase_simulation_t* sim, hase_time_state_t* state);
 
-/* Device times of the loop so far: ASE evaluations and the integrator kernels. */
-int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* integrator_ms);
+/* Totals of the loop so far (each pointer may be NULL): wall time of the ASE
+ * evaluations (beta refresh, proposal, rays), device time of the integrator
+ * kernels, rays traced. */
+int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* integrator_ms, uint64_t* rays);
 
 void hase_simulation_destroy(hase_simulation_t* sim);
 

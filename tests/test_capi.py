@@ -162,7 +162,7 @@ def test_time_loop_entry_points_reject_bad_arguments_on_the_host():
     assert lib.hase_simulation_set_beta(None, rp) == -1
     assert lib.hase_simulation_step(None, 1) == -1
     assert lib.hase_simulation_state(None, None) == -1
-    assert lib.hase_simulation_times(None, None, None) == -1
+    assert lib.hase_simulation_times(None, None, None, None) == -1
     lib.hase_simulation_destroy(None)
     text = HEADER.read_text()
     for name, code in _native.INTEGRATORS.items():

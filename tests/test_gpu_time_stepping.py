@@ -157,6 +157,7 @@ def test_production_disk_stage_composition():
     np.testing.assert_array_equal(st["beta_volume"][clad], beta0[clad])
     assert np.all(st["phi_ase"][~clad] > 0.0)
     assert times["ase_ms"] > 0.0 and times["integrator_ms"] > 0.0
+    assert times["rays"] == 20000 * w.samples
 
 
 def test_simulation_front_end_schedules_pumps_outputs_and_control():
