@@ -29,6 +29,7 @@ struct hase_context {
     std::vector<double> beta_host;     // [P]
     std::vector<double> beta_cells;    // [S]
     std::vector<uint32_t> tri_vertex;  // [3T] vertex k of device triangle t at 3t + k
+    std::vector<float2> pts_host;      // [Np] point coordinates as stored on the device
     double Bsum = 0.0;
     hase_experiment_t exp{};
     std::vector<double> sigma_a, sigma_e;

@@ -11,13 +11,14 @@ namespace hase {
 struct SampleGeom {
     float px, py;
     int plane;   // level of the sample point (its height in layer units)
+    uint32_t p;  // mesh point under the sample
 };
 
 __device__ __forceinline__ SampleGeom sample_point(const float2* pts, uint32_t np, uint32_t sample)
 {
     uint32_t p = sample % np, lv = sample / np;
     float2 xy = pts[p];
-    return SampleGeom{xy.x, xy.y, (int)lv};
+    return SampleGeom{xy.x, xy.y, (int)lv, p};
 }
 
 __device__ __forceinline__ double wave_sum(double v)

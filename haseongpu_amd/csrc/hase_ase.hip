@@ -28,6 +28,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hase_ase.h"
@@ -173,14 +174,15 @@ constexpr int kLevelGroup = 4;
 // prisms visited.
 template <bool kReflect>
 __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0, int lv0, int nlev, float px,
-                                                  float py, int end_plane, uint32_t nrefl, uint32_t first,
+                                                  float py, uint32_t pe, int end_plane, uint32_t nrefl, uint32_t first,
                                                   float out_bl[kLevelGroup], float out_gl[kLevelGroup],
                                                   float out_cl[kLevelGroup], float out_rf[kLevelGroup],
                                                   float D[kLevelGroup], unsigned long long& steps, bool& ok)
 {
     const int layers = (int)m.layers;
     const float h = m.h;
-    const uint4 c0 = m.rec[3u * t0], c1 = m.rec[3u * t0 + 1u];
+    uint4 c0, c1;
+    load_corners(m, t0, c0, c1);
     const float cx = (bits_to_float(c0.x) + bits_to_float(c0.z) + bits_to_float(c1.x)) * (1.0f / 3.0f);
     const float cy = (bits_to_float(c0.y) + bits_to_float(c0.w) + bits_to_float(c1.y)) * (1.0f / 3.0f);
     const float dx = px - cx, dy = py - cy;
@@ -208,8 +210,9 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
     float s = 0.0f;
     ok = true;
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
-        const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
-        const bool clad = q2.y != 0u;
+        uint4 q0, q1, q2;
+        load_triangle(m, t, q0, q1, q2);
+        const bool clad = record_cladding(q2) != 0u;
         const float* bcol = m.beta_col + (size_t)t * (uint32_t)layers;
         // this column's beta at every level's current layer: independent loads, one round trip
         float bcur[kLevelGroup];
@@ -231,19 +234,8 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
         s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
         s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
         s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-        int bestk = 0;
-        float best = s0;
-        if (s1 < best) { best = s1; bestk = 1; }
-        if (s2 < best) { best = s2; bestk = 2; }
-        {
-            const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
-            const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
-            const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
-            const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
-            const bool touches = (bits_to_float(axb) == px && bits_to_float(ayb) == py) ||
-                                 (bits_to_float(bxb) == px && bits_to_float(byb) == py);
-            best = (touches && best < INFINITY) ? Dxy : best;
-        }
+        const ExitEdge x = exit_edge(q1, q2, s0, s1, s2, pe);
+        const float best = (x.touches & (int)(x.s < INFINITY)) ? Dxy : x.s;
         const float snext = fmaxf(fminf(best, Dxy), s);
         const bool at_end = snext >= Dxy;
         const float sig0 = s * inv_xy;
@@ -290,7 +282,7 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
             }
         }
         s = snext;
-        const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
+        const uint32_t nw = x.nw;
         bool done = at_end;
         if (!done && nw == 0xFFFFFFFFu) {
             if (Dxy - s <= tol) done = true;
@@ -349,9 +341,11 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         float bl[kLevelGroup], gl[kLevelGroup], cl[kLevelGroup], rf[kLevelGroup], len[kLevelGroup];
         bool ok;
         if (nr == 0)
-            column_importance<false>(m, t0, lv0, nlev, sp.px, sp.py, end_plane, 0u, fp, bl, gl, cl, rf, len, steps, ok);
+            column_importance<false>(m, t0, lv0, nlev, sp.px, sp.py, sp.p, end_plane, 0u, fp, bl, gl, cl, rf, len,
+                                     steps, ok);
         else
-            column_importance<true>(m, t0, lv0, nlev, sp.px, sp.py, end_plane, nr, fp, bl, gl, cl, rf, len, steps, ok);
+            column_importance<true>(m, t0, lv0, nlev, sp.px, sp.py, sp.p, end_plane, nr, fp, bl, gl, cl, rf, len,
+                                    steps, ok);
 
         walks += (unsigned long long)nlev;
 #pragma unroll
@@ -525,7 +519,8 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
         philox4x32_10(ray, sample, p.launch, kTagRay, p.seed, kKeyTag, rnd);
         float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
         if (u1 + u2 > 1.0f) { u1 = 1.0f - u1; u2 = 1.0f - u2; }
-        uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u];
+        uint4 q0, q1;
+        load_corners(m, t, q0, q1);
         float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
         float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
         float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
@@ -534,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
         float az = (float)l + u3;   // layer units
         uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
         float ez = (float)image_plane(sp.plane, (int)m.layers, nr, fp);
-        WalkOut w = walk_ray(m, ax, ay, az, t, l, sp.px, sp.py, ez, nr, fp);
+        WalkOut w = walk_ray(m, ax, ay, az, t, l, sp.px, sp.py, ez, nr, fp, sp.p);
         steps += w.steps;
         rays++;
         if (w.status != 0 || !(w.length > 0.0f)) {
@@ -634,16 +629,16 @@ __global__ void hase_finalize_kernel(FinalizeParams p)
 }
 
 __global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* start, const uint32_t* prism,
-                                        const float* end, const uint32_t* nrefl, const uint32_t* first,
-                                        float* bl, float* gl, float* cl, float* refl, uint32_t* steps,
-                                        int32_t* status)
+                                        const float* end, const uint32_t* end_point, const uint32_t* nrefl,
+                                        const uint32_t* first, float* bl, float* gl, float* cl, float* refl,
+                                        uint32_t* steps, int32_t* status)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t j = prism[i];
     uint32_t t = j % m.T, l = j / m.T;
     WalkOut w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
-                         end[3 * i + 2], nrefl[i], first[i]);
+                         end[3 * i + 2], nrefl[i], first[i], end_point[i]);
     bl[i] = w.beta_len;
     gl[i] = w.gain_len;
     cl[i] = w.clad_len;
@@ -723,7 +718,10 @@ void validate(const hase_mesh_t* mesh, const hase_experiment_t* e, const hase_co
         throw HaseError(HASE_E_INVALID, "sample range must satisfy min <= max < number_of_points * number_of_levels");
     if (c->max_repetitions == 0) throw HaseError(HASE_E_INVALID, "max_repetitions must be >= 1");
     uint64_t P = (uint64_t)mesh->number_of_triangles * (mesh->number_of_levels - 1);
-    if (P * 3u > 0xFFFFFFFFull) throw HaseError(HASE_E_INVALID, "mesh too large");
+    // 24-bit triangle and level indices, 31-bit point indices
+    if (P * 3u > 0xFFFFFFFFull || mesh->number_of_triangles >= (1u << 24) || mesh->number_of_levels >= (1u << 24) ||
+        mesh->number_of_points >= 0x80000000u)
+        throw HaseError(HASE_E_INVALID, "mesh too large");
 }
 
 void set_params(hase_context* ctx, const hase_experiment_t* e, const hase_compute_t* c)
@@ -889,14 +887,15 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         auto fb = [](double x) { float f = (float)x; uint32_t u; std::memcpy(&u, &f, 4); return u; };
         rec[3u * td] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
         rec[3u * td + 1u] = make_uint4(fb(px[v[2]]), fb(py[v[2]]), nw[0], nw[1]);
-        rec[3u * td + 2u] = make_uint4(nw[2], clad, 0u, 0u);
+        rec[3u * td + 2u] = make_uint4(nw[2], v[0] | (clad << 31), v[1], v[2]);
         const float x0 = (float)px[v[0]], x1 = (float)px[v[1]], x2 = (float)px[v[2]];
         const float y0 = (float)py[v[0]], y1 = (float)py[v[1]], y2 = (float)py[v[2]];
         ctx->centroid_host[td] = make_float2((x0 + x1 + x2) * (1.0f / 3.0f), (y0 + y1 + y2) * (1.0f / 3.0f));
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
     ctx->centroid.upload(ctx->centroid_host.data(), T, ctx->stream);
-    std::vector<float2> pts(np);
+    std::vector<float2>& pts = ctx->pts_host;
+    pts.resize(np);
     for (uint32_t i = 0; i < np; ++i) pts[i] = make_float2((float)px[i], (float)py[i]);
     ctx->pts.upload(pts.data(), np, ctx->stream);
     if (mesh->reflectivities) {
@@ -1750,18 +1749,35 @@ int hase_context_trace_rays(hase_context_t* context, uint32_t n, const float* st
             uint32_t j = start_prism[i];
             dev_prism[i] = context->user2dev[j % context->T] + (j / context->T) * context->T;
         }
+        // the mesh point under each end (x, y), compared as stored in FP32
+        std::unordered_map<uint64_t, uint32_t> at;
+        at.reserve(context->pts_host.size());
+        auto key = [](float x, float y) {
+            uint32_t a, b;
+            std::memcpy(&a, &x, 4);
+            std::memcpy(&b, &y, 4);
+            return ((uint64_t)a << 32) | b;
+        };
+        for (uint32_t i = 0; i < (uint32_t)context->pts_host.size(); ++i)
+            at.emplace(key(context->pts_host[i].x, context->pts_host[i].y), i);
+        std::vector<uint32_t> end_point(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            auto it = at.find(key(end[3 * i], end[3 * i + 1]));
+            end_point[i] = it == at.end() ? hase::kNoPoint : it->second;
+        }
         DevBuf<float> d_start, d_end, d_bl, d_gl, d_cl, d_rf;
-        DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps;
+        DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps, d_pe;
         DevBuf<int32_t> d_status;
         d_start.upload(hs.data(), 3u * (size_t)n, st);
         d_end.upload(he.data(), 3u * (size_t)n, st);
+        d_pe.upload(end_point.data(), n, st);
         d_prism.upload(dev_prism.data(), n, st);
         d_nr.upload(reflections, n, st);
         d_fp.upload(first_plane, n, st);
         d_bl.alloc(n); d_gl.alloc(n); d_cl.alloc(n); d_rf.alloc(n); d_steps.alloc(n); d_status.alloc(n);
         hipLaunchKernelGGL(hase_single_rays_kernel, dim3((n + 127) / 128), dim3(128), 0, st, mesh_view(context), n,
-                           d_start.p, d_prism.p, d_end.p, d_nr.p, d_fp.p, d_bl.p, d_gl.p, d_cl.p, d_rf.p, d_steps.p,
-                           d_status.p);
+                           d_start.p, d_prism.p, d_end.p, d_pe.p, d_nr.p, d_fp.p, d_bl.p, d_gl.p, d_cl.p, d_rf.p,
+                           d_steps.p, d_status.p);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(beta_length, d_bl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(gain_length, d_gl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));

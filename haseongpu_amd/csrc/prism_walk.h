@@ -23,8 +23,13 @@ constexpr uint32_t kMaxSteps = 10000u;
 
 HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float, u); }
 
+// Triangle records, 48 B per triangle t at byte 48 t:
+//   {x0, y0, x1, y1}  {x2, y2, n0, n1}  {n2, v0 | cladding << 31, v1, v2}
+// with vertex coordinates as FP32 bits, n_k = neighbour << 2 | shared edge
+// index inside the neighbour (0xFFFFFFFF at the boundary) and v_k the mesh
+// point index of vertex k.
 struct MeshDev {
-    const uint4* rec;      // 3 records per triangle (see pack_triangles)
+    const uint4* rec;      // 3 records per triangle
     const float* beta;     // [P] level-major: t + l * T
     const float* beta_col; // [P] column-major: t * layers + l
     const float* bv;       // [P] beta * prism volume, level-major
@@ -85,6 +90,73 @@ HASE_HD inline float surface_factor(const MeshDev& m, uint32_t t, uint32_t plane
     return r > 0.0f ? r : 0.0f;
 }
 
+constexpr uint32_t kNoPoint = 0xFFFFFFFFu;
+
+// a * b for a, b < 2^24 (full-rate 24-bit multiply on the device); triangle
+// and level counts are kept below 2^24
+HASE_HD inline uint32_t mul24(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+// The three records of triangle t.  Offsets are formed in 32 bits from the
+// table base so that the loads use the scalar-base, 32-bit-offset form.
+HASE_HD inline void load_triangle(const MeshDev& m, uint32_t t, uint4& q0, uint4& q1, uint4& q2)
+{
+    const char* base = reinterpret_cast<const char*>(m.rec);
+    const uint32_t off = mul24(t, 48u);
+    q0 = *reinterpret_cast<const uint4*>(base + off);
+    q1 = *reinterpret_cast<const uint4*>(base + off + 16u);
+    q2 = *reinterpret_cast<const uint4*>(base + off + 32u);
+}
+
+HASE_HD inline void load_corners(const MeshDev& m, uint32_t t, uint4& q0, uint4& q1)
+{
+    const char* base = reinterpret_cast<const char*>(m.rec);
+    const uint32_t off = mul24(t, 48u);
+    q0 = *reinterpret_cast<const uint4*>(base + off);
+    q1 = *reinterpret_cast<const uint4*>(base + off + 16u);
+}
+
+HASE_HD inline float load_f32(const float* table, uint32_t index)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(table) + index * 4u);
+}
+
+HASE_HD inline uint32_t record_cladding(const uint4& q2) { return q2.y >> 31; }
+
+// Exit edge among the candidates s_k (edge k joins vertices k and k+1): its
+// parameter, the neighbour word behind it, and whether it ends at the mesh
+// point pe - the end is a vertex of the triangle other than the one opposite
+// the edge.
+struct ExitEdge {
+    float s;
+    uint32_t nw;
+    int touches;
+};
+
+HASE_HD inline ExitEdge exit_edge(const uint4& q1, const uint4& q2, float s0, float s1, float s2, uint32_t pe)
+{
+    const uint32_t v0 = q2.y & 0x7FFFFFFFu, v1 = q2.z, v2 = q2.w;
+    ExitEdge x{s0, q1.z, 0};
+    uint32_t opposite = v2;
+    const bool c1 = s1 < x.s;
+    x.s = c1 ? s1 : x.s;
+    x.nw = c1 ? q1.w : x.nw;
+    opposite = c1 ? v0 : opposite;
+    const bool c2 = s2 < x.s;
+    x.s = c2 ? s2 : x.s;
+    x.nw = c2 ? q2.x : x.nw;
+    opposite = c2 ? v1 : opposite;
+    const int vertex = (int)(v0 == pe) | (int)(v1 == pe) | (int)(v2 == pe);
+    x.touches = vertex & (int)(opposite != pe);
+    return x;
+}
+
 // 1/x with the hardware reciprocal (1 ulp) on the device.
 HASE_HD inline float fast_rcp(float x)
 {
@@ -101,8 +173,9 @@ HASE_HD inline float fast_rcp(float x)
 // sits on an exact integer plane, and the crossing of that plane is the end
 // of the walk.  Crossing parameters are measured from a; the exit edge is
 // the nearest one whose outward normal faces the ray (entry edge excluded);
-// an exit edge with an end point at the end's (x, y) meets the ray exactly
-// at the end; reflections are the copy boundaries of the unfolded z axis.
+// an exit edge with an end point at the end's (x, y) - the mesh point pe of
+// the sample, or of its mirror images - meets the ray exactly at the end;
+// reflections are the copy boundaries of the unfolded z axis.
 //
 // The walk is resumable: walk_begin sets up a WalkState and walk_advance
 // takes up to `budget` prism steps, so a lane can hand a finished ray back
@@ -113,6 +186,7 @@ struct WalkState {
     float vx, vy, D, kz, tol, sin2;
     int zdir, end_plane, end_on_plane;
     uint32_t nrefl, first;
+    uint32_t pe;              // mesh point under the end (kNoPoint: none)
     uint32_t t;               // current triangle
     int u, copy, r, entry;    // unfolded layer, mirror copy, layer in copy, entry edge
     float s;                  // distance walked
@@ -124,12 +198,12 @@ struct WalkState {
 };
 
 HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                               float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+                               float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
 {
     w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
     w.steps = 0; w.status = 0; w.done = 0;
     w.ax = ax; w.ay = ay; w.za = za; w.ex = ex; w.ey = ey;
-    w.nrefl = nrefl; w.first = first;
+    w.nrefl = nrefl; w.first = first; w.pe = pe;
     const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
     const float D = sqrtf(dx * dx + dy * dy + dz * dz);
     w.D = D;
@@ -173,8 +247,9 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         const uint32_t t = w.t;
         const int level = kReflect ? ((w.copy & 1) ? (layers - 1 - w.r) : w.r) : w.u;
         // the triangle record and this prism's beta are independent loads
-        const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u], q2 = m.rec[3u * t + 2u];
-        const float beta = m.beta[t + (uint32_t)level * m.T];
+        uint4 q0, q1, q2;
+        load_triangle(m, t, q0, q1, q2);
+        const float beta = load_f32(m.beta, t + mul24((uint32_t)level, m.T));
         const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy, D = w.D;
         const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
         const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
@@ -190,23 +265,11 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         s0 = (w.entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
         s1 = (w.entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
         s2 = (w.entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-        int bestk = 0;
-        float best = s0;
-        if (s1 < best) { best = s1; bestk = 1; }
-        if (s2 < best) { best = s2; bestk = 2; }
-        // the winning edge through the end's (x, y) meets the ray exactly there;
-        // compared on the stored coordinates, which no contraction of the
-        // relative ones can perturb
-        {
-            const uint32_t axb = bestk == 0 ? q0.x : (bestk == 1 ? q0.z : q1.x);
-            const uint32_t ayb = bestk == 0 ? q0.y : (bestk == 1 ? q0.w : q1.y);
-            const uint32_t bxb = bestk == 0 ? q0.z : (bestk == 1 ? q1.x : q0.x);
-            const uint32_t byb = bestk == 0 ? q0.w : (bestk == 1 ? q1.y : q0.y);
-            // bitwise, not short-circuit: no branches (and exec-mask traffic) per step
-            const int touches = ((int)(bits_to_float(axb) == w.ex) & (int)(bits_to_float(ayb) == w.ey)) |
-                                ((int)(bits_to_float(bxb) == w.ex) & (int)(bits_to_float(byb) == w.ey));
-            best = (touches & (int)(best < INFINITY)) ? D : best;
-        }
+        // the winning edge through the end's (x, y) meets the ray exactly
+        // there; decided on the vertices' point indices, which no rounding of
+        // the crossing parameters can perturb
+        const ExitEdge x = exit_edge(q1, q2, s0, s1, s2, w.pe);
+        const float best = (x.touches & (int)(x.s < INFINITY)) ? D : x.s;
         // next level plane along the ray (exactly D for the end's plane)
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
         float sz = w.zdir != 0 ? ((float)plane - w.za) * w.kz : INFINITY;
@@ -214,12 +277,12 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         float snext = fminf(fminf(best, sz), D);
         snext = fmaxf(snext, w.s);
         const float seg = snext - w.s;
-        const uint32_t clad = q2.y;
+        const uint32_t clad = record_cladding(q2);
         w.clad_len += clad ? seg : 0.0f;
         w.gain_len += clad ? 0.0f : seg;
         w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
         w.s = snext;
-        const uint32_t nw = bestk == 0 ? q1.z : (bestk == 1 ? q1.w : q2.x);
+        const uint32_t nw = x.nw;
         const bool zstep = sz <= best;
         bool done = w.s >= D;
         if (!done && !zstep && nw == 0xFFFFFFFFu) {
@@ -288,19 +351,19 @@ HASE_HD inline WalkOut walk_result(const WalkState& s)
 
 template <bool kReflect>
 HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
 {
     WalkState s;
-    walk_begin(s, m.h, ax, ay, za, t, layer, ex, ey, ze, nrefl, first);
+    walk_begin(s, m.h, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe);
     walk_advance<kReflect>(m, s, kMaxSteps + 1u);
     return walk_result(s);
 }
 
 HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first)
+                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
 {
-    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first)
-                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first);
+    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first, pe)
+                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe);
 }
 
 // Height, in layer units, of the mirror image of a sample point on plane

@@ -151,7 +151,8 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
     philox4x32_10(ray, c.sample, c.launch, kTagRay, c.seed, kKeyTag, rnd);
     float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
     if (u1 + u2 > 1.0f) { u1 = 1.0f - u1; u2 = 1.0f - u2; }
-    const uint4 q0 = m.rec[3u * t], q1 = m.rec[3u * t + 1u];
+    uint4 q0, q1;
+    load_corners(m, t, q0, q1);
     const float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
     const float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
     const float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
@@ -170,7 +171,7 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
 __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, LaneRay& lr)
 {
     const RayStart r = draw_ray(m, c, ray, lr.q, lr.j);
-    walk_begin(lr.w, m.h, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp);
+    walk_begin(lr.w, m.h, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p);
 }
 
 //
@@ -243,7 +244,8 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                     double qprob;
                     uint32_t j;
                     const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j);
-                    const WalkOut w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp);
+                    const WalkOut w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp,
+                                               c.sp.p);
                     finish(w, qprob, j);
                 }
             } else {
