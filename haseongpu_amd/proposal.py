@@ -100,19 +100,20 @@ def triangle_centroids_f32(topology):
 
 
 def random_pump_beta(topology, rng, *, spots=(1, 4), widths=(0.3, 1.5), amplitude=(0.1, 0.45), noise=0.01,
-                     cladding=None):
+                     cladding=None, center=(0.0, 0.0)):
     """Seeded random pump field: 1-4 Gaussian spots of random width and height
     (plus a random absorption profile through the thickness and N(0, noise)),
-    clipped to [0, 1]; cladding triangles get beta = 0."""
+    clipped to [0, 1]; cladding triangles get beta = 0.  Spot centres fall
+    within 0.77 of the mesh's extent around `center`."""
     cent = topology.prismCentroids()
-    pr = np.hypot(topology.points[:, 0], topology.points[:, 1])
+    pr = np.hypot(topology.points[:, 0] - center[0], topology.points[:, 1] - center[1])
     extent = float(pr.max())
     beta = np.zeros(cent.shape[0])
     H = float(topology.thickness) * (int(topology.levels) - 1)
     for _ in range(int(rng.integers(spots[0], spots[1] + 1))):
         r0 = extent * np.sqrt(rng.uniform(0.0, 0.6))
         a0 = rng.uniform(0.0, 2.0 * np.pi)
-        cx, cy = r0 * np.cos(a0), r0 * np.sin(a0)
+        cx, cy = center[0] + r0 * np.cos(a0), center[1] + r0 * np.sin(a0)
         w = rng.uniform(*widths)
         amp = rng.uniform(*amplitude)
         alpha = rng.uniform(0.0, 2.0) / max(H, 1e-9)
@@ -225,19 +226,21 @@ def fit(X, Y, *, epochs=12, batch=4096, learning_rate=2e-3, weight_decay=1e-5, s
 
 def train_on_workload(workload, *, fields=32, samples_per_field=64, rows_per_sample=1024, seed=0, device=0,
                       epochs=12, batch=4096, learning_rate=2e-3, cluster_triangles=4, cluster_levels=2,
-                      log=None):
+                      log=None, center=None):
     """Training set from seeded random pump fields on the workload's mesh and
-    a network fitted to it.  Returns (proposal, (X, Y))."""
+    a network fitted to it (pump spots around `center`, default the origin).
+    Returns (proposal, (X, Y))."""
     from .simulation import native_problem
 
     rng = np.random.default_rng(seed)
     topo = workload.topology
+    center = (0.0, 0.0) if center is None else center
     clad = np.asarray(workload.medium.value("claddingCellTypes")) == int(workload.medium.value("claddingNumber"))
     mesh, exp, comp, log_sigma = native_problem(workload, device=device)
     Xs, Ys = [], []
     with _native.Context(mesh, exp, comp) as ctx:
         for f in range(int(fields)):
-            beta = random_pump_beta(topo, rng, cladding=clad)
+            beta = random_pump_beta(topo, rng, cladding=clad, center=center)
             ctx.update_beta(beta)
             X, Y = training_rows(ctx, topo, beta, log_sigma=log_sigma, rng=rng, samples=samples_per_field,
                                  rows_per_sample=rows_per_sample)
