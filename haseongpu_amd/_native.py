@@ -197,6 +197,7 @@ class Tuning(C.Structure):
         ("refill_lanes", C.c_uint32),
         ("step_budget", C.c_uint32),
         ("use_graph", C.c_uint32),
+        ("stage_lds", C.c_uint32),
         ("probe_ms", C.c_double),
     ]
 

@@ -34,7 +34,7 @@ struct hase_context {
     hase_experiment_t exp{};
     std::vector<double> sigma_a, sigma_e;
     hase_compute_t comp{};
-    hase::DevBuf<uint4> rec;
+    hase::DevBuf<uint4> rec, lds_rec;
     hase::DevBuf<float2> pts;
     hase::DevBuf<float> beta, beta_col, bv, refl, c1, c2;
     hase::DevBuf<float> imp;
@@ -66,7 +66,7 @@ struct hase_context {
     hase_proposal_stats_t pstats{};
 
     // persistent work queue
-    hase_tuning_t tuning{HASE_TRACE_QUEUE, 512u, 256u, 1u, 0u, 16u, 1u, 0.0};
+    hase_tuning_t tuning{HASE_TRACE_QUEUE, 512u, 256u, 1u, 0u, 16u, 1u, 0u, 0.0};
     hase::DevBuf<uint32_t> q_flags, q_k, q_extra, q_launch, q_nrays, q_item_rays, q_target;   // [slots]
     hase::DevBuf<uint32_t> q_list, q_item_off;                                  // [slots], [slots + 1]
     hase::DevBuf<uint32_t> q_scalars;                                           // n_active, head, slots
@@ -81,6 +81,7 @@ struct hase_context {
     bool phi_keep = false;
     hase::DevBuf<float> phi_all;                                                // [S]
     hase::DevBuf<double> tri_area_dev;                                          // [T] device order
+    hase::DevBuf<uint32_t> clad_dev;                                            // [T] device order
 
     ~hase_context()
     {

@@ -210,9 +210,9 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
     float s = 0.0f;
     ok = true;
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
-        uint4 q0, q1, q2;
-        load_triangle(m, t, q0, q1, q2);
-        const bool clad = record_cladding(q2) != 0u;
+        uint4 qa, qb, qc;
+        load_triangle(m, t, qa, qb, qc);
+        const bool clad = record_cladding(qc) != 0u;
         const float* bcol = m.beta_col + (size_t)t * (uint32_t)layers;
         // this column's beta at every level's current layer: independent loads, one round trip
         float bcur[kLevelGroup];
@@ -221,9 +221,9 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
             const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
             bcur[g] = (g < nlev && !clad) ? bcol[lvl] : 0.0f;
         }
-        const float X0 = bits_to_float(q0.x) - cx, Y0 = bits_to_float(q0.y) - cy;
-        const float X1 = bits_to_float(q0.z) - cx, Y1 = bits_to_float(q0.w) - cy;
-        const float X2 = bits_to_float(q1.x) - cx, Y2 = bits_to_float(q1.y) - cy;
+        const float X0 = bits_to_float(qa.x) - cx, Y0 = bits_to_float(qa.y) - cy;
+        const float X1 = bits_to_float(qa.z) - cx, Y1 = bits_to_float(qa.w) - cy;
+        const float X2 = bits_to_float(qb.x) - cx, Y2 = bits_to_float(qb.y) - cy;
         const float ex0 = X1 - X0, ey0 = Y1 - Y0;
         const float ex1 = X2 - X1, ey1 = Y2 - Y1;
         const float ex2 = X0 - X2, ey2 = Y0 - Y2;
@@ -234,7 +234,7 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
         s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
         s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
         s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-        const ExitEdge x = exit_edge(q1, q2, s0, s1, s2, pe);
+        const ExitEdge x = exit_edge<GlobalTriangles>(qa, qb, qc, s0, s1, s2, pe, px, py);
         const float best = (x.touches & (int)(x.s < INFINITY)) ? Dxy : x.s;
         const float snext = fmaxf(fminf(best, Dxy), s);
         const bool at_end = snext >= Dxy;
@@ -658,7 +658,8 @@ __global__ void hase_phi_scatter_kernel(const float* __restrict__ phi, const uin
 // beta [P] (FP64, device prism order) -> FP32 beta, beta * V, column-major
 // beta, and per-block sums of beta * V (fixed order: deterministic).
 __global__ void __launch_bounds__(256) hase_refresh_beta_kernel(const double* __restrict__ beta,
-                                                                const double* __restrict__ area, double h,
+                                                                const double* __restrict__ area,
+                                                                const uint32_t* __restrict__ clad, double h,
                                                                 uint32_t T, uint32_t layers, float* __restrict__ b32,
                                                                 float* __restrict__ bv, float* __restrict__ bcol,
                                                                 double* __restrict__ block_sums)
@@ -671,9 +672,10 @@ __global__ void __launch_bounds__(256) hase_refresh_beta_kernel(const double* __
         const double v = beta[j];
         const uint32_t t = j % T, l = j / T;
         w = v > 0.0 ? v * area[t] * h : 0.0;
-        b32[j] = (float)v;
+        const float walk = clad[t] ? 0.0f : (float)v;   // the walks' copy: no beta in cladding prisms
+        b32[j] = walk;
         bv[j] = (float)w;
-        bcol[(size_t)t * layers + l] = (float)v;
+        bcol[(size_t)t * layers + l] = walk;
     }
     red[threadIdx.x] = w;
     __syncthreads();
@@ -753,7 +755,8 @@ void upload_beta(hase_context* ctx, const double* beta_volume, const double* bet
         // device prism j = t_dev + l * T reads the caller's prism dev2user[t_dev] + l * T
         double v = beta_volume[ctx->dev2user[j % T] + (j / T) * (size_t)T];
         if (!std::isfinite(v)) throw HaseError(HASE_E_INVALID, "beta_volume must be finite");
-        b[j] = (float)v;
+        // the walks' copy: cladding prisms absorb with -alpha and take no beta
+        b[j] = ctx->clad_flag[j % T] ? 0.0f : (float)v;
         double w = v > 0.0 ? v * ctx->tri_area[j % T] * ctx->h : 0.0;
         bv[j] = (float)w;
         B += w;
@@ -893,6 +896,23 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         ctx->centroid_host[td] = make_float2((x0 + x1 + x2) * (1.0f / 3.0f), (y0 + y1 + y2) * (1.0f / 3.0f));
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
+    if (T <= hase::kLdsTriangles) {
+        // compact records for LDS staging: corners + 21-bit neighbour fields + cladding bit
+        std::vector<uint4> lrec(2u * (size_t)T);
+        for (uint32_t td = 0; td < T; ++td) {
+            const uint4 a = rec[3u * td], b = rec[3u * td + 1u], c = rec[3u * td + 2u];
+            const uint32_t nwk[3] = {b.z, b.w, c.x};
+            uint64_t W = 0;
+            for (int k = 0; k < 3; ++k) {
+                const uint64_t f = nwk[k] == 0xFFFFFFFFu ? 0x1FFFFFull : (uint64_t)nwk[k];
+                W |= f << (21 * k);
+            }
+            W |= (uint64_t)(c.y >> 31) << 63;
+            lrec[2u * td] = a;
+            lrec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
+        }
+        ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
+    }
     ctx->centroid.upload(ctx->centroid_host.data(), T, ctx->stream);
     std::vector<float2>& pts = ctx->pts_host;
     pts.resize(np);
@@ -925,6 +945,7 @@ MeshDev mesh_view(const hase_context* ctx)
 {
     MeshDev m;
     m.rec = ctx->rec.p;
+    m.lds_rec = ctx->lds_rec.p;
     m.beta = ctx->beta.p;
     m.beta_col = ctx->beta_col.p;
     m.bv = ctx->bv.p;
@@ -1326,10 +1347,11 @@ void refresh_beta_device(hase_context* ctx, const double* beta, double* block_su
     const uint32_t T = ctx->T, layers = ctx->L - 1, P = ctx->P;
     static_assert(sizeof(double) == 8, "FP64");
     if (ctx->tri_area_dev.n < T) ctx->tri_area_dev.upload(ctx->tri_area.data(), T, ctx->stream);
+    if (ctx->clad_dev.n < T) ctx->clad_dev.upload(ctx->clad_flag.data(), T, ctx->stream);
     const uint32_t blocks = (P + 255u) / 256u;
     hipLaunchKernelGGL(hase_refresh_beta_kernel, dim3(blocks), dim3(256), 0, ctx->stream, beta,
-                       (const double*)ctx->tri_area_dev.p, ctx->h, T, layers, ctx->beta.p, ctx->bv.p,
-                       ctx->beta_col.p, block_sums);
+                       (const double*)ctx->tri_area_dev.p, (const uint32_t*)ctx->clad_dev.p, ctx->h, T, layers,
+                       ctx->beta.p, ctx->bv.p, ctx->beta_col.p, block_sums);
     HIP_CHECK(hipGetLastError());
     std::vector<double> sums(blocks);
     HIP_CHECK(hipMemcpyAsync(sums.data(), block_sums, blocks * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1590,7 +1612,7 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
         *best = cur;
         return;
     }
-    static const uint32_t blocks[] = {128u, 256u, 512u};
+    static const uint32_t blocks[] = {128u, 256u, 512u, 1024u};
     static const uint32_t items[] = {256u, 512u, 1024u, 2048u};
     static const uint32_t chunks[] = {1u, 2u, 4u};
     static const uint32_t refills[] = {0u, 8u, 16u, 32u};
@@ -1626,7 +1648,18 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
                 }
             }
         };
-        sweep(blocks, 3, &hase_tuning_t::block_size);
+        if (ctx->lds_rec.p) {
+            // one work-group per CU holds the staged mesh: try it with the largest block
+            hase_tuning_t t = cur;
+            t.stage_lds = cur.stage_lds ? 0u : 1u;
+            t.block_size = t.stage_lds ? 1024u : cur.block_size;
+            const double ms = time_it(t);
+            if (ms < 0.98 * best_ms) {
+                best_ms = ms;
+                cur = t;
+            }
+        }
+        sweep(blocks, 4, &hase_tuning_t::block_size);
         sweep(items, 4, &hase_tuning_t::rays_per_item);
         sweep(chunks, 3, &hase_tuning_t::queue_chunk);
         sweep(refills, 4, &hase_tuning_t::refill_lanes);

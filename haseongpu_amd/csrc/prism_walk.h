@@ -28,10 +28,17 @@ HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float
 // with vertex coordinates as FP32 bits, n_k = neighbour << 2 | shared edge
 // index inside the neighbour (0xFFFFFFFF at the boundary) and v_k the mesh
 // point index of vertex k.
+// Compact records, 32 B per triangle for staging the mesh in LDS:
+//   {x0, y0, x1, y1}  {x2, y2, W & 0xFFFFFFFF, W >> 32}
+// with W = f0 | f1 << 21 | f2 << 42 | cladding << 63 and f_k = n_k (21 bits:
+// neighbour < 2^19, shared edge index) or 0x1FFFFF at the boundary.
+constexpr uint32_t kLdsTriangles = 4800;   // 150 KiB of LDS
+
 struct MeshDev {
     const uint4* rec;      // 3 records per triangle
-    const float* beta;     // [P] level-major: t + l * T
-    const float* beta_col; // [P] column-major: t * layers + l
+    const uint4* lds_rec;  // 2 compact records per triangle, or null when the mesh exceeds kLdsTriangles
+    const float* beta;     // [P] level-major: t + l * T, zero in cladding prisms
+    const float* beta_col; // [P] column-major: t * layers + l, zero in cladding prisms
     const float* bv;       // [P] beta * prism volume, level-major
     const float* refl;     // [2T] or nullptr
     float nin_b, nout_b, nin_t, nout_t;
@@ -129,31 +136,81 @@ HASE_HD inline float load_f32(const float* table, uint32_t index)
 
 HASE_HD inline uint32_t record_cladding(const uint4& q2) { return q2.y >> 31; }
 
-// Exit edge among the candidates s_k (edge k joins vertices k and k+1): its
-// parameter, the neighbour word behind it, and whether it ends at the mesh
-// point pe - the end is a vertex of the triangle other than the one opposite
-// the edge.
+// A fetch gives the walk one triangle in the layout of the 48-byte records,
+// as three vectors (kept as separate values so that the walk's selects stay
+// selects of values, not of addresses into a stack copy):
+//   a = {x0, y0, x1, y1}, b = {x2, y2, nw0, nw1}, c = {nw2, v0 | cladding << 31, v1, v2}
+// with nw_k = neighbour << 2 | shared edge index, or the fetch's kBoundary.
+
+// From the 48-byte records in global memory; the end point is identified by
+// its mesh point index pe.
+struct GlobalTriangles {
+    static constexpr uint32_t kBoundary = 0xFFFFFFFFu;   // neighbour word at the mesh boundary
+    HASE_HD void operator()(const MeshDev& m, uint32_t t, uint4& a, uint4& b, uint4& c) const
+    {
+        load_triangle(m, t, a, b, c);
+    }
+    // the end is a corner other than the one facing the exit edge
+    HASE_HD static int touches(const uint4&, const uint4&, const uint4& c, bool c1, bool c2, uint32_t pe, float,
+                               float)
+    {
+        const uint32_t v0 = c.y & 0x7FFFFFFFu, v1 = c.z, v2 = c.w;
+        uint32_t opposite = c1 ? v0 : v2;
+        opposite = c2 ? v1 : opposite;
+        const int vertex = (int)(v0 == pe) | (int)(v1 == pe) | (int)(v2 == pe);
+        return vertex & (int)(opposite != pe);
+    }
+};
+
+// From the 32-byte compact records staged in LDS; the end point is
+// identified by its stored FP32 coordinates (ex, ey), which equal the
+// corners' of the same mesh point bit for bit.
+struct LdsTriangles {
+    static constexpr uint32_t kBoundary = 0x1FFFFFu;
+    const uint4* rec;
+    HASE_HD void operator()(const MeshDev&, uint32_t t, uint4& a, uint4& b, uint4& c) const
+    {
+        const uint4 q1 = rec[2u * t + 1u];
+        a = rec[2u * t];
+        b = make_uint4(q1.x, q1.y, q1.z & 0x1FFFFFu, ((q1.z >> 21) | (q1.w << 11)) & 0x1FFFFFu);
+        c = make_uint4((q1.w >> 10) & 0x1FFFFFu, q1.w & 0x80000000u, 0u, 0u);
+    }
+    HASE_HD static int touches(const uint4& a, const uint4& b, const uint4&, bool c1, bool c2, uint32_t, float ex,
+                               float ey)
+    {
+        const float x0 = bits_to_float(a.x), y0 = bits_to_float(a.y);
+        const float x1 = bits_to_float(a.z), y1 = bits_to_float(a.w);
+        const float x2 = bits_to_float(b.x), y2 = bits_to_float(b.y);
+        float ox = c1 ? x0 : x2, oy = c1 ? y0 : y2;
+        ox = c2 ? x1 : ox;
+        oy = c2 ? y1 : oy;
+        const int vertex = ((int)(x0 == ex) & (int)(y0 == ey)) | ((int)(x1 == ex) & (int)(y1 == ey)) |
+                           ((int)(x2 == ex) & (int)(y2 == ey));
+        return vertex & ((int)(ox != ex) | (int)(oy != ey));
+    }
+};
+
+// Exit edge among the candidates s_k (edge k joins corners k and k+1): its
+// parameter, the neighbour word behind it, and whether it ends at the end
+// point.
 struct ExitEdge {
     float s;
     uint32_t nw;
     int touches;
 };
 
-HASE_HD inline ExitEdge exit_edge(const uint4& q1, const uint4& q2, float s0, float s1, float s2, uint32_t pe)
+template <class Fetch>
+HASE_HD inline ExitEdge exit_edge(const uint4& a, const uint4& b, const uint4& c, float s0, float s1, float s2,
+                                  uint32_t pe, float ex, float ey)
 {
-    const uint32_t v0 = q2.y & 0x7FFFFFFFu, v1 = q2.z, v2 = q2.w;
-    ExitEdge x{s0, q1.z, 0};
-    uint32_t opposite = v2;
+    ExitEdge x{s0, b.z, 0};
     const bool c1 = s1 < x.s;
     x.s = c1 ? s1 : x.s;
-    x.nw = c1 ? q1.w : x.nw;
-    opposite = c1 ? v0 : opposite;
+    x.nw = c1 ? b.w : x.nw;
     const bool c2 = s2 < x.s;
     x.s = c2 ? s2 : x.s;
-    x.nw = c2 ? q2.x : x.nw;
-    opposite = c2 ? v1 : opposite;
-    const int vertex = (int)(v0 == pe) | (int)(v1 == pe) | (int)(v2 == pe);
-    x.touches = vertex & (int)(opposite != pe);
+    x.nw = c2 ? c.x : x.nw;
+    x.touches = Fetch::touches(a, b, c, c1, c2, pe, ex, ey);
     return x;
 }
 
@@ -191,7 +248,7 @@ struct WalkState {
     int u, copy, r, entry;    // unfolded layer, mirror copy, layer in copy, entry edge
     float s;                  // distance walked
     uint32_t applied;         // surface factors applied
-    float beta_len, gain_len, clad_len, refl;
+    float beta_len, clad_len, refl;   // gain-prism length = walked length - clad_len
     uint32_t steps;
     int status;               // 0 ok, 1 left the mesh, 2 step limit
     int done;
@@ -200,7 +257,7 @@ struct WalkState {
 HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float za, uint32_t t, uint32_t layer,
                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
 {
-    w.beta_len = 0.0f; w.gain_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
+    w.beta_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
     w.steps = 0; w.status = 0; w.done = 0;
     w.ax = ax; w.ay = ay; w.za = za; w.ex = ex; w.ey = ey;
     w.nrefl = nrefl; w.first = first; w.pe = pe;
@@ -233,8 +290,8 @@ HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float 
 
 // kReflect = false is the direct walk (start and end inside the slab), which
 // needs neither the mirror folding nor surface factors.
-template <bool kReflect>
-HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget)
+template <bool kReflect, class Fetch = GlobalTriangles>
+HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget, const Fetch& fetch = Fetch())
 {
     const int layers = (int)m.layers;
     for (uint32_t it = 0; it < budget && !w.done; ++it) {
@@ -247,13 +304,13 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         const uint32_t t = w.t;
         const int level = kReflect ? ((w.copy & 1) ? (layers - 1 - w.r) : w.r) : w.u;
         // the triangle record and this prism's beta are independent loads
-        uint4 q0, q1, q2;
-        load_triangle(m, t, q0, q1, q2);
+        uint4 qa, qb, qc;
+        fetch(m, t, qa, qb, qc);
         const float beta = load_f32(m.beta, t + mul24((uint32_t)level, m.T));
         const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy, D = w.D;
-        const float X0 = bits_to_float(q0.x) - ax, Y0 = bits_to_float(q0.y) - ay;
-        const float X1 = bits_to_float(q0.z) - ax, Y1 = bits_to_float(q0.w) - ay;
-        const float X2 = bits_to_float(q1.x) - ax, Y2 = bits_to_float(q1.y) - ay;
+        const float X0 = bits_to_float(qa.x) - ax, Y0 = bits_to_float(qa.y) - ay;
+        const float X1 = bits_to_float(qa.z) - ax, Y1 = bits_to_float(qa.w) - ay;
+        const float X2 = bits_to_float(qb.x) - ax, Y2 = bits_to_float(qb.y) - ay;
         // edge k joins vertex k and k+1; crossing parameter num / nv for nv > 0
         const float ex0 = X1 - X0, ey0 = Y1 - Y0;
         const float ex1 = X2 - X1, ey1 = Y2 - Y1;
@@ -268,7 +325,7 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         // the winning edge through the end's (x, y) meets the ray exactly
         // there; decided on the vertices' point indices, which no rounding of
         // the crossing parameters can perturb
-        const ExitEdge x = exit_edge(q1, q2, s0, s1, s2, w.pe);
+        const ExitEdge x = exit_edge<Fetch>(qa, qb, qc, s0, s1, s2, w.pe, w.ex, w.ey);
         const float best = (x.touches & (int)(x.s < INFINITY)) ? D : x.s;
         // next level plane along the ray (exactly D for the end's plane)
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
@@ -277,15 +334,15 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
         float snext = fminf(fminf(best, sz), D);
         snext = fmaxf(snext, w.s);
         const float seg = snext - w.s;
-        const uint32_t clad = record_cladding(q2);
+        const uint32_t clad = record_cladding(qc);
+        // the walk's beta copy is zero in cladding prisms
         w.clad_len += clad ? seg : 0.0f;
-        w.gain_len += clad ? 0.0f : seg;
-        w.beta_len = fmaf(clad ? 0.0f : beta, seg, w.beta_len);
+        w.beta_len = fmaf(beta, seg, w.beta_len);
         w.s = snext;
         const uint32_t nw = x.nw;
         const bool zstep = sz <= best;
         bool done = w.s >= D;
-        if (!done && !zstep && nw == 0xFFFFFFFFu) {
+        if (!done && !zstep && nw == Fetch::kBoundary) {
             // leaving the mesh sideways: only rounding short of the end (a sample
             // point on the rim) counts as arrival
             if (D - w.s <= w.tol) done = true;
@@ -340,7 +397,7 @@ HASE_HD inline WalkOut walk_result(const WalkState& s)
 {
     WalkOut w;
     w.beta_len = s.beta_len;
-    w.gain_len = s.gain_len;
+    w.gain_len = s.s - s.clad_len;
     w.clad_len = s.clad_len;
     w.refl = s.refl;
     w.length = s.D;
@@ -349,21 +406,24 @@ HASE_HD inline WalkOut walk_result(const WalkState& s)
     return w;
 }
 
-template <bool kReflect>
+template <bool kReflect, class Fetch = GlobalTriangles>
 HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
+                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe,
+                                  const Fetch& fetch = Fetch())
 {
     WalkState s;
     walk_begin(s, m.h, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe);
-    walk_advance<kReflect>(m, s, kMaxSteps + 1u);
+    walk_advance<kReflect>(m, s, kMaxSteps + 1u, fetch);
     return walk_result(s);
 }
 
+template <class Fetch = GlobalTriangles>
 HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
+                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe,
+                                const Fetch& fetch = Fetch())
 {
-    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first, pe)
-                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe);
+    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first, pe, fetch)
+                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe, fetch);
 }
 
 // Height, in layer units, of the mirror image of a sample point on plane

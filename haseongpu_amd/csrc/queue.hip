@@ -39,7 +39,7 @@ namespace hase {
 namespace {
 
 constexpr int kPlanBlock = 1024;
-constexpr int kQueueMaxBlock = 512;
+constexpr int kQueueMaxBlock = 1024;
 enum { kNActive = 0, kHead = 1, kSlots = 2 };
 
 __global__ void __launch_bounds__(kPlanBlock) hase_queue_plan_kernel(QueueState q)
@@ -177,12 +177,22 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
 //
 // kTable: per-problem constants from the device table (batches of several
 // problems); otherwise from the kernel argument `one` (a single problem), so
-// they sit in scalar registers.
-template <bool kRefill, bool kTable>
+// they sit in scalar registers.  kLds (single problem, mesh of at most
+// kLdsTriangles triangles): the work-group first copies the compact triangle
+// records into LDS and the walks read them there; beta stays in global memory.
+template <bool kRefill, bool kTable, bool kLds>
 __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q, ProblemDev one)
 {
     const uint32_t n_active = q.scalars[kNActive];
     if (n_active == 0u) return;
+    const uint4* lds_rec = nullptr;
+    if constexpr (kLds) {
+        __shared__ uint4 s_tri[2u * kLdsTriangles];
+        const uint32_t n = 2u * one.m.T;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_tri[i] = one.m.lds_rec[i];
+        __syncthreads();
+        lds_rec = s_tri;
+    }
     const uint32_t total = q.item_off[n_active];
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -244,8 +254,12 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                     double qprob;
                     uint32_t j;
                     const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j);
-                    const WalkOut w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp,
-                                               c.sp.p);
+                    WalkOut w;
+                    if constexpr (kLds)
+                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p,
+                                     LdsTriangles{lds_rec});
+                    else
+                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p);
                     finish(w, qprob, j);
                 }
             } else {
@@ -363,7 +377,8 @@ void check_tuning(const hase_tuning_t& t)
 {
     if (t.mode != HASE_TRACE_QUEUE && t.mode != HASE_TRACE_ROUNDS) throw HaseError(HASE_E_INVALID, "unknown trace mode");
     if (t.block_size < 64 || t.block_size > (uint32_t)kQueueMaxBlock || t.block_size % 64)
-        throw HaseError(HASE_E_INVALID, "block_size must be a multiple of 64 in [64, 512]");
+        throw HaseError(HASE_E_INVALID, "block_size must be a multiple of 64 in [64, 1024]");
+    if (t.stage_lds > 1) throw HaseError(HASE_E_INVALID, "stage_lds must be 0 or 1");
     if (t.rays_per_item < 64 || t.rays_per_item % 64 || t.rays_per_item > (1u << 20))
         throw HaseError(HASE_E_INVALID, "rays_per_item must be a multiple of 64 up to 2^20");
     if (t.queue_chunk < 1 || t.queue_chunk > 1024) throw HaseError(HASE_E_INVALID, "queue_chunk must be in [1, 1024]");
@@ -471,8 +486,11 @@ float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const Q
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     const bool refill = tu.refill_lanes != 0u;
     const bool table = single == nullptr;
-    auto kernel = refill ? (table ? hase_queue_trace_kernel<true, true> : hase_queue_trace_kernel<true, false>)
-                         : (table ? hase_queue_trace_kernel<false, true> : hase_queue_trace_kernel<false, false>);
+    // LDS staging: single problem, no lane refill, mesh small enough
+    const bool lds = tu.stage_lds && !table && !refill && single->m.lds_rec != nullptr;
+    auto kernel = lds ? hase_queue_trace_kernel<false, false, true>
+                : refill ? (table ? hase_queue_trace_kernel<true, true, false> : hase_queue_trace_kernel<true, false, false>)
+                         : (table ? hase_queue_trace_kernel<false, true, false> : hase_queue_trace_kernel<false, false, false>);
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)tu.block_size, 0));
     ProblemDev one;
     std::memset(&one, 0, sizeof(one));

@@ -305,12 +305,13 @@ int hase_context_proposal_stats(const hase_context_t* context, hase_proposal_sta
 
 typedef struct hase_tuning {
     uint32_t mode;            /* HASE_TRACE_QUEUE (default) or HASE_TRACE_ROUNDS (one launch per round) */
-    uint32_t block_size;      /* queue kernel threads per work-group: 64..512, multiple of 64 */
+    uint32_t block_size;      /* queue kernel threads per work-group: 64..1024, multiple of 64 */
     uint32_t rays_per_item;   /* rays per queue item, multiple of 64 */
     uint32_t queue_chunk;     /* items a wavefront takes per queue access, >= 1 */
     uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64; 0: each lane walks its rays to the end */
     uint32_t step_budget;     /* prism steps between refill checks, >= 1 */
     uint32_t use_graph;       /* 1: replay the rounds of a batch as a captured hipGraph */
+    uint32_t stage_lds;       /* 1: stage the triangle records in LDS (meshes up to 4800 triangles, no refill) */
     double probe_ms;          /* hase_context_autotune: device time of the chosen setting's probe */
 } hase_tuning_t;
 

@@ -131,7 +131,7 @@ def test_network_entry_points_reject_bad_arguments_on_the_host():
 
 def test_tuning_entry_points_reject_bad_arguments_on_the_host():
     lib = _native.load()
-    t = _native.Tuning(0, 256, 1024, 2, 16, 8, 1, 0.0)
+    t = _native.Tuning(0, 256, 1024, 2, 16, 8, 1, 0, 0.0)
     assert lib.hase_context_set_tuning(None, C.byref(t)) == -1
     assert lib.hase_context_get_tuning(None, C.byref(t)) == -1
     assert lib.hase_context_autotune(None, 0, C.byref(t)) == -1

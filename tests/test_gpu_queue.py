@@ -28,7 +28,8 @@ def _modes(ctx):
     out = {}
     for name, kw in (("rounds", dict(mode=_native.TRACE_ROUNDS)),
                      ("queue", dict(mode=_native.TRACE_QUEUE, use_graph=0)),
-                     ("graph", dict(mode=_native.TRACE_QUEUE, use_graph=1))):
+                     ("graph", dict(mode=_native.TRACE_QUEUE, use_graph=1)),
+                     ("lds", dict(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=1, block_size=1024))):
         ctx.set_tuning(**kw)
         out[name] = ctx.evaluate()
     return out
@@ -50,12 +51,17 @@ def test_queue_traces_the_same_rays_as_rounds(case):
     assert len(np.unique(r["rounds"].totalRays)) > 2       # the schedule escalated
     _same(r["rounds"], r["queue"])
     _same(r["rounds"], r["graph"])
+    # triangle records staged in LDS: the same walks, bit for bit
+    np.testing.assert_array_equal(r["lds"].phiAse, r["graph"].phiAse)
+    np.testing.assert_array_equal(r["lds"].totalRays, r["graph"].totalRays)
     assert st["rays"] == int(r["graph"].totalRays.astype(np.int64).sum())
 
 
 @pytest.mark.parametrize("shape", [dict(block_size=64, rays_per_item=64, queue_chunk=1, refill_lanes=1, step_budget=1),
                                    dict(block_size=512, rays_per_item=4096, queue_chunk=7, refill_lanes=64,
-                                        step_budget=10001)])
+                                        step_budget=10001),
+                                   dict(block_size=64, rays_per_item=128, queue_chunk=3, stage_lds=1),
+                                   dict(block_size=1024, rays_per_item=4096, queue_chunk=1, stage_lds=1)])
 def test_queue_extreme_launch_shapes(shape):
     w = plain_small()
     ctx, _ = _context(w, min_rays=3000, max_rays=12000, mse_threshold=1e-4, adaptive_steps=2, rng_seed=3)
@@ -108,7 +114,8 @@ def test_autotune_picks_a_valid_setting_and_keeps_results():
         ref = ctx.evaluate()
         best = ctx.autotune(256)
         assert best["mode"] == _native.TRACE_QUEUE and best["probe_ms"] > 0
-        assert best["block_size"] in (128, 256, 512) and best["rays_per_item"] in (256, 512, 1024, 2048)
+        assert best["block_size"] in (128, 256, 512, 1024) and best["rays_per_item"] in (256, 512, 1024, 2048)
+        assert best["stage_lds"] in (0, 1)
         assert ctx.tuning() == best
         got = ctx.evaluate()
     _same(ref, got)
