@@ -146,8 +146,8 @@ struct ImportanceParams {
     uint32_t slots;           // samples in this batch
     uint32_t R;               // reflection slices
     uint32_t n_lambda;
-    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a)
-    const float* c2;          // [n_lambda] nTot sigma_a
+    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a) log2(e)
+    const float* c2;          // [n_lambda] nTot sigma_a log2(e)
     float inv_lambda;
     double eps;               // defensive fraction
     double Bsum;              // sum of beta * V
@@ -456,8 +456,8 @@ struct TraceParams {
     const uint32_t* sample_of;// slot -> sample (nullptr: sample_base + slot)
     uint32_t R;
     uint32_t n_lambda;
-    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a)
-    const float* c2;          // [n_lambda] nTot sigma_a
+    const float* c1;          // [n_lambda] nTot (sigma_e + sigma_a) log2(e)
+    const float* c2;          // [n_lambda] nTot sigma_a log2(e)
     float inv_lambda;
     uint32_t seed, launch, nrays;
     uint32_t chunks;          // ray chunks per active sample
@@ -736,8 +736,10 @@ void set_params(hase_context* ctx, const hase_experiment_t* e, const hase_comput
     ctx->comp = *c;
     std::vector<float> c1(e->number_of_wavelengths), c2(e->number_of_wavelengths);
     for (uint32_t i = 0; i < e->number_of_wavelengths; ++i) {
-        c1[i] = (float)(ctx->n_tot * (e->sigma_e[i] + e->sigma_a[i]));
-        c2[i] = (float)(ctx->n_tot * e->sigma_a[i]);
+        // base-2 coefficients of the spectral gain (its exponent in units of log2)
+        const double log2e = 1.4426950408889634074;
+        c1[i] = (float)(ctx->n_tot * (e->sigma_e[i] + e->sigma_a[i]) * log2e);
+        c2[i] = (float)(ctx->n_tot * e->sigma_a[i] * log2e);
     }
     ctx->c1.upload(c1.data(), c1.size(), ctx->stream);
     ctx->c2.upload(c2.data(), c2.size(), ctx->stream);

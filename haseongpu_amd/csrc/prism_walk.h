@@ -438,17 +438,42 @@ HASE_HD inline int image_plane(int lv, int layers, uint32_t nrefl, uint32_t firs
     return (nrefl & 1u) ? -(mr - 1) * layers - lv : -mr * layers + lv;
 }
 
+constexpr float kLog2e = 1.44269504088896341f;
+
+// 2^x with the hardware exponential on the device.
+HASE_HD inline float fast_exp2(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_exp2f(x);
+#else
+    return exp2f(x);
+#endif
+}
+
 // Gain of a walked path averaged over the spectral samples.  The exponent is
 // linear in the walk's length sums, so one walk serves every wavelength:
 // E = nTot (se + sa) sum(beta l) - nTot sa sum(l) - alpha_clad sum(l_clad)
 // (g = nTot (beta (se + sa) - sa), include/kernels/forward/rayWalk.hpp:170-182).
-HASE_HD inline float spectral_gain(const float* c1, const float* c2, uint32_t n_lambda,
-                                               float inv_lambda, float clad_alpha, const WalkOut& w)
+// The tables hold the coefficients in base 2, c1 = nTot (se + sa) log2(e) and
+// c2 = nTot sa log2(e), so each wavelength costs two FMAs and one exp2.
+#if defined(__HIP_DEVICE_COMPILE__)
+// read-only for the whole launch and indexed uniformly across the wavefront:
+// the constant address space lets them be read through the scalar cache
+typedef const __attribute__((address_space(4))) float ConstF32;
+#else
+typedef const float ConstF32;
+#endif
+
+HASE_HD inline float spectral_gain(const float* c1, const float* c2, uint32_t n_lambda, float inv_lambda,
+                                   float clad_alpha, const WalkOut& w)
 {
-    const float tail = clad_alpha * w.clad_len;
+    ConstF32* k1 = (ConstF32*)c1;
+    ConstF32* k2 = (ConstF32*)c2;
+    const float tail = clad_alpha * kLog2e * w.clad_len;
     float g = 0.0f;
+#pragma unroll 4
     for (uint32_t lam = 0; lam < n_lambda; ++lam)
-        g += __expf(c1[lam] * w.beta_len - c2[lam] * w.gain_len - tail);
+        g += fast_exp2(fmaf(k1[lam], w.beta_len, fmaf(-k2[lam], w.gain_len, -tail)));
     return g * inv_lambda;
 }
 

@@ -22,8 +22,8 @@ namespace hase {
 struct ProblemDev {
     MeshDev m;
     const float2* pts;
-    const float* c1;           // [n_lambda] nTot (sigma_e + sigma_a)
-    const float* c2;           // [n_lambda] nTot sigma_a
+    const float* c1;           // [n_lambda] nTot (sigma_e + sigma_a) log2(e)
+    const float* c2;           // [n_lambda] nTot sigma_a log2(e)
     const double* prefix;      // [local slots * n] proposal prefix
     const double* Q;           // [local slots] proposal totals
     const uint32_t* sample_of; // local slot -> sample, or null: sample_base + local slot
