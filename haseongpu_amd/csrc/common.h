@@ -15,6 +15,10 @@
 namespace hase {
 
 constexpr int kBlock = 256;
+// Buckets of the proposal prefix per sample: bucket k holds the first entry
+// whose prefix exceeds k / kPrefixBuckets, so the ray search for a target in
+// [k, k+1) / kPrefixBuckets covers only the entries between two buckets.
+constexpr uint32_t kPrefixBuckets = 2048;
 constexpr float kPi = 3.14159265358979f;
 
 // message of the last failed call on this thread (hase_last_error)

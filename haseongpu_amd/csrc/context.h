@@ -39,6 +39,7 @@ struct hase_context {
     hase::DevBuf<float> beta, beta_col, bv, refl, c1, c2;
     hase::DevBuf<float> imp;
     hase::DevBuf<double> prefix, Q, s1, s2, bcells;
+    hase::DevBuf<uint32_t> buckets;    // [batch * kPrefixBuckets] prefix buckets per resident sample
     hase::DevBuf<double2> partial;
     hase::DevBuf<uint32_t> partial_drop, N, dropped, active_list, active_next, active_count, slot_sample;
     hase::DevBuf<int32_t> active_flag;

@@ -78,11 +78,14 @@ __device__ __forceinline__ int scan_slot(int i) { return i + i / kScanPer; }
 template <class F>
 __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_t P, uint32_t T, const float* bv,
                                                double eps, double Bsum, uint32_t R, F value, double* pre,
-                                               double* Qout, double* red, double* buf)
+                                               double* Qout, double* red, double* buf, uint32_t* bucket)
 {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     double* carry_s = red + kBlock / 64;
     double* total_s = red + kBlock / 64 + 1;
+    // buckets beyond the total (rounding) keep the last entry
+    if (bucket)
+        for (uint32_t k = (uint32_t)tid; k < kPrefixBuckets; k += (uint32_t)kBlock) bucket[k] = n - 1u;
     local = wave_sum(local);
     if (lane == 0) red[wid] = local;
     __syncthreads();
@@ -127,6 +130,21 @@ __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_
         const double excl = off + before;
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) buf[scan_slot(tid * kScanPer + i)] = excl + v[i];
+        if (bucket) {
+            // entry e takes the buckets k with prefix(e - 1) <= k / B < prefix(e)
+            const double B = (double)kPrefixBuckets, inv_b = 1.0 / B;
+            double lo = excl;
+#pragma unroll
+            for (int i = 0; i < kScanPer; ++i) {
+                const uint32_t e = base + (uint32_t)(tid * kScanPer + i);
+                const double hi = excl + v[i];
+                if (e < n && hi > lo) {
+                    for (uint32_t k = (uint32_t)ceil(lo * B); k < kPrefixBuckets && (double)k * inv_b < hi; ++k)
+                        bucket[k] = e;
+                }
+                lo = hi;
+            }
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
@@ -154,6 +172,7 @@ struct ImportanceParams {
     float* imp;               // [slots * R * P] scratch
     double* prefix;           // [slots * R * P]
     double* Q;                // [slots]
+    uint32_t* buckets = nullptr; // [slots * kPrefixBuckets] or null
     unsigned long long* counters;
     const uint32_t* active;   // block -> slot (nullptr: block index is the slot)
     const uint32_t* sample_of;// slot -> sample (nullptr: sample_base + slot)
@@ -373,7 +392,8 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     }
     if (p.transfer || p.parts > 1u) return;
     mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
-                   [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf);
+                   [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf,
+                   p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
 }
 
 // Prefix of importance maps written by split work-groups (one per sample here).
@@ -390,7 +410,8 @@ __global__ void __launch_bounds__(kBlock) hase_prefix_kernel(ImportanceParams p)
     double local = 0.0;
     for (uint32_t e = threadIdx.x; e < n; e += kBlock) local += (double)imp[e];
     mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, [&](const EntryIndex&, uint32_t e) { return imp[e]; },
-                   p.prefix + (size_t)slot * n, p.Q + slot, red, buf);
+                   p.prefix + (size_t)slot * n, p.Q + slot, red, buf,
+                   p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
 }
 
 // Learned proposal for one sample per work-group: the network's log transfer
@@ -409,6 +430,7 @@ struct LearnedParams {
     double eps, Bsum;
     double* prefix;
     double* Q;
+    uint32_t* buckets = nullptr;   // [slots * kPrefixBuckets] or null
 };
 
 __device__ __forceinline__ float learned_importance(const LearnedParams& p, const SampleGeom& sp, const float* lg,
@@ -446,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
         local += (double)value(x, e);
     }
     mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, value, p.prefix + (size_t)slot * n, p.Q + slot, red,
-                   buf);
+                   buf, p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
 }
 
 struct TraceParams {
@@ -465,6 +487,7 @@ struct TraceParams {
     const uint32_t* active;   // [active count] batch slots
     const double* prefix;
     const double* Q;
+    const uint32_t* buckets;  // [slots * kPrefixBuckets]
     double2* partial;         // [slots * max_chunks]
     uint32_t* partial_drop;   // [slots * max_chunks]
     uint32_t max_chunks;
@@ -506,7 +529,10 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
         if (ray >= p.nrays) break;
         double target = ((double)ray + U) * scale;
         // first entry whose prefix exceeds the target
-        size_t lo = 0, hi = n;
+        const uint32_t* bk = p.buckets + (size_t)slot * kPrefixBuckets;
+        const double kt = target * (double)kPrefixBuckets;
+        const uint32_t kb = kt >= (double)(kPrefixBuckets - 1u) ? kPrefixBuckets - 1u : (uint32_t)kt;
+        size_t lo = bk[kb], hi = (kb + 1u < kPrefixBuckets ? (size_t)bk[kb + 1u] : n - 1) + 1;
         while (lo < hi) {
             size_t mid = (lo + hi) >> 1;
             if (pre[mid] > target) hi = mid; else lo = mid + 1;
@@ -997,12 +1023,14 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t
         const uint32_t per_item = ctx->tuning.mode == HASE_TRACE_QUEUE
                                       ? (uint32_t)queue_items_per_slot(ctx, ctx->tuning)
                                       : max_chunks;
-        size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + (learned ? (size_t)R * ctx->C * 4u : 0u);
+        size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + kPrefixBuckets * 4u +
+                            (learned ? (size_t)R * ctx->C * 4u : 0u);
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
     batch = std::min(batch, want);
     ctx->imp.alloc((size_t)batch * n_ent);
     ctx->prefix.alloc((size_t)batch * n_ent);
+    ctx->buckets.alloc((size_t)batch * kPrefixBuckets);
     if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
     ctx->Q.alloc(batch);
     ctx->slot_sample.alloc(batch);
@@ -1071,6 +1099,7 @@ void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, u
         lp.Bsum = ctx->Bsum;
         lp.prefix = ctx->prefix.p;
         lp.Q = ctx->Q.p;
+        lp.buckets = ctx->buckets.p;
         hipLaunchKernelGGL(hase_learned_importance_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, lp);
         HIP_CHECK(hipGetLastError());
         tm.inf.emplace_back(a0, a1);
@@ -1091,6 +1120,7 @@ void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, u
         ip.imp = ctx->imp.p;
         ip.prefix = ctx->prefix.p;
         ip.Q = ctx->Q.p;
+        ip.buckets = ctx->buckets.p;
         ip.counters = ctx->counters.p;
         ip.active = nullptr;
         ip.sample_of = sample_of;
@@ -1191,6 +1221,7 @@ void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_
         tp.active = ctx->active_list.p;
         tp.prefix = ctx->prefix.p;
         tp.Q = ctx->Q.p;
+        tp.buckets = ctx->buckets.p;
         tp.partial = ctx->partial.p;
         tp.partial_drop = ctx->partial_drop.p;
         tp.max_chunks = max_chunks;

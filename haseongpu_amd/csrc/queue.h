@@ -26,6 +26,7 @@ struct ProblemDev {
     const float* c2;           // [n_lambda] nTot sigma_a log2(e)
     const double* prefix;      // [local slots * n] proposal prefix
     const double* Q;           // [local slots] proposal totals
+    const uint32_t* buckets;   // [local slots * kPrefixBuckets] prefix buckets
     const uint32_t* sample_of; // local slot -> sample, or null: sample_base + local slot
     const uint32_t* targets;   // cumulative ray target of schedule increase k, k < n_targets
     size_t n;                  // R * P entries per slot

@@ -121,6 +121,7 @@ struct LaneRay {
 struct ItemCtx {
     SampleGeom sp;
     const double* pre;
+    const uint32_t* bk;       // prefix buckets of the item's sample
     size_t n;
     double Q, U, scale;
     uint32_t sample, launch, r0, seed;
@@ -136,8 +137,10 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
 {
     const uint32_t T = m.T, P = T * m.layers;
     const double target = ((double)ray + c.U) * c.scale;
-    // first entry whose prefix exceeds the target
-    size_t lo = 0, hi = c.n;
+    // first entry whose prefix exceeds the target, between the target's buckets
+    const double kt = target * (double)kPrefixBuckets;
+    const uint32_t kb = kt >= (double)(kPrefixBuckets - 1u) ? kPrefixBuckets - 1u : (uint32_t)kt;
+    size_t lo = c.bk[kb], hi = (kb + 1u < kPrefixBuckets ? (size_t)c.bk[kb + 1u] : c.n - 1) + 1;
     while (lo < hi) {
         const size_t mid = (lo + hi) >> 1;
         if (c.pre[mid] > target) hi = mid; else lo = mid + 1;
@@ -226,6 +229,7 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
             c.sp = sample_point(pd.pts, m.np, c.sample);
             c.n = pd.n;
             c.pre = pd.prefix + (size_t)local * pd.n;
+            c.bk = pd.buckets + (size_t)local * kPrefixBuckets;
             c.Q = pd.Q[local];
             c.U = systematic_offset(pd.seed, c.sample, c.launch);
             c.scale = c.Q / (double)nrays;
@@ -460,6 +464,7 @@ ProblemDev queue_problem(const hase_context* ctx, uint32_t slot0, uint32_t base,
     pd.c2 = ctx->c2.p;
     pd.prefix = ctx->prefix.p;
     pd.Q = ctx->Q.p;
+    pd.buckets = ctx->buckets.p;
     pd.sample_of = mapped ? ctx->slot_sample.p : nullptr;
     pd.targets = ctx->q_targets.p;
     pd.n = (size_t)slices(ctx) * ctx->P;
