@@ -940,6 +940,10 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
             lrec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
         }
         ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
+        // staged walks run one 1024-thread work-group per CU
+        ctx->tuning.stage_lds = 1u;
+        ctx->tuning.block_size = 1024u;
+        ctx->tuning.rays_per_item = 512u;
     }
     ctx->centroid.upload(ctx->centroid_host.data(), T, ctx->stream);
     std::vector<float2>& pts = ctx->pts_host;

@@ -27,8 +27,8 @@ def _same(a, b):
 def _modes(ctx):
     out = {}
     for name, kw in (("rounds", dict(mode=_native.TRACE_ROUNDS)),
-                     ("queue", dict(mode=_native.TRACE_QUEUE, use_graph=0)),
-                     ("graph", dict(mode=_native.TRACE_QUEUE, use_graph=1)),
+                     ("queue", dict(mode=_native.TRACE_QUEUE, use_graph=0, stage_lds=0, block_size=512)),
+                     ("graph", dict(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=0, block_size=512)),
                      ("lds", dict(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=1, block_size=1024))):
         ctx.set_tuning(**kw)
         out[name] = ctx.evaluate()
