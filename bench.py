@@ -201,6 +201,64 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def forward_bench(args):
+    """Forward Tet4 estimator on the production disk split into tetrahedra
+    (3 per prism, cladding ring alpha 5.5 / cm, 64 wavelengths): one warm-up
+    and `--steps` timed evaluations of a fixed ray count; the FP64 CPU oracle
+    traces a bounded sample of the same rays on the host cores."""
+    from haseongpu_amd import _native
+    from haseongpu_amd.tet_mesh import TetMesh
+    from haseongpu_amd.workloads import production_disk
+
+    w = production_disk(levels=args.levels)
+    topo = w.topology
+    beta = np.asarray(w.medium.value("betaVolume"), dtype=np.float64)
+    mesh = TetMesh.fromPrisms(topo, beta, cladding_types=np.asarray(w.medium.value("claddingCellTypes")),
+                              cladding_number=int(w.medium.value("claddingNumber")), cladding_absorption=5.5,
+                              n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")))
+    sa, se, msa, mse_ = w.spectra.spectralPairs(False)
+    rays = int(args.forward)
+    exp = _native.NativeExperiment(sigma_a=sa, sigma_e=se, min_rays=rays, max_rays=rays, max_sigma_a=msa,
+                                   max_sigma_e=mse_)
+    comp = _native.make_compute(n_samples=1, adaptive_steps=0, rng_seed=5489)
+    log(f"forward: {mesh.n_cells} cells, {mesh.n_points} points, {rays} rays per evaluation")
+    ctx = _native.ForwardContext(mesh, exp, comp)
+    ctx.evaluate()
+    walls, traces, segs = [], [], 0
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        res = ctx.evaluate()
+        walls.append(time.perf_counter() - t0)
+        st = ctx.stats()
+        traces.append(st["trace_ms"] / 1e3)
+        segs = st["segments"]
+    ctx.close()
+    wall, trace = float(np.median(walls)), float(np.median(traces))
+    # per segment: 4 face planes (128 B), 4 links (16 B), 4 vertex ids (16 B), beta (8 B), cladding flag (1 B)
+    bytes_per_segment = 169
+    out = {"measurement": "forward", "workload": "production_disk_tet4", "cells": mesh.n_cells,
+           "rays": rays, "segments": segs, "segments_per_ray": segs / rays, "wall_s": wall, "trace_s": trace,
+           "rays_per_s": rays / wall, "trace_rays_per_s": rays / trace, "segments_per_s": segs / trace,
+           "bytes_per_segment": bytes_per_segment, "achieved_GBps": segs * bytes_per_segment / trace / 1e9,
+           "dropped_rays": int(res.droppedRays.sum()), "dtype": "f64"}
+    if not args.no_cpu_baseline:
+        from oracle import forward as F
+
+        o = F.ForwardOracle(mesh)
+        threads = min(16, os.cpu_count() or 1)
+        sample = 1000
+        while True:
+            t0 = time.perf_counter()
+            o.trace(sa, se, 5489, 0, sample, threads=threads)
+            dt = time.perf_counter() - t0
+            if dt > 3.0 or sample >= 1 << 24:
+                break
+            sample *= 4
+        out["cpu_baseline"] = {"value": sample / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "sample": f"{sample} rays of the same launch ({dt:.1f}s on {threads} threads)"}
+    print(json.dumps(out), flush=True)
+
+
 def time_bench(args):
     """Time-stepped beta loop on the production disk: a saturable Gaussian
     pump (peak 1500 /s, width 1.2 cm, absorbed along z) plus ASE, every ASE
@@ -339,6 +397,8 @@ def main():
                     help="separate measurement: N warm steps of the time-stepped beta loop (plain and learned)")
     ap.add_argument("--integrator", default="frozen-phi-ase-runge-kutta-4",
                     help="time integrator of --time-steps")
+    ap.add_argument("--forward", type=float, default=0, metavar="RAYS",
+                    help="forward Tet4 estimator on the production disk with this many rays per evaluation")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
@@ -357,6 +417,9 @@ def main():
         return
     if args.time_steps:
         time_bench(args)
+        return
+    if args.forward:
+        forward_bench(args)
         return
     w = production_disk(levels=args.levels)
     if args.traffic_pass:

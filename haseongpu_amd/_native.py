@@ -54,6 +54,13 @@ EXPORTED_SYMBOLS = (
     "hase_simulation_state",
     "hase_simulation_times",
     "hase_simulation_destroy",
+    "hase_forward_create",
+    "hase_forward_update_beta",
+    "hase_forward_evaluate",
+    "hase_forward_trace",
+    "hase_forward_stats",
+    "hase_forward_destroy",
+    "hase_calc_forward_phi_ase",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -255,6 +262,29 @@ PROPOSAL_LEARNED = 1
 _lib = None
 
 
+class TetMeshC(C.Structure):
+    _fields_ = [
+        ("number_of_points", C.c_uint32), ("number_of_cells", C.c_uint32), ("points", _f64p),
+        ("cells", _u32p), ("beta_volume", _f64p), ("cladding_cell_types", _u32p),
+        ("cladding_number", C.c_uint32), ("reserved", C.c_uint32), ("cladding_absorption", C.c_double),
+        ("n_tot", C.c_double), ("crystal_t_fluo", C.c_double),
+    ]
+
+
+class ForwardResultC(C.Structure):
+    _fields_ = [
+        ("phi_ase", _f32p), ("standard_error", _f64p), ("relative_standard_error", _f64p), ("dndt_ase", _f64p),
+        ("ray_visits", _u32p), ("dropped_rays", _u32p), ("convergence_rays", _u32p),
+    ]
+
+
+class ForwardStats(C.Structure):
+    _fields_ = [
+        ("trace_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64), ("segments", C.c_uint64),
+        ("dropped_rays", C.c_uint64), ("launches", C.c_uint32), ("open_cells", C.c_uint32),
+    ]
+
+
 def load():
     """Load the HIP library; raise if it has not been built."""
     global _lib
@@ -302,6 +332,16 @@ def load():
     L.hase_simulation_times.argtypes = [ctx_p, _f64p, _f64p, C.POINTER(C.c_uint64)]
     L.hase_simulation_destroy.argtypes = [ctx_p]
     L.hase_simulation_destroy.restype = None
+    L.hase_forward_create.argtypes = [C.POINTER(TetMeshC), C.POINTER(Experiment), C.POINTER(Compute),
+                                      C.POINTER(ctx_p)]
+    L.hase_forward_update_beta.argtypes = [ctx_p, _f64p]
+    L.hase_forward_evaluate.argtypes = [ctx_p, C.POINTER(ForwardResultC)]
+    L.hase_forward_trace.argtypes = [ctx_p, C.c_uint32, C.c_uint32, C.c_uint32, _f64p, _u32p, _u32p,
+                                     C.POINTER(C.c_uint64)]
+    L.hase_forward_stats.argtypes = [ctx_p, C.POINTER(ForwardStats)]
+    L.hase_forward_destroy.argtypes = [ctx_p]
+    L.hase_calc_forward_phi_ase.argtypes = [C.POINTER(TetMeshC), C.POINTER(Experiment), C.POINTER(Compute),
+                                            C.POINTER(ForwardResultC)]
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
@@ -313,7 +353,9 @@ def load():
                  "hase_context_get_tuning", "hase_context_autotune", "hase_batch_create",
                  "hase_batch_evaluate", "hase_batch_stats", "hase_simulation_create", "hase_simulation_set_pump",
                  "hase_simulation_set_beta", "hase_simulation_step", "hase_simulation_state",
-                 "hase_simulation_times"):
+                 "hase_simulation_times", "hase_forward_create", "hase_forward_update_beta",
+                 "hase_forward_evaluate", "hase_forward_trace", "hase_forward_stats", "hase_forward_destroy",
+                 "hase_calc_forward_phi_ase"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -715,4 +757,102 @@ def calc_phi_ase(mesh: NativeMesh, experiment: NativeExperiment, compute: Comput
     res = ResultArrays(n)
     check(load().hase_calc_phi_ase(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
                                    C.byref(res.struct)))
+    return res
+
+
+class NativeTetMesh:
+    """Owns the arrays a ``hase_tet_mesh_t`` points into (from a ``TetMesh``)."""
+
+    def __init__(self, mesh):
+        self.points = np.ascontiguousarray(mesh.points.T.reshape(-1), dtype=np.float64)
+        self.cells = np.array(mesh.cells.reshape(-1), dtype=np.uint32)
+        self.beta = np.array(mesh.beta, dtype=np.float64)
+        self.types = None if mesh.cell_types is None else np.array(mesh.cell_types, dtype=np.uint32)
+        self.struct = TetMeshC(mesh.n_points, mesh.n_cells, _ptr(self.points, C.c_double),
+                               _ptr(self.cells, C.c_uint32), _ptr(self.beta, C.c_double),
+                               _ptr(self.types, C.c_uint32), int(mesh.cladding_number), 0,
+                               float(mesh.cladding_absorption), float(mesh.n_tot), float(mesh.tau))
+
+
+class ForwardResultArrays:
+    def __init__(self, n_cells):
+        self.phiAse = np.zeros(n_cells, dtype=np.float32)
+        self.standardError = np.zeros(n_cells)
+        self.relativeStandardError = np.zeros(n_cells)
+        self.dndtAse = np.zeros(n_cells)
+        self.rayVisits = np.zeros(n_cells, dtype=np.uint32)
+        self.droppedRays = np.zeros(n_cells, dtype=np.uint32)
+        self.convergenceRays = np.zeros(n_cells, dtype=np.uint32)
+        self.struct = ForwardResultC(_ptr(self.phiAse, C.c_float), _ptr(self.standardError, C.c_double),
+                                     _ptr(self.relativeStandardError, C.c_double), _ptr(self.dndtAse, C.c_double),
+                                     _ptr(self.rayVisits, C.c_uint32), _ptr(self.droppedRays, C.c_uint32),
+                                     _ptr(self.convergenceRays, C.c_uint32))
+
+
+class ForwardContext:
+    """Persistent forward Tet4 estimator (``hase_forward_*``)."""
+
+    BATCHES = 8
+
+    def __init__(self, mesh, experiment: NativeExperiment, compute: Compute):
+        self._mesh = mesh if isinstance(mesh, NativeTetMesh) else NativeTetMesh(mesh)
+        self._exp, self._comp = experiment, compute
+        self.n_cells = self._mesh.struct.number_of_cells
+        self.n_points = self._mesh.struct.number_of_points
+        handle = C.c_void_p()
+        check(load().hase_forward_create(C.byref(self._mesh.struct), C.byref(experiment.struct), C.byref(compute),
+                                         C.byref(handle)))
+        self._h = handle
+
+    def update_beta(self, beta):
+        b = np.ascontiguousarray(beta, dtype=np.float64).reshape(-1)
+        if b.size != self.n_cells:
+            raise ValueError(f"beta needs {self.n_cells} values, got {b.size}")
+        check(load().hase_forward_update_beta(self._h, _ptr(b, C.c_double)))
+
+    def evaluate(self):
+        res = ForwardResultArrays(self.n_cells)
+        check(load().hase_forward_evaluate(self._h, C.byref(res.struct)))
+        return res
+
+    def trace(self, seed, launch, rays):
+        """Raw accumulators of one launch: scores (8, 2 Np), visits, dropped, segments."""
+        scores = np.zeros(self.BATCHES * 2 * self.n_points)
+        visits = np.zeros(self.n_cells, np.uint32)
+        dropped = np.zeros(self.n_cells, np.uint32)
+        seg = C.c_uint64(0)
+        check(load().hase_forward_trace(self._h, int(seed) & 0xFFFFFFFF, int(launch), int(rays),
+                                        _ptr(scores, C.c_double), _ptr(visits, C.c_uint32),
+                                        _ptr(dropped, C.c_uint32), C.byref(seg)))
+        return scores.reshape(self.BATCHES, -1), visits, dropped, int(seg.value)
+
+    def stats(self):
+        st = ForwardStats()
+        check(load().hase_forward_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ForwardStats._fields_}
+
+    def close(self):
+        if self._h:
+            load().hase_forward_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calc_forward_phi_ase(mesh, experiment: NativeExperiment, compute: Compute):
+    """One-shot ``hase_calc_forward_phi_ase`` call."""
+    m = mesh if isinstance(mesh, NativeTetMesh) else NativeTetMesh(mesh)
+    res = ForwardResultArrays(m.struct.number_of_cells)
+    check(load().hase_calc_forward_phi_ase(C.byref(m.struct), C.byref(experiment.struct), C.byref(compute),
+                                           C.byref(res.struct)))
     return res

@@ -1,0 +1,870 @@
+// forward.hip - forward Tet4 ASE estimator: rays leave beta*V-stratified
+// points of a tetrahedral gain medium in isotropic directions and score the
+// track-length integral of their gain into the vertices of every cell they
+// cross; each cell's flux is its mean lumped-vertex density.
+//
+// Follows the reference's present forward path without surface reflections:
+// AccumulateForwardPhiAse / ForwardAseCellPolicy
+// (include/kernels/forward/accumulation.hpp:300-405), the walk and its tie
+// and recovery rules (policyRay.hpp:379-465, rayWalk.hpp:106-208,
+// rayTransition.hpp:17-188), midpoint vertex weights (barycentric.hpp:69-99),
+// stratified beta*V sources and wavelengths (volumeSampling.hpp:19-137),
+// face planes and lumped vertex volumes (include/core/mesh.hpp:630-725,
+// include/kernels/vertexAccumulation.hpp:26-39), FinalizeForwardVolumePhiAse
+// (include/kernels/forwardPhiAseMapping.hpp:47-147) and the adaptive loop
+// (include/core/forwardPhiAseEvaluator.hpp:114-178, types.hpp:333-396).
+//
+// Layout in HBM (C cells, Np points): face planes 4 x double4 per cell
+// (128 B), neighbour links int32[4C] (neighbour * 4 + its local face, -1 at
+// the boundary), vertex ids uint4[C], beta f64[C], cladding flags u8[C],
+// beta*V prefix f64[C], lumped vertex volumes f64[2 Np], scores f64[8 x 2 Np]
+// (eight RSE batches x gain | cladding vertex copies), visits and dropped
+// u32[C].  One thread walks one ray in FP64 with contraction off, so a ray
+// takes exactly the cells the FP64 oracle takes; vertex scores are FP64
+// hardware atomics (their order, and exp/expm1 to the last bit, differ from
+// the oracle's).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+#include "prism_walk.h"
+
+namespace hase {
+namespace fwd {
+
+constexpr uint32_t kBatches = 8;
+constexpr uint32_t kKeyTag = 0x46574431u;
+constexpr uint32_t kMaxRecovery = 32;
+constexpr uint32_t kDirectionCalls = 32;
+constexpr double kTol = 64.0 * DBL_EPSILON;   // rayWalk.hpp:17
+constexpr double kProbe = 1.0e-12;            // rayTransition.hpp:18
+constexpr int kTraceBlock = 256;
+
+enum { kEntered = 0, kBoundary = 1, kFailed = 2 };
+
+struct TetDev {
+    const double4* planes;    // 4 per cell
+    const int32_t* link;      // 4 per cell
+    const uint4* verts;       // point ids of the 4 vertices
+    const double* beta;
+    const uint8_t* clad;
+    const double* bv_prefix;
+    const double* px;
+    const double* py;
+    const double* pz;
+    uint32_t n_cells, n_points;
+    double n_tot, clad_alpha;
+};
+
+struct TraceArgs {
+    TetDev m;
+    const double* sigma_a;
+    const double* sigma_e;
+    uint32_t n_lambda;
+    uint32_t seed, launch, rays;
+    double shift[kBatches];
+    uint32_t phase[kBatches];
+    double* scores;
+    uint32_t* visits;
+    uint32_t* dropped;
+    unsigned long long* segments;
+};
+
+struct Hit {
+    int face;
+    double length;
+    uint32_t tied;
+};
+
+struct Cross {
+    uint32_t cell;
+    int forbidden;
+    int status;
+};
+
+__device__ __forceinline__ double plane_at(double4 pl, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    return pl.x * x + pl.y * y + pl.z * z + pl.w;
+}
+
+__device__ __forceinline__ double plane_dir(double4 pl, double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    return pl.x * x + pl.y * y + pl.z * z;
+}
+
+__device__ __forceinline__ Hit next_face(const TetDev& m, uint32_t cell, const double p[3], const double u[3],
+                                         int forbidden)
+{
+#pragma clang fp contract(off)
+    Hit h{-1, DBL_MAX, 0u};
+    double4 pl[4];
+    double c[4], d[4], cand[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        pl[k] = m.planes[4 * (size_t)cell + k];
+        c[k] = plane_at(pl[k], p[0], p[1], p[2]);
+        d[k] = plane_dir(pl[k], u[0], u[1], u[2]);
+        cand[k] = 0.0;
+        if (k != forbidden && d[k] < 0.0) {
+            const double len = -c[k] / d[k];
+            cand[k] = (len > 0.0 && len <= DBL_MAX) ? len : 0.0;
+            if (cand[k] > 0.0 && cand[k] < h.length) {
+                h.length = cand[k];
+                h.face = k;
+            }
+        }
+    }
+    if (h.face < 0) return h;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k != forbidden && cand[k] > 0.0 && fabs(c[k] + h.length * d[k]) <= kTol) h.tied |= 1u << k;
+    h.tied |= 1u << h.face;
+    h.face = __builtin_ctz(h.tied);
+    return h;
+}
+
+// Crossing at a point where several faces meet (rayTransition.hpp:74-160).
+__device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const double p[3], const double u[3])
+{
+#pragma clang fp contract(off)
+    Cross r{cell, -1, kEntered};
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t recent0 = cell, recent1 = kNone, recent2 = kNone;
+    for (uint32_t step = 0; step < kMaxRecovery; ++step) {
+        const int32_t link = m.link[4 * (size_t)cell + exit_face];
+        if (link < 0) {
+            r.cell = cell;
+            r.status = kBoundary;
+            return r;
+        }
+        cell = (uint32_t)link >> 2;
+        r.cell = cell;
+        r.forbidden = link & 3;
+        recent2 = recent1;
+        recent1 = recent0;
+        recent0 = cell;
+        int fallback = -1, preferred = -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (preferred >= 0) continue;
+            const double4 pl = m.planes[4 * (size_t)cell + k];
+            double c = plane_at(pl, p[0], p[1], p[2]);
+            if (fabs(c) <= kTol) c = 0.0;
+            const double d = plane_dir(pl, u[0], u[1], u[2]);
+            if (c + kProbe * d >= 0.0) continue;
+            if (fallback < 0) fallback = k;
+            const int32_t cl = m.link[4 * (size_t)cell + k];
+            if (cl < 0) continue;
+            const uint32_t cn = (uint32_t)cl >> 2;
+            if (cn != recent0 && cn != recent1 && cn != recent2) preferred = k;
+        }
+        exit_face = preferred >= 0 ? preferred : fallback;
+        if (exit_face < 0) return r;
+    }
+    r.status = kFailed;
+    return r;
+}
+
+__device__ __forceinline__ Cross cross(const TetDev& m, uint32_t cell, Hit h, const double p[3], const double u[3])
+{
+    if (h.tied & (h.tied - 1u)) return recover(m, cell, h.face, p, u);
+    const int32_t link = m.link[4 * (size_t)cell + h.face];
+    if (link < 0) return Cross{cell, -1, kBoundary};
+    return Cross{(uint32_t)link >> 2, link & 3, kEntered};
+}
+
+// Start of ray r of a launch: contiguous RSE batch, stratified beta*V cell,
+// folded-cube point (Rocchini & Cignoni 2000), Marsaglia (1972) direction,
+// stratified wavelength.
+__device__ __forceinline__ void ray_start(const TraceArgs& a, uint32_t r, uint32_t& batch, uint32_t& cell,
+                                          double p[3], double u[3], uint32_t& lambda)
+{
+#pragma clang fp contract(off)
+    const uint32_t base = a.rays / kBatches, rem = a.rays % kBatches;
+    const uint32_t split = rem * (base + 1u);
+    const uint32_t b = r < split ? r / (base + 1u) : rem + (r - split) / base;
+    const uint32_t count = base + (b < rem ? 1u : 0u);
+    const uint32_t first = b * base + (b < rem ? b : rem);
+    const uint32_t i = r - first;
+    batch = b;
+    const uint32_t n = a.m.n_cells;
+    const double total = a.m.bv_prefix[n - 1];
+    const double target = ((double)i + a.shift[b]) / (double)count * total;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (target < a.m.bv_prefix[mid]) hi = mid;
+        else lo = mid + 1u;
+    }
+    cell = lo < n ? lo : n - 1u;
+    uint32_t o[4];
+    philox4x32_10(r, a.launch, 0u, 0u, a.seed, kKeyTag, o);
+    double s = (double)u32_to_unit(o[0]), t = (double)u32_to_unit(o[1]), v = (double)u32_to_unit(o[2]);
+    if (s + t > 1.0) {
+        s = 1.0 - s;
+        t = 1.0 - t;
+    }
+    if (t + v > 1.0) {
+        const double tmp = v;
+        v = 1.0 - s - t;
+        t = 1.0 - tmp;
+    } else if (s + t + v > 1.0) {
+        const double tmp = v;
+        v = s + t + v - 1.0;
+        s = 1.0 - t - tmp;
+    }
+    const double w0 = 1.0 - s - t - v;
+    const uint4 cv = a.m.verts[cell];
+    p[0] = w0 * a.m.px[cv.x] + s * a.m.px[cv.y] + t * a.m.px[cv.z] + v * a.m.px[cv.w];
+    p[1] = w0 * a.m.py[cv.x] + s * a.m.py[cv.y] + t * a.m.py[cv.z] + v * a.m.py[cv.w];
+    p[2] = w0 * a.m.pz[cv.x] + s * a.m.pz[cv.y] + t * a.m.pz[cv.z] + v * a.m.pz[cv.w];
+    u[0] = 0.0;
+    u[1] = 0.0;
+    u[2] = 1.0;
+    bool done = false;
+    for (uint32_t call = 0; call < kDirectionCalls && !done; ++call) {
+        philox4x32_10(r, a.launch, call + 1u, 0u, a.seed, kKeyTag, o);
+#pragma unroll
+        for (int pair = 0; pair < 2; ++pair) {
+            if (done) continue;
+            const double x1 = 2.0 * (double)u32_to_unit(o[2 * pair]) - 1.0;
+            const double x2 = 2.0 * (double)u32_to_unit(o[2 * pair + 1]) - 1.0;
+            const double q = x1 * x1 + x2 * x2;
+            if (q >= 1.0) continue;
+            const double rr = sqrt(1.0 - q);
+            u[0] = 2.0 * x1 * rr;
+            u[1] = 2.0 * x2 * rr;
+            u[2] = 1.0 - 2.0 * q;
+            done = true;
+        }
+    }
+    lambda = a.n_lambda ? (uint32_t)(((uint64_t)i * a.n_lambda / count + a.phase[b]) % a.n_lambda) : 0u;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceArgs a)
+{
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
+    if (r >= a.rays) return;
+    const TetDev& m = a.m;
+    uint32_t batch, cell, lambda;
+    double p[3], u[3];
+    ray_start(a, r, batch, cell, p, u, lambda);
+    const double sa = a.sigma_a[lambda], se = a.sigma_e[lambda];
+    double* scores = a.scores + (size_t)batch * 2u * m.n_points;
+    int forbidden = -1;
+    double gain = 1.0;
+    uint32_t segments = 0;
+    bool dropped = true;
+    for (uint32_t step = 0; step < kMaxSteps; ++step) {
+        const Hit h = next_face(m, cell, p, u, forbidden);
+        if (h.face < 0) {
+            // on a face with no exit ahead: leave through a face the ray
+            // already lies on (rayTransition.hpp:41-72, policyRay.hpp:394-425)
+            int face = -1;
+            bool near = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                near |= fabs(plane_at(m.planes[4 * (size_t)cell + k], p[0], p[1], p[2])) <= kTol;
+            if (near) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (face >= 0 || k == forbidden) continue;
+                    const double4 pl = m.planes[4 * (size_t)cell + k];
+                    if (fabs(plane_at(pl, p[0], p[1], p[2])) <= kTol && plane_dir(pl, u[0], u[1], u[2]) < 0.0)
+                        face = k;
+                }
+            }
+            if (face < 0) break;
+            const Cross t = recover(m, cell, face, p, u);
+            if (t.status == kFailed) {
+                cell = t.cell;
+                break;
+            }
+            if (t.status == kBoundary) {
+                dropped = false;
+                break;
+            }
+            cell = t.cell;
+            forbidden = t.forbidden;
+            continue;
+        }
+        // score the segment (accumulation.hpp:310-345)
+        const bool clad = m.clad[cell] != 0;
+        const double g = clad ? -m.clad_alpha : m.n_tot * (m.beta[cell] * (se + sa) - sa);
+        const double gl = g * h.length;
+        const double contribution = gain * (gl == 0.0 ? h.length : expm1(gl) / g);
+        if (isfinite(contribution)) {
+            atomicAdd(&a.visits[cell], 1u);
+            ++segments;
+            const double half = 0.5 * h.length;
+            const double mx = p[0] + u[0] * half, my = p[1] + u[1] * half, mz = p[2] + u[2] * half;
+            double w[4], sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                w[k] = fmax(0.0, plane_at(m.planes[4 * (size_t)cell + k], mx, my, mz));
+                sum += w[k];
+            }
+            const uint4 cv = m.verts[cell];
+            const uint32_t off = clad ? m.n_points : 0u;
+            const uint32_t vid[4] = {cv.x + off, cv.y + off, cv.z + off, cv.w + off};
+            if (sum <= DBL_EPSILON) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&scores[vid[k]], contribution * 0.25);
+            } else {
+                const double inv = 1.0 / sum;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&scores[vid[k]], contribution * (w[k] * inv));
+            }
+        } else {
+            atomicAdd(&a.dropped[cell], 1u);
+        }
+        gain *= exp(gl);
+        p[0] = p[0] + u[0] * h.length;
+        p[1] = p[1] + u[1] * h.length;
+        p[2] = p[2] + u[2] * h.length;
+        const Cross t = cross(m, cell, h, p, u);
+        if (t.status == kFailed) {
+            cell = t.cell;
+            break;
+        }
+        if (t.status == kBoundary) {
+            dropped = false;
+            break;
+        }
+        cell = t.cell;
+        forbidden = t.forbidden;
+    }
+    if (dropped) atomicAdd(&a.dropped[cell], 1u);
+    atomicAdd(a.segments, (unsigned long long)segments);
+}
+
+struct FinalizeArgs {
+    TetDev m;
+    const double* volume;
+    const double* lumped;
+    const double* scores;
+    const uint32_t* dropped;
+    uint32_t counts[kBatches];
+    double rays;
+    double total;
+    double rate;
+    double sigma_a, sigma_e;
+    double threshold;
+    uint32_t target;
+    float* phi;
+    double* se;
+    double* rse;
+    double* dndt;
+    uint32_t* convergence;
+    uint32_t* open_cells;
+};
+
+// FinalizeForwardVolumePhiAse (forwardPhiAseMapping.hpp:75-147) plus the
+// convergence record and stop count (types.hpp:356-396).
+__global__ void __launch_bounds__(256) hase_forward_finalize_kernel(FinalizeArgs a)
+{
+#pragma clang fp contract(off)
+    const uint32_t c = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
+    if (c >= a.m.n_cells) return;
+    const TetDev& m = a.m;
+    const uint4 cv = m.verts[c];
+    const uint32_t off = m.clad[c] ? m.n_points : 0u;
+    const uint32_t vid[4] = {cv.x + off, cv.y + off, cv.z + off, cv.w + off};
+    const double vol = a.volume[c];
+    const size_t nv = 2u * (size_t)m.n_points;
+    double rel = DBL_MAX, abs_err = DBL_MAX, estimate = 0.0;
+    if (a.rays > 0.0 && vol > 0.0) {
+        double score_sum = 0.0, mean_sum = 0.0, mean_sq = 0.0;
+        uint32_t active = 0;
+        for (uint32_t b = 0; b < kBatches; ++b) {
+            double dens = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double lv = a.lumped[vid[k]];
+                dens += lv > 0.0 ? a.scores[b * nv + vid[k]] / lv : 0.0;
+            }
+            dens /= 4.0;
+            const double bs = dens * vol;
+            score_sum += bs;
+            if (a.counts[b] == 0) continue;
+            const double mean = bs / (double)a.counts[b];
+            mean_sum += mean;
+            mean_sq += mean * mean;
+            ++active;
+        }
+        estimate = score_sum * a.total / (a.rays * vol);
+        if (a.dropped[c] == 0 && active >= 2) {
+            const double n = (double)active;
+            const double mean = mean_sum / n;
+            if (mean == 0.0) {
+                rel = __builtin_nan("");
+                abs_err = 0.0;
+            } else {
+                const double var = fmax(0.0, (mean_sq - mean_sum * mean_sum / n) / (n - 1.0));
+                rel = sqrt(var / n) / fabs(mean);
+                abs_err = rel * fabs(estimate) * a.rate;
+            }
+        }
+    }
+    const float f = (float)(estimate * a.rate);
+    a.phi[c] = f;
+    a.se[c] = abs_err;
+    a.rse[c] = rel;
+    a.dndt[c] = (m.beta[c] * (a.sigma_e + a.sigma_a) - a.sigma_a) * (double)f;
+    const bool ok = a.dropped[c] == 0 && isfinite(rel) && rel * rel <= a.threshold;
+    if (ok && a.convergence[c] == 0) a.convergence[c] = a.target;
+    if (!ok) atomicAdd(a.open_cells, 1u);
+}
+
+__global__ void hase_forward_clear_kernel(double* scores, size_t n_scores, uint32_t* counters, size_t n_counters)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n_scores) scores[i] = 0.0;
+    if (i < n_counters) counters[i] = 0u;
+}
+
+}  // namespace fwd
+}  // namespace hase
+
+struct hase_forward {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n_cells = 0, n_points = 0, n_lambda = 0;
+    uint32_t min_rays = 0, max_rays = 0, steps = 0, seed = 0;
+    double threshold = 0.0, max_sigma_a = 0.0, max_sigma_e = 0.0, n_tot = 0.0, tau = 0.0, clad_alpha = 0.0;
+    std::vector<uint32_t> verts;
+    std::vector<uint8_t> clad;
+    std::vector<double> volume;
+    std::vector<double> beta;
+    std::vector<double> prefix;
+    hase::DevBuf<double4> planes;
+    hase::DevBuf<int32_t> link;
+    hase::DevBuf<uint4> vdev;
+    hase::DevBuf<double> beta_dev, prefix_dev, px, py, pz, volume_dev, lumped, sigma_a, sigma_e, scores;
+    hase::DevBuf<uint8_t> clad_dev;
+    // counters: visits [C], dropped [C], convergence [C], open cells [1]
+    hase::DevBuf<uint32_t> counters;
+    hase::DevBuf<unsigned long long> segments;
+    hase::DevBuf<float> phi;
+    hase::DevBuf<double> se, rse, dndt;
+    hase_forward_stats_t stats{};
+
+    ~hase_forward()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    hase::fwd::TetDev view() const
+    {
+        return hase::fwd::TetDev{planes.p, link.p, vdev.p, beta_dev.p, clad_dev.p, prefix_dev.p, px.p, py.p, pz.p,
+                                 n_cells, n_points, n_tot, clad_alpha};
+    }
+};
+
+namespace hase {
+namespace fwd {
+
+using hase::HaseError;
+
+static void validate(const hase_tet_mesh_t* mesh, const hase_experiment_t* exp, const hase_compute_t* comp)
+{
+    if (!mesh || !exp || !comp) throw HaseError(HASE_E_INVALID, "mesh, experiment and compute must not be NULL");
+    if (!mesh->points || !mesh->cells || !mesh->beta_volume)
+        throw HaseError(HASE_E_INVALID, "points, cells and beta_volume are required");
+    if (mesh->number_of_cells == 0 || mesh->number_of_points < 4)
+        throw HaseError(HASE_E_INVALID, "a Tet4 mesh needs at least one cell and four points");
+    if (mesh->number_of_cells >= (1u << 29)) throw HaseError(HASE_E_INVALID, "number_of_cells must be < 2^29");
+    if (mesh->number_of_points >= (1u << 30)) throw HaseError(HASE_E_INVALID, "number_of_points must be < 2^30");
+    if (!(mesh->n_tot > 0.0) || !(mesh->crystal_t_fluo > 0.0) || !std::isfinite(mesh->n_tot) ||
+        !std::isfinite(mesh->crystal_t_fluo))
+        throw HaseError(HASE_E_INVALID, "n_tot and crystal_t_fluo must be positive");
+    if (!(mesh->cladding_absorption >= 0.0) || !std::isfinite(mesh->cladding_absorption))
+        throw HaseError(HASE_E_INVALID, "cladding_absorption must be finite and >= 0");
+    const size_t np = mesh->number_of_points;
+    for (size_t i = 0; i < 3 * np; ++i)
+        if (!std::isfinite(mesh->points[i])) throw HaseError(HASE_E_INVALID, "points must be finite");
+    for (size_t i = 0; i < 4 * (size_t)mesh->number_of_cells; ++i)
+        if (mesh->cells[i] >= mesh->number_of_points)
+            throw HaseError(HASE_E_INVALID, "cell " + std::to_string(i / 4) + " names a point out of range");
+    for (size_t c = 0; c < mesh->number_of_cells; ++c)
+        if (!std::isfinite(mesh->beta_volume[c])) throw HaseError(HASE_E_INVALID, "beta_volume must be finite");
+    if (exp->min_rays_per_sample == 0) throw HaseError(HASE_E_INVALID, "min rays must be > 0");
+    if (exp->max_rays_per_sample < exp->min_rays_per_sample)
+        throw HaseError(HASE_E_INVALID, "max rays must be >= min rays");
+    if (!(exp->mse_threshold >= 0.0)) throw HaseError(HASE_E_INVALID, "mse_threshold must be >= 0");
+    if (exp->number_of_wavelengths == 0 || !exp->sigma_a || !exp->sigma_e)
+        throw HaseError(HASE_E_INVALID, "at least one (sigma_a, sigma_e) pair is required");
+    for (uint32_t w = 0; w < exp->number_of_wavelengths; ++w)
+        if (!std::isfinite(exp->sigma_a[w]) || !std::isfinite(exp->sigma_e[w]) || exp->sigma_a[w] < 0.0 ||
+            exp->sigma_e[w] < 0.0)
+            throw HaseError(HASE_E_INVALID, "cross sections must be finite and >= 0");
+    if (exp->use_reflections)
+        throw HaseError(HASE_E_INVALID,
+                        "forward estimator: surface reflections (the surface reservoir) are not built; "
+                        "set use_reflections = 0");
+}
+
+// Face planes, neighbours, volumes and lumped vertex volumes on the host,
+// in the operation order of the oracle so the walks match bit for bit.
+static void prepare(hase_forward* f, const hase_tet_mesh_t* mesh)
+{
+    const uint32_t n = mesh->number_of_cells, np = mesh->number_of_points;
+    const double* px = mesh->points;
+    const double* py = mesh->points + np;
+    const double* pz = mesh->points + 2 * (size_t)np;
+    std::vector<double4> planes(4 * (size_t)n);
+    std::vector<int32_t> link(4 * (size_t)n, -1);
+    struct Key {
+        uint32_t v[3];
+        uint32_t slot;
+    };
+    std::vector<Key> keys(4 * (size_t)n);
+    std::vector<double> lumped(2 * (size_t)np, 0.0);
+    f->verts.assign(mesh->cells, mesh->cells + 4 * (size_t)n);
+    f->clad.assign(n, 0);
+    f->volume.assign(n, 0.0);
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint32_t* cv = mesh->cells + 4 * (size_t)c;
+        f->clad[c] = (mesh->cladding_cell_types && mesh->cladding_cell_types[c] == mesh->cladding_number) ? 1 : 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            uint32_t fv[3], j = 0;
+            for (uint32_t q = 0; q < 4; ++q)
+                if (q != k) fv[j++] = cv[q];
+            std::sort(fv, fv + 3);
+            const uint32_t o = cv[k];
+            const double ax = px[fv[0]], ay = py[fv[0]], az = pz[fv[0]];
+            const double bx = px[fv[1]] - ax, by = py[fv[1]] - ay, bz = pz[fv[1]] - az;
+            const double cx = px[fv[2]] - ax, cy = py[fv[2]] - ay, cz = pz[fv[2]] - az;
+            const double nx = by * cz - bz * cy, ny = bz * cx - bx * cz, nz = bx * cy - by * cx;
+            const double den = nx * (px[o] - ax) + ny * (py[o] - ay) + nz * (pz[o] - az);
+            if (std::fabs(den) <= DBL_EPSILON)
+                throw HaseError(HASE_E_INVALID, "cell " + std::to_string(c) + " is degenerate");
+            const double s = 1.0 / den;
+            double4 pl;
+            pl.x = nx * s;
+            pl.y = ny * s;
+            pl.z = nz * s;
+            pl.w = -(pl.x * ax + pl.y * ay + pl.z * az);
+            planes[4 * (size_t)c + k] = pl;
+            keys[4 * (size_t)c + k] = Key{{fv[0], fv[1], fv[2]}, 4 * c + k};
+        }
+        const double ax = px[cv[0]], ay = py[cv[0]], az = pz[cv[0]];
+        const double bx = px[cv[1]] - ax, by = py[cv[1]] - ay, bz = pz[cv[1]] - az;
+        const double cx = px[cv[2]] - ax, cy = py[cv[2]] - ay, cz = pz[cv[2]] - az;
+        const double dx = px[cv[3]] - ax, dy = py[cv[3]] - ay, dz = pz[cv[3]] - az;
+        const double vol =
+            std::fabs((by * cz - bz * cy) * dx + (bz * cx - bx * cz) * dy + (bx * cy - by * cx) * dz) / 6.0;
+        f->volume[c] = vol;
+        const uint32_t off = f->clad[c] ? np : 0u;
+        for (uint32_t k = 0; k < 4; ++k) lumped[off + cv[k]] += vol / 4.0;
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+        for (int i = 0; i < 3; ++i)
+            if (a.v[i] != b.v[i]) return a.v[i] < b.v[i];
+        return a.slot < b.slot;
+    });
+    auto same = [](const Key& a, const Key& b) { return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2]; };
+    for (size_t i = 0; i + 1 < keys.size(); ++i) {
+        if (!same(keys[i], keys[i + 1])) continue;
+        if (i + 2 < keys.size() && same(keys[i], keys[i + 2]))
+            throw HaseError(HASE_E_INVALID, "a face is shared by more than two cells (cell " +
+                                                std::to_string(keys[i].slot / 4) + ")");
+        const uint32_t a = keys[i].slot, b = keys[i + 1].slot;
+        link[a] = (int32_t)((b / 4) * 4 + b % 4);
+        link[b] = (int32_t)((a / 4) * 4 + a % 4);
+        ++i;
+    }
+    f->planes.upload(planes.data(), planes.size(), f->stream);
+    f->link.upload(link.data(), link.size(), f->stream);
+    f->vdev.upload(reinterpret_cast<const uint4*>(f->verts.data()), n, f->stream);
+    f->clad_dev.upload(f->clad.data(), n, f->stream);
+    f->px.upload(px, np, f->stream);
+    f->py.upload(py, np, f->stream);
+    f->pz.upload(pz, np, f->stream);
+    f->lumped.upload(lumped.data(), lumped.size(), f->stream);
+    f->volume_dev.upload(f->volume.data(), n, f->stream);
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+}
+
+static void set_beta(hase_forward* f, const double* beta)
+{
+    f->beta.assign(beta, beta + f->n_cells);
+    f->prefix.resize(f->n_cells);
+    double run = 0.0;
+    for (uint32_t c = 0; c < f->n_cells; ++c) {
+        run += f->beta[c] * f->volume[c];
+        f->prefix[c] = run;
+    }
+    f->beta_dev.upload(f->beta.data(), f->n_cells, f->stream);
+    f->prefix_dev.upload(f->prefix.data(), f->n_cells, f->stream);
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+}
+
+static std::unique_ptr<hase_forward> create(const hase_tet_mesh_t* mesh, const hase_experiment_t* exp,
+                                            const hase_compute_t* comp)
+{
+    validate(mesh, exp, comp);
+    auto f = std::make_unique<hase_forward>();
+    f->device = comp->device;
+    HIP_CHECK(hipSetDevice(f->device));
+    HIP_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+    f->n_cells = mesh->number_of_cells;
+    f->n_points = mesh->number_of_points;
+    f->n_lambda = exp->number_of_wavelengths;
+    f->min_rays = exp->min_rays_per_sample;
+    f->max_rays = exp->max_rays_per_sample;
+    f->steps = comp->adaptive_steps;
+    f->seed = comp->rng_seed;
+    f->threshold = exp->mse_threshold;
+    f->max_sigma_a = exp->max_sigma_a;
+    f->max_sigma_e = exp->max_sigma_e;
+    f->n_tot = mesh->n_tot;
+    f->tau = mesh->crystal_t_fluo;
+    f->clad_alpha = mesh->cladding_absorption;
+    prepare(f.get(), mesh);
+    set_beta(f.get(), mesh->beta_volume);
+    f->sigma_a.upload(exp->sigma_a, f->n_lambda, f->stream);
+    f->sigma_e.upload(exp->sigma_e, f->n_lambda, f->stream);
+    f->scores.alloc((size_t)kBatches * 2u * f->n_points);
+    f->counters.alloc(3 * (size_t)f->n_cells + 1);
+    f->segments.alloc(1);
+    f->phi.alloc(f->n_cells);
+    f->se.alloc(f->n_cells);
+    f->rse.alloc(f->n_cells);
+    f->dndt.alloc(f->n_cells);
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+    return f;
+}
+
+static void clear(hase_forward* f)
+{
+    const size_t ns = (size_t)kBatches * 2u * f->n_points, nc = 3 * (size_t)f->n_cells + 1;
+    const size_t n = std::max(ns, nc);
+    hipLaunchKernelGGL(hase_forward_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, f->stream,
+                       f->scores.p, ns, f->counters.p, nc);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemsetAsync(f->segments.p, 0, sizeof(unsigned long long), f->stream));
+}
+
+static TraceArgs trace_args(hase_forward* f, uint32_t seed, uint32_t launch, uint32_t rays)
+{
+    TraceArgs a{};
+    a.m = f->view();
+    a.sigma_a = f->sigma_a.p;
+    a.sigma_e = f->sigma_e.p;
+    a.n_lambda = f->n_lambda;
+    a.seed = seed;
+    a.launch = launch;
+    a.rays = rays;
+    for (uint32_t b = 0; b < kBatches; ++b) {
+        uint32_t o[4];
+        philox4x32_10(0xFFFFFFFFu, b, launch, 2u, seed, kKeyTag, o);
+        a.shift[b] = (double)u32_to_unit(o[0]);
+        a.phase[b] = o[1] % f->n_lambda;
+    }
+    a.scores = f->scores.p;
+    a.visits = f->counters.p;
+    a.dropped = f->counters.p + f->n_cells;
+    a.segments = f->segments.p;
+    return a;
+}
+
+// One launch of `rays` rays; returns its device time in ms.
+static double launch_trace(hase_forward* f, uint32_t seed, uint32_t launch, uint32_t rays)
+{
+    if (rays == 0 || !(f->prefix.back() > 0.0)) return 0.0;
+    const TraceArgs a = trace_args(f, seed, launch, rays);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, f->stream));
+    const unsigned blocks = (unsigned)(((uint64_t)rays + kTraceBlock - 1) / kTraceBlock);
+    hipLaunchKernelGGL(hase_forward_trace_kernel, dim3(blocks), dim3(kTraceBlock), 0, f->stream, a);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(e1, f->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+static uint32_t adaptive_target(uint32_t min_rays, uint32_t max_rays, uint32_t steps, uint32_t k)
+{
+    if (max_rays <= min_rays || steps == 0) return min_rays;
+    if (k >= steps) return max_rays;
+    const double growth = std::pow((double)max_rays / (double)min_rays, 1.0 / (double)steps);
+    double r = std::ceil((double)min_rays * std::pow(growth, (double)k));
+    r = std::min(std::max(r, (double)min_rays), (double)max_rays);
+    return (uint32_t)r;
+}
+
+static void evaluate(hase_forward* f, hase_forward_result_t* out)
+{
+    if (!out || !out->phi_ase) throw HaseError(HASE_E_INVALID, "result.phi_ase must not be NULL");
+    HIP_CHECK(hipSetDevice(f->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    clear(f);
+    f->stats = hase_forward_stats_t{};
+    uint32_t counts[kBatches] = {0};
+    uint64_t done = 0;
+    uint32_t launches = 0, open = 0;
+    for (uint32_t k = 0;; ++k) {
+        const uint32_t target = adaptive_target(f->min_rays, f->max_rays, f->steps, k);
+        const uint32_t rays = (uint32_t)(target - done);
+        for (uint32_t b = 0; b < kBatches; ++b) counts[b] += rays / kBatches + (b < rays % kBatches ? 1u : 0u);
+        f->stats.trace_ms += launch_trace(f, f->seed, launches, rays);
+        done = target;
+        ++launches;
+        FinalizeArgs fa{};
+        fa.m = f->view();
+        fa.volume = f->volume_dev.p;
+        fa.lumped = f->lumped.p;
+        fa.scores = f->scores.p;
+        fa.dropped = f->counters.p + f->n_cells;
+        for (uint32_t b = 0; b < kBatches; ++b) fa.counts[b] = counts[b];
+        fa.rays = (double)done;
+        fa.total = f->prefix.back();
+        fa.rate = f->n_tot / f->tau;
+        fa.sigma_a = f->max_sigma_a;
+        fa.sigma_e = f->max_sigma_e;
+        fa.threshold = f->threshold;
+        fa.target = target;
+        fa.phi = f->phi.p;
+        fa.se = f->se.p;
+        fa.rse = f->rse.p;
+        fa.dndt = f->dndt.p;
+        fa.convergence = f->counters.p + 2 * (size_t)f->n_cells;
+        fa.open_cells = f->counters.p + 3 * (size_t)f->n_cells;
+        HIP_CHECK(hipMemsetAsync(fa.open_cells, 0, sizeof(uint32_t), f->stream));
+        hipLaunchKernelGGL(hase_forward_finalize_kernel, dim3((f->n_cells + 255) / 256), dim3(256), 0, f->stream,
+                           fa);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(&open, fa.open_cells, sizeof(uint32_t), hipMemcpyDeviceToHost, f->stream));
+        HIP_CHECK(hipStreamSynchronize(f->stream));
+        if (open == 0 || target >= f->max_rays || f->steps == 0 || f->max_rays <= f->min_rays) break;
+    }
+    const size_t n = f->n_cells;
+    HIP_CHECK(hipMemcpyAsync(out->phi_ase, f->phi.p, n * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+    if (out->standard_error)
+        HIP_CHECK(hipMemcpyAsync(out->standard_error, f->se.p, n * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    if (out->relative_standard_error)
+        HIP_CHECK(hipMemcpyAsync(out->relative_standard_error, f->rse.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 f->stream));
+    if (out->dndt_ase)
+        HIP_CHECK(hipMemcpyAsync(out->dndt_ase, f->dndt.p, n * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    if (out->ray_visits)
+        HIP_CHECK(hipMemcpyAsync(out->ray_visits, f->counters.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 f->stream));
+    if (out->dropped_rays)
+        HIP_CHECK(hipMemcpyAsync(out->dropped_rays, f->counters.p + n, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 f->stream));
+    if (out->convergence_rays)
+        HIP_CHECK(hipMemcpyAsync(out->convergence_rays, f->counters.p + 2 * n, n * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, f->stream));
+    unsigned long long seg = 0;
+    std::vector<uint32_t> dropped(n);
+    HIP_CHECK(hipMemcpyAsync(&seg, f->segments.p, sizeof(seg), hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipMemcpyAsync(dropped.data(), f->counters.p + n, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             f->stream));
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+    f->stats.rays = done;
+    f->stats.segments = seg;
+    f->stats.dropped_rays = std::accumulate(dropped.begin(), dropped.end(), (uint64_t)0);
+    f->stats.launches = launches;
+    f->stats.open_cells = open;
+    f->stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace fwd
+}  // namespace hase
+
+extern "C" {
+
+int hase_forward_create(const hase_tet_mesh_t* mesh, const hase_experiment_t* experiment,
+                        const hase_compute_t* compute, hase_forward_t** out)
+{
+    return hase::guarded([&] {
+        if (!out) throw hase::HaseError(HASE_E_INVALID, "out must not be NULL");
+        *out = nullptr;
+        *out = hase::fwd::create(mesh, experiment, compute).release();
+    });
+}
+
+int hase_forward_update_beta(hase_forward_t* f, const double* beta_volume)
+{
+    return hase::guarded([&] {
+        if (!f || !beta_volume) throw hase::HaseError(HASE_E_INVALID, "forward and beta_volume must not be NULL");
+        for (uint32_t c = 0; c < f->n_cells; ++c)
+            if (!std::isfinite(beta_volume[c])) throw hase::HaseError(HASE_E_INVALID, "beta_volume must be finite");
+        HIP_CHECK(hipSetDevice(f->device));
+        hase::fwd::set_beta(f, beta_volume);
+    });
+}
+
+int hase_forward_evaluate(hase_forward_t* f, hase_forward_result_t* result)
+{
+    return hase::guarded([&] {
+        if (!f) throw hase::HaseError(HASE_E_INVALID, "forward must not be NULL");
+        hase::fwd::evaluate(f, result);
+    });
+}
+
+int hase_forward_trace(hase_forward_t* f, uint32_t seed, uint32_t launch, uint32_t rays, double* scores,
+                       uint32_t* ray_visits, uint32_t* dropped_rays, uint64_t* segments)
+{
+    return hase::guarded([&] {
+        if (!f || !scores || !ray_visits || !dropped_rays)
+            throw hase::HaseError(HASE_E_INVALID, "forward, scores, ray_visits and dropped_rays must not be NULL");
+        HIP_CHECK(hipSetDevice(f->device));
+        hase::fwd::clear(f);
+        f->stats.trace_ms = hase::fwd::launch_trace(f, seed, launch, rays);
+        const size_t n = f->n_cells;
+        HIP_CHECK(hipMemcpyAsync(scores, f->scores.p, (size_t)hase::fwd::kBatches * 2u * f->n_points * sizeof(double),
+                                 hipMemcpyDeviceToHost, f->stream));
+        HIP_CHECK(hipMemcpyAsync(ray_visits, f->counters.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, f->stream));
+        HIP_CHECK(hipMemcpyAsync(dropped_rays, f->counters.p + n, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 f->stream));
+        unsigned long long seg = 0;
+        HIP_CHECK(hipMemcpyAsync(&seg, f->segments.p, sizeof(seg), hipMemcpyDeviceToHost, f->stream));
+        HIP_CHECK(hipStreamSynchronize(f->stream));
+        if (segments) *segments = seg;
+    });
+}
+
+int hase_forward_stats(const hase_forward_t* f, hase_forward_stats_t* stats)
+{
+    return hase::guarded([&] {
+        if (!f || !stats) throw hase::HaseError(HASE_E_INVALID, "forward and stats must not be NULL");
+        *stats = f->stats;
+    });
+}
+
+int hase_forward_destroy(hase_forward_t* f)
+{
+    return hase::guarded([&] {
+        if (!f) return;
+        (void)hipSetDevice(f->device);
+        (void)hipStreamSynchronize(f->stream);
+        delete f;
+    });
+}
+
+int hase_calc_forward_phi_ase(const hase_tet_mesh_t* mesh, const hase_experiment_t* experiment,
+                              const hase_compute_t* compute, hase_forward_result_t* result)
+{
+    return hase::guarded([&] {
+        auto f = hase::fwd::create(mesh, experiment, compute);
+        hase::fwd::evaluate(f.get(), result);
+    });
+}
+
+}  // extern "C"

@@ -12,6 +12,7 @@ its C ABI in-process.
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -49,6 +50,7 @@ _CONFIG_KEYS = {
     "rngSeed": "rngSeed", "rng_seed": "rngSeed", "minSampleRange": "minSampleRange",
     "maxSampleRange": "maxSampleRange", "device": "device", "defensiveFraction": "defensiveFraction",
     "raysPerThread": "raysPerThread", "sampleBatch": "sampleBatch",
+    "writeVtk": "writeVtk", "write_vtk": "writeVtk", "outputPath": "outputPath", "output_path": "outputPath",
 }
 
 
@@ -75,6 +77,8 @@ class PhiASE:
     sampleBatch: int = 0
     proposal: object | None = None   # a proposal.LearnedProposal; None = walked importance map
     ase_steps: int | None = None     # time-stepped runs: ASE in steps < ase_steps (None counts as 0)
+    writeVtk: bool = False           # run(): per-sample results as VTK files under <outputPath>/vtk
+    outputPath: object = "."
 
     _result: Result | None = field(default=None, init=False, repr=False)
 
@@ -166,7 +170,9 @@ class PhiASE:
         with _native.Context(mesh, exp, comp) as ctx:
             if self.proposal is not None:
                 ctx.set_proposal(self.proposal)
+            t0 = time.perf_counter()
             res = ctx.evaluate()
+            runtime = time.perf_counter() - t0
             stats = ctx.stats()
             if self.proposal is not None:
                 stats["proposal"] = ctx.proposal_stats()
@@ -174,7 +180,22 @@ class PhiASE:
         self._result = Result(phiAse=res.phiAse, mse=res.mse, totalRays=res.totalRays, dndtAse=res.dndtAse,
                               standardError=res.standardError, relativeStandardError=rse,
                               droppedRays=res.droppedRays, stats=stats)
+        if self.writeVtk:
+            self._writeVtk(gainMedium if gainMedium is not None else self.gainMedium, int(stats["rays"]), runtime)
         return self._result
+
+    def _writeVtk(self, medium, rays, runtime):
+        """Per-sample results as POINT_DATA files <outputPath>/vtk/<name>_<unix time>.vtk
+        (include/core/simulation.hpp:161-224)."""
+        from .vtk_wedge import writePointsToVtk
+
+        r = self._result
+        folder = Path(self.outputPath) / "vtk"
+        stamp = str(int(time.time()))
+        for name, values in (("dndt", r.dndtAse), ("phiase", r.phiAse), ("standard_error", r.standardError),
+                             ("total_rays", r.totalRays), ("relative_standard_error", r.relativeStandardError)):
+            writePointsToVtk(medium, values, folder / f"{name}_{stamp}.vtk", rays, int(self.maxRaysPerSample),
+                             float(self.mseThreshold), bool(self.useReflections), runtime)
 
     def getResults(self):
         if self._result is None:

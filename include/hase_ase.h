@@ -432,6 +432,88 @@ int hase_simulation_times(const hase_simulation_t* sim, double* ase_ms, double* 
 
 void hase_simulation_destroy(hase_simulation_t* sim);
 
+/* ---------------------------------------------------------------------
+ * Forward Tet4 estimator: the reference's present method
+ * (include/core/calcForwardPhiAse.hpp:60-300, ForwardPhiAseContext), without
+ * surface reflections.  Rays start at beta*V-stratified points of a
+ * tetrahedral gain medium, travel in isotropic directions until they leave
+ * it, and score the track-length integral of their gain into the vertices of
+ * every traversed cell; a cell's phiAse is the mean lumped-vertex density of
+ * its four vertices (gain and cladding cells keep separate vertex copies),
+ * its relative standard error the spread of eight batch means
+ * (include/kernels/forwardPhiAseMapping.hpp:47-147).
+ *
+ * Of hase_experiment_t it reads min/max_rays_per_sample (rays of the WHOLE
+ * evaluation, grown geometrically over compute.adaptive_steps as in
+ * include/core/types.hpp:333-354), the wavelength tables (each ray takes a
+ * stratified wavelength), max_sigma_a/e (dndtAse) and mse_threshold (stop
+ * when every cell has relative_standard_error^2 <= it); use_reflections must
+ * be 0.  Of hase_compute_t: adaptive_steps, rng_seed, device. */
+typedef struct hase_tet_mesh {
+    uint32_t number_of_points;
+    uint32_t number_of_cells;
+    const double* points;           /* [3*Np] SoA: x then y then z (mesh.hpp:182-187) */
+    const uint32_t* cells;          /* [4C] point indices, cell-major (cellPointIndices) */
+    const double* beta_volume;      /* [C] excited-state fraction per cell */
+    const uint32_t* cladding_cell_types; /* [C] or NULL (claddingCellTypes) */
+    uint32_t cladding_number;       /* cells whose type equals this absorb */
+    uint32_t reserved;
+    double cladding_absorption;     /* cm^-1 */
+    double n_tot;                   /* cm^-3 */
+    double crystal_t_fluo;          /* s */
+} hase_tet_mesh_t;
+
+/* Per-cell outputs, caller-owned, C entries each; all but phi_ase may be NULL.
+ * Mirrors the forward Result (phiAse, standardError, relativeStandardError,
+ * dndtAse; include/core/types.hpp:201-246) plus the walk counters
+ * (cellRayVisits, cellDroppedRays; accumulation.hpp:110-117). */
+typedef struct hase_forward_result {
+    float* phi_ase;
+    double* standard_error;
+    double* relative_standard_error;  /* DBL_MAX where undetermined, NaN where the flux is zero */
+    double* dndt_ase;
+    uint32_t* ray_visits;             /* scored segments in the cell */
+    uint32_t* dropped_rays;           /* rays whose walk failed in the cell, non-finite scores */
+    uint32_t* convergence_rays;       /* ray target at which the cell first met the threshold, 0 = never */
+} hase_forward_result_t;
+
+typedef struct hase_forward_stats {
+    double trace_ms;                /* summed device time of the trace kernel */
+    double total_ms;                /* wall time of the evaluation */
+    uint64_t rays;
+    uint64_t segments;              /* scored cell segments */
+    uint64_t dropped_rays;
+    uint32_t launches;              /* adaptive launches */
+    uint32_t open_cells;            /* cells short of the threshold at the end */
+} hase_forward_stats_t;
+
+typedef struct hase_forward hase_forward_t;
+
+/* Prepares a Tet4 mesh (face planes, neighbours by shared faces, volumes,
+ * lumped vertex volumes) on the device.  Degenerate cells and faces shared by
+ * more than two cells are rejected. */
+int hase_forward_create(const hase_tet_mesh_t* mesh, const hase_experiment_t* experiment,
+                        const hase_compute_t* compute, hase_forward_t** out);
+
+/* New beta per cell (the beta*V source prefix is rebuilt). */
+int hase_forward_update_beta(hase_forward_t* forward, const double* beta_volume);
+
+/* Adaptive evaluation into `result`. */
+int hase_forward_evaluate(hase_forward_t* forward, hase_forward_result_t* result);
+
+/* One launch of `rays` rays with the given seed and launch index, without
+ * finalisation: scores [8 * 2 * Np] (batch-major; gain vertex copies then
+ * cladding copies), visits and dropped [C], scored segments. */
+int hase_forward_trace(hase_forward_t* forward, uint32_t seed, uint32_t launch, uint32_t rays, double* scores,
+                       uint32_t* ray_visits, uint32_t* dropped_rays, uint64_t* segments);
+
+int hase_forward_stats(const hase_forward_t* forward, hase_forward_stats_t* stats);
+int hase_forward_destroy(hase_forward_t* forward);
+
+/* One-shot create + evaluate + destroy (calcForwardPhiAse). */
+int hase_calc_forward_phi_ase(const hase_tet_mesh_t* mesh, const hase_experiment_t* experiment,
+                              const hase_compute_t* compute, hase_forward_result_t* result);
+
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);
 

@@ -10,6 +10,9 @@ ray that reaches a face while still inside the lateral walls continues
 through the mirrored box, weighted by the surface factor of that face
 (reflectivity, or 1 under total internal reflection for
 (n_in/n_out)^2 sin^2(theta) > 1, include/kernels/reflection.hpp:33-45).
+
+``hat_smoothed_cell_flux`` turns such a point flux into the expectation of
+the forward estimator's cell value (vertex hat-function averages).
 """
 from __future__ import annotations
 
@@ -63,3 +66,44 @@ def box_point_flux(p, box, Z, beta, g, scale, *, reflect=False, refl_bottom=0.0,
         total = total + refl
     w = (2.0 / n_mu) * (2.0 * np.pi / n_phi)
     return scale * beta * float(np.sum(total)) * w / (4.0 * np.pi)
+
+
+# Stroud's degree-3 rule on the tetrahedron (barycentric points, weights
+# summing to 1): exact for the product of a quadratic and a linear function.
+_TET3_POINTS = np.array([[0.25, 0.25, 0.25, 0.25], [0.5, 1 / 6, 1 / 6, 1 / 6], [1 / 6, 0.5, 1 / 6, 1 / 6],
+                         [1 / 6, 1 / 6, 0.5, 1 / 6], [1 / 6, 1 / 6, 1 / 6, 0.5]])
+_TET3_WEIGHTS = np.array([-0.8, 0.45, 0.45, 0.45, 0.45])
+
+
+def hat_smoothed_cell_flux(points, cells, volumes, targets, flux):
+    """Expected value of the forward estimator's cell flux for each target
+    cell of a gain-only Tet4 mesh: the mean over the cell's four vertices of
+    the hat-function average of the point flux,
+        D_v = integral phi(x) lambda_v(x) dx / integral lambda_v(x) dx,
+    over the cells around v (the vertex deposit weights the track by the
+    barycentric coordinate of its midpoint, which is exact in expectation for
+    a linear lambda and zero gain).  ``flux(p)`` gives the point flux."""
+    patch = {}
+    for c, cv in enumerate(cells):
+        for k, v in enumerate(cv):
+            patch.setdefault(int(v), []).append((c, k))
+    cache = {}
+
+    def tet_values(c):
+        if c not in cache:
+            x = _TET3_POINTS @ points[cells[c]]
+            cache[c] = np.array([flux(p) for p in x])
+        return cache[c]
+
+    out = []
+    for c in targets:
+        dens = []
+        for v in cells[c]:
+            num = den = 0.0
+            for t, k in patch[int(v)]:
+                lam = _TET3_POINTS[:, k]
+                num += volumes[t] * np.sum(_TET3_WEIGHTS * lam * tet_values(t))
+                den += volumes[t] * 0.25
+            dens.append(num / den)
+        out.append(np.mean(dens))
+    return np.array(out)

@@ -38,7 +38,9 @@ def test_struct_layouts_match_header(tmp_path):
                "hase_compute_t": _native.Compute, "hase_result_t": _native.ResultC, "hase_stats_t": _native.Stats,
                "hase_train_t": _native.Train, "hase_proposal_t": _native.Proposal,
                "hase_proposal_stats_t": _native.ProposalStats, "hase_tuning_t": _native.Tuning,
-               "hase_run_control_t": _native.RunControl, "hase_time_state_t": _native.TimeState}
+               "hase_run_control_t": _native.RunControl, "hase_time_state_t": _native.TimeState,
+               "hase_tet_mesh_t": _native.TetMeshC, "hase_forward_result_t": _native.ForwardResultC,
+               "hase_forward_stats_t": _native.ForwardStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():
@@ -195,3 +197,27 @@ def test_time_loop_front_end_validation():
         LocalPump(rate=np.zeros(3)).arrays(w.topology.numberOfPrisms)
     with pytest.raises(TypeError):
         PhiASE(crossSections=w.spectra, ase_steps=1.5)
+
+
+def test_forward_host_rejections():
+    """The forward entry points validate on the host before touching a device."""
+    import numpy as np
+
+    from haseongpu_amd.tet_mesh import TetMesh
+
+    pts = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    mesh = TetMesh(points=pts, cells=[[0, 1, 2, 3]], beta=[0.1])
+    comp = _native.make_compute(n_samples=1)
+    refl = _native.NativeExperiment(sigma_a=[1e-21], sigma_e=[2e-20], min_rays=10, max_rays=10, use_reflections=True)
+    with pytest.raises(_native.HaseError, match="surface reservoir"):
+        _native.ForwardContext(mesh, refl, comp)
+    ok = _native.NativeExperiment(sigma_a=[1e-21], sigma_e=[2e-20], min_rays=10, max_rays=10)
+    bad = _native.NativeTetMesh(mesh)
+    bad.cells[3] = 9
+    with pytest.raises(_native.HaseError, match="out of range"):
+        _native.ForwardContext(bad, ok, comp)
+    zero = _native.NativeExperiment(sigma_a=[1e-21], sigma_e=[2e-20], min_rays=0, max_rays=10)
+    with pytest.raises(_native.HaseError, match="min rays"):
+        _native.ForwardContext(mesh, zero, comp)
+    with pytest.raises(ValueError):
+        TetMesh(points=pts, cells=[[0, 1, 2, 4]], beta=[0.1])
