@@ -20,9 +20,12 @@
 // beta*V prefix f64[C], lumped vertex volumes f64[2 Np], scores f64[8 x 2 Np]
 // (eight RSE batches x gain | cladding vertex copies), visits and dropped
 // u32[C].  One thread walks one ray in FP64 with contraction off, so a ray
-// takes exactly the cells the FP64 oracle takes; vertex scores are FP64
-// hardware atomics (their order, and exp/expm1 to the last bit, differ from
-// the oracle's).
+// takes exactly the cells the FP64 oracle takes.  Vertex scores are FP64
+// hardware atomics, which bound the kernel; since consecutive cells of a walk
+// share three vertices, a ray keeps the current cell's four vertex shares in
+// registers and writes a share only when the walk leaves its vertex (about
+// one atomic per segment instead of four).  Summation order, and exp/expm1 in
+// their last bits, differ from the oracle's.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -263,6 +266,9 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
     double gain = 1.0;
     uint32_t segments = 0;
     bool dropped = true;
+    // vertex shares of the last scored cell, not yet added to `scores`
+    uint32_t pend_v[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    double pend_s[4] = {0.0, 0.0, 0.0, 0.0};
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
         const Hit h = next_face(m, cell, p, u, forbidden);
         if (h.face < 0) {
@@ -315,13 +321,26 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
             const uint4 cv = m.verts[cell];
             const uint32_t off = clad ? m.n_points : 0u;
             const uint32_t vid[4] = {cv.x + off, cv.y + off, cv.z + off, cv.w + off};
-            if (sum <= DBL_EPSILON) {
+            // consecutive cells share a face: carry the pending shares of the
+            // vertices this cell keeps, write out the ones it drops
+            double carried[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&scores[vid[k]], contribution * 0.25);
-            } else {
-                const double inv = 1.0 / sum;
+            for (int k = 0; k < 4; ++k) {
+                bool kept = false;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) unsafeAtomicAdd(&scores[vid[k]], contribution * (w[k] * inv));
+                for (int j = 0; j < 4; ++j)
+                    if (pend_v[k] == vid[j]) {
+                        carried[j] = pend_s[k];
+                        kept = true;
+                    }
+                if (!kept && pend_s[k] != 0.0) unsafeAtomicAdd(&scores[pend_v[k]], pend_s[k]);
+            }
+            const bool flat = sum <= DBL_EPSILON;
+            const double inv = flat ? 0.0 : 1.0 / sum;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                pend_v[k] = vid[k];
+                pend_s[k] = carried[k] + contribution * (flat ? 0.25 : w[k] * inv);
             }
         } else {
             atomicAdd(&a.dropped[cell], 1u);
@@ -342,6 +361,9 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
         cell = t.cell;
         forbidden = t.forbidden;
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (pend_s[k] != 0.0) unsafeAtomicAdd(&scores[pend_v[k]], pend_s[k]);
     if (dropped) atomicAdd(&a.dropped[cell], 1u);
     atomicAdd(a.segments, (unsigned long long)segments);
 }
