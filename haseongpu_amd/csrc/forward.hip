@@ -15,8 +15,8 @@
 // (include/core/forwardPhiAseEvaluator.hpp:114-178, types.hpp:333-396).
 //
 // Layout in HBM (C cells, Np points): face planes 4 x double4 per cell
-// (128 B), neighbour links int32[4C] (neighbour * 4 + its local face, -1 at
-// the boundary), vertex ids uint4[C], beta f64[C], cladding flags u8[C],
+// (128 B, one cache line), a 64-B cell record (neighbour links: neighbour * 4
+// + its local face, -1 at the boundary; vertex ids; beta; cladding flag),
 // beta*V prefix f64[C], lumped vertex volumes f64[2 Np], scores f64[8 x 2 Np]
 // (eight RSE batches x gain | cladding vertex copies), visits and dropped
 // u32[C].  One thread walks one ray in FP64 with contraction off, so a ray
@@ -50,12 +50,20 @@ constexpr int kTraceBlock = 256;
 
 enum { kEntered = 0, kBoundary = 1, kFailed = 2 };
 
+// Everything but the face planes that a walk step reads of its cell, in one
+// 64-byte record: a step touches two cache lines (planes, record).
+struct CellAux {
+    int32_t link[4];   // neighbour * 4 + its local face, -1 at the boundary
+    uint32_t verts[4]; // point ids of the 4 vertices
+    double beta;
+    uint32_t clad;
+    uint32_t pad[5];
+};
+static_assert(sizeof(CellAux) == 64, "cell record is one half cache line");
+
 struct TetDev {
     const double4* planes;    // 4 per cell
-    const int32_t* link;      // 4 per cell
-    const uint4* verts;       // point ids of the 4 vertices
-    const double* beta;
-    const uint8_t* clad;
+    const CellAux* aux;
     const double* bv_prefix;
     const double* px;
     const double* py;
@@ -141,7 +149,7 @@ __device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const do
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t recent0 = cell, recent1 = kNone, recent2 = kNone;
     for (uint32_t step = 0; step < kMaxRecovery; ++step) {
-        const int32_t link = m.link[4 * (size_t)cell + exit_face];
+        const int32_t link = m.aux[cell].link[exit_face];
         if (link < 0) {
             r.cell = cell;
             r.status = kBoundary;
@@ -163,7 +171,7 @@ __device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const do
             const double d = plane_dir(pl, u[0], u[1], u[2]);
             if (c + kProbe * d >= 0.0) continue;
             if (fallback < 0) fallback = k;
-            const int32_t cl = m.link[4 * (size_t)cell + k];
+            const int32_t cl = m.aux[cell].link[k];
             if (cl < 0) continue;
             const uint32_t cn = (uint32_t)cl >> 2;
             if (cn != recent0 && cn != recent1 && cn != recent2) preferred = k;
@@ -178,7 +186,7 @@ __device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const do
 __device__ __forceinline__ Cross cross(const TetDev& m, uint32_t cell, Hit h, const double p[3], const double u[3])
 {
     if (h.tied & (h.tied - 1u)) return recover(m, cell, h.face, p, u);
-    const int32_t link = m.link[4 * (size_t)cell + h.face];
+    const int32_t link = m.aux[cell].link[h.face];
     if (link < 0) return Cross{cell, -1, kBoundary};
     return Cross{(uint32_t)link >> 2, link & 3, kEntered};
 }
@@ -224,10 +232,10 @@ __device__ __forceinline__ void ray_start(const TraceArgs& a, uint32_t r, uint32
         s = 1.0 - t - tmp;
     }
     const double w0 = 1.0 - s - t - v;
-    const uint4 cv = a.m.verts[cell];
-    p[0] = w0 * a.m.px[cv.x] + s * a.m.px[cv.y] + t * a.m.px[cv.z] + v * a.m.px[cv.w];
-    p[1] = w0 * a.m.py[cv.x] + s * a.m.py[cv.y] + t * a.m.py[cv.z] + v * a.m.py[cv.w];
-    p[2] = w0 * a.m.pz[cv.x] + s * a.m.pz[cv.y] + t * a.m.pz[cv.z] + v * a.m.pz[cv.w];
+    const uint32_t* cv = a.m.aux[cell].verts;
+    p[0] = w0 * a.m.px[cv[0]] + s * a.m.px[cv[1]] + t * a.m.px[cv[2]] + v * a.m.px[cv[3]];
+    p[1] = w0 * a.m.py[cv[0]] + s * a.m.py[cv[1]] + t * a.m.py[cv[2]] + v * a.m.py[cv[3]];
+    p[2] = w0 * a.m.pz[cv[0]] + s * a.m.pz[cv[1]] + t * a.m.pz[cv[2]] + v * a.m.pz[cv[3]];
     u[0] = 0.0;
     u[1] = 0.0;
     u[2] = 1.0;
@@ -303,10 +311,13 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
             continue;
         }
         // score the segment (accumulation.hpp:310-345)
-        const bool clad = m.clad[cell] != 0;
-        const double g = clad ? -m.clad_alpha : m.n_tot * (m.beta[cell] * (se + sa) - sa);
+        const CellAux& ax = m.aux[cell];
+        const bool clad = ax.clad != 0;
+        const double g = clad ? -m.clad_alpha : m.n_tot * (ax.beta * (se + sa) - sa);
         const double gl = g * h.length;
-        const double contribution = gain * (gl == 0.0 ? h.length : expm1(gl) / g);
+        // one expm1 gives the track integral and the gain factor
+        const double em1 = expm1(gl);
+        const double contribution = gain * (gl == 0.0 ? h.length : em1 / g);
         if (isfinite(contribution)) {
             atomicAdd(&a.visits[cell], 1u);
             ++segments;
@@ -318,9 +329,8 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
                 w[k] = fmax(0.0, plane_at(m.planes[4 * (size_t)cell + k], mx, my, mz));
                 sum += w[k];
             }
-            const uint4 cv = m.verts[cell];
             const uint32_t off = clad ? m.n_points : 0u;
-            const uint32_t vid[4] = {cv.x + off, cv.y + off, cv.z + off, cv.w + off};
+            const uint32_t vid[4] = {ax.verts[0] + off, ax.verts[1] + off, ax.verts[2] + off, ax.verts[3] + off};
             // consecutive cells share a face: carry the pending shares of the
             // vertices this cell keeps, write out the ones it drops
             double carried[4] = {0.0, 0.0, 0.0, 0.0};
@@ -345,7 +355,7 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
         } else {
             atomicAdd(&a.dropped[cell], 1u);
         }
-        gain *= exp(gl);
+        gain *= 1.0 + em1;
         p[0] = p[0] + u[0] * h.length;
         p[1] = p[1] + u[1] * h.length;
         p[2] = p[2] + u[2] * h.length;
@@ -397,9 +407,9 @@ __global__ void __launch_bounds__(256) hase_forward_finalize_kernel(FinalizeArgs
     const uint32_t c = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
     if (c >= a.m.n_cells) return;
     const TetDev& m = a.m;
-    const uint4 cv = m.verts[c];
-    const uint32_t off = m.clad[c] ? m.n_points : 0u;
-    const uint32_t vid[4] = {cv.x + off, cv.y + off, cv.z + off, cv.w + off};
+    const CellAux& ax = m.aux[c];
+    const uint32_t off = ax.clad ? m.n_points : 0u;
+    const uint32_t vid[4] = {ax.verts[0] + off, ax.verts[1] + off, ax.verts[2] + off, ax.verts[3] + off};
     const double vol = a.volume[c];
     const size_t nv = 2u * (size_t)m.n_points;
     double rel = DBL_MAX, abs_err = DBL_MAX, estimate = 0.0;
@@ -440,7 +450,7 @@ __global__ void __launch_bounds__(256) hase_forward_finalize_kernel(FinalizeArgs
     a.phi[c] = f;
     a.se[c] = abs_err;
     a.rse[c] = rel;
-    a.dndt[c] = (m.beta[c] * (a.sigma_e + a.sigma_a) - a.sigma_a) * (double)f;
+    a.dndt[c] = (ax.beta * (a.sigma_e + a.sigma_a) - a.sigma_a) * (double)f;
     const bool ok = a.dropped[c] == 0 && isfinite(rel) && rel * rel <= a.threshold;
     if (ok && a.convergence[c] == 0) a.convergence[c] = a.target;
     if (!ok) atomicAdd(a.open_cells, 1u);
@@ -467,11 +477,10 @@ struct hase_forward {
     std::vector<double> volume;
     std::vector<double> beta;
     std::vector<double> prefix;
+    std::vector<hase::fwd::CellAux> aux_host;
     hase::DevBuf<double4> planes;
-    hase::DevBuf<int32_t> link;
-    hase::DevBuf<uint4> vdev;
-    hase::DevBuf<double> beta_dev, prefix_dev, px, py, pz, volume_dev, lumped, sigma_a, sigma_e, scores;
-    hase::DevBuf<uint8_t> clad_dev;
+    hase::DevBuf<hase::fwd::CellAux> aux;
+    hase::DevBuf<double> prefix_dev, px, py, pz, volume_dev, lumped, sigma_a, sigma_e, scores;
     // counters: visits [C], dropped [C], convergence [C], open cells [1]
     hase::DevBuf<uint32_t> counters;
     hase::DevBuf<unsigned long long> segments;
@@ -485,7 +494,7 @@ struct hase_forward {
     }
     hase::fwd::TetDev view() const
     {
-        return hase::fwd::TetDev{planes.p, link.p, vdev.p, beta_dev.p, clad_dev.p, prefix_dev.p, px.p, py.p, pz.p,
+        return hase::fwd::TetDev{planes.p, aux.p, prefix_dev.p, px.p, py.p, pz.p,
                                  n_cells, n_points, n_tot, clad_alpha};
     }
 };
@@ -604,9 +613,15 @@ static void prepare(hase_forward* f, const hase_tet_mesh_t* mesh)
         ++i;
     }
     f->planes.upload(planes.data(), planes.size(), f->stream);
-    f->link.upload(link.data(), link.size(), f->stream);
-    f->vdev.upload(reinterpret_cast<const uint4*>(f->verts.data()), n, f->stream);
-    f->clad_dev.upload(f->clad.data(), n, f->stream);
+    f->aux_host.assign(n, CellAux{});
+    for (uint32_t c = 0; c < n; ++c) {
+        CellAux& ax = f->aux_host[c];
+        for (uint32_t k = 0; k < 4; ++k) {
+            ax.link[k] = link[4 * (size_t)c + k];
+            ax.verts[k] = f->verts[4 * (size_t)c + k];
+        }
+        ax.clad = f->clad[c];
+    }
     f->px.upload(px, np, f->stream);
     f->py.upload(py, np, f->stream);
     f->pz.upload(pz, np, f->stream);
@@ -624,7 +639,8 @@ static void set_beta(hase_forward* f, const double* beta)
         run += f->beta[c] * f->volume[c];
         f->prefix[c] = run;
     }
-    f->beta_dev.upload(f->beta.data(), f->n_cells, f->stream);
+    for (uint32_t c = 0; c < f->n_cells; ++c) f->aux_host[c].beta = f->beta[c];
+    f->aux.upload(f->aux_host.data(), f->n_cells, f->stream);
     f->prefix_dev.upload(f->prefix.data(), f->n_cells, f->stream);
     HIP_CHECK(hipStreamSynchronize(f->stream));
 }

@@ -233,13 +233,6 @@ static inline double orf_gain_coefficient(const orf_mesh* m, uint32_t cell, doub
     return m->n_tot * (m->beta[cell] * (se + sa) - sa);
 }
 
-/* integral_0^len exp(g s) ds (rayWalk.hpp:194-208, with expm1) */
-static inline double orf_track(double g, double len)
-{
-    const double gl = g * len;
-    return gl == 0.0 ? len : expm1(gl) / g;
-}
-
 int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
              const double u[3], double* scores, uint32_t* visits, uint32_t* dropped, uint64_t* segments)
 {
@@ -272,9 +265,13 @@ int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, c
             forbidden = t.forbidden;
             continue;
         }
-        /* score the segment (accumulation.hpp:310-345) */
+        /* score the segment (accumulation.hpp:310-345): the track-length
+         * integral of exp(g s) over the segment (rayWalk.hpp:194-208, with
+         * expm1), then the gain carried to the next cell */
         const double g = orf_gain_coefficient(m, cell, sigma_a, sigma_e);
-        const double contribution = gain * orf_track(g, h.length);
+        const double gl = g * h.length;
+        const double em1 = expm1(gl); /* track integral and gain factor */
+        const double contribution = gain * (gl == 0.0 ? h.length : em1 / g);
         if (isfinite(contribution)) {
             visits[cell] += 1;
             *segments += 1;
@@ -297,7 +294,7 @@ int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, c
         } else {
             dropped[cell] += 1;
         }
-        gain *= exp(g * h.length);
+        gain *= 1.0 + em1;
         p[0] = p[0] + u[0] * h.length;
         p[1] = p[1] + u[1] * h.length;
         p[2] = p[2] + u[2] * h.length;
