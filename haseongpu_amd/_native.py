@@ -59,6 +59,7 @@ EXPORTED_SYMBOLS = (
     "hase_forward_evaluate",
     "hase_forward_trace",
     "hase_forward_stats",
+    "hase_forward_set_reflections",
     "hase_forward_destroy",
     "hase_calc_forward_phi_ase",
     "hase_last_error",
@@ -282,7 +283,19 @@ class ForwardStats(C.Structure):
     _fields_ = [
         ("trace_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64), ("segments", C.c_uint64),
         ("dropped_rays", C.c_uint64), ("launches", C.c_uint32), ("open_cells", C.c_uint32),
+        ("srm_status", C.c_uint32), ("srm_passes", C.c_uint32), ("srm_remaining", C.c_double),
     ]
+
+
+class ForwardReflections(C.Structure):
+    _fields_ = [
+        ("face_boundaries", _i32p), ("number_of_surfaces", C.c_uint32), ("reservoir_slots", C.c_uint32),
+        ("reflectivity", _f32p), ("n_inside", _f32p), ("n_outside", _f32p), ("max_passes", C.c_uint32),
+        ("divergence_streak", C.c_uint32), ("tolerance", C.c_double),
+    ]
+
+
+SRM_STATUS = {0: "DISABLED", 1: "CONVERGED", 2: "STABLE", 3: "MAX_PASSES", 4: "DIVERGED"}
 
 
 def load():
@@ -339,6 +352,7 @@ def load():
     L.hase_forward_trace.argtypes = [ctx_p, C.c_uint32, C.c_uint32, C.c_uint32, _f64p, _u32p, _u32p,
                                      C.POINTER(C.c_uint64)]
     L.hase_forward_stats.argtypes = [ctx_p, C.POINTER(ForwardStats)]
+    L.hase_forward_set_reflections.argtypes = [ctx_p, C.POINTER(ForwardReflections)]
     L.hase_forward_destroy.argtypes = [ctx_p]
     L.hase_calc_forward_phi_ase.argtypes = [C.POINTER(TetMeshC), C.POINTER(Experiment), C.POINTER(Compute),
                                             C.POINTER(ForwardResultC)]
@@ -355,6 +369,7 @@ def load():
                  "hase_simulation_set_beta", "hase_simulation_step", "hase_simulation_state",
                  "hase_simulation_times", "hase_forward_create", "hase_forward_update_beta",
                  "hase_forward_evaluate", "hase_forward_trace", "hase_forward_stats", "hase_forward_destroy",
+                 "hase_forward_set_reflections",
                  "hase_calc_forward_phi_ase"):
         getattr(L, name).restype = C.c_int
     _lib = L
@@ -826,10 +841,32 @@ class ForwardContext:
                                         _ptr(dropped, C.c_uint32), C.byref(seg)))
         return scores.reshape(self.BATCHES, -1), visits, dropped, int(seg.value)
 
+    def set_reflections(self, face_boundaries, reflectivity, n_inside=None, n_outside=None, *, slots=32,
+                        max_passes=40, divergence_streak=3, tolerance=1e-4):
+        """Reflective surfaces for the surface reservoir passes: face_boundaries
+        (C, 4) surface ids (face k opposite vertex k, 0 = none), per-surface
+        reflectivity and optional refractive indices (index 0 unused)."""
+        fb = np.array(face_boundaries, dtype=np.int32).reshape(-1)
+        if fb.size != 4 * self.n_cells:
+            raise ValueError(f"face_boundaries needs {4 * self.n_cells} values, got {fb.size}")
+        refl = np.array(reflectivity, dtype=np.float32).reshape(-1)
+        nin = None if n_inside is None else np.array(n_inside, dtype=np.float32).reshape(-1)
+        nout = None if n_outside is None else np.array(n_outside, dtype=np.float32).reshape(-1)
+        for name, arr in (("n_inside", nin), ("n_outside", nout)):
+            if arr is not None and arr.size != refl.size:
+                raise ValueError(f"{name} needs {refl.size} values")
+        self._refl_arrays = (fb, refl, nin, nout)
+        st = ForwardReflections(_ptr(fb, C.c_int32), refl.size, int(slots), _ptr(refl, C.c_float),
+                                _ptr(nin, C.c_float), _ptr(nout, C.c_float), int(max_passes),
+                                int(divergence_streak), float(tolerance))
+        check(load().hase_forward_set_reflections(self._h, C.byref(st)))
+
     def stats(self):
         st = ForwardStats()
         check(load().hase_forward_stats(self._h, C.byref(st)))
-        return {name: getattr(st, name) for name, _ in ForwardStats._fields_}
+        out = {name: getattr(st, name) for name, _ in ForwardStats._fields_}
+        out["srm_status_name"] = SRM_STATUS.get(out["srm_status"], "?")
+        return out
 
     def close(self):
         if self._h:

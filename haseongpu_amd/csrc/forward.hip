@@ -3,16 +3,21 @@
 // track-length integral of their gain into the vertices of every cell they
 // cross; each cell's flux is its mean lumped-vertex density.
 //
-// Follows the reference's present forward path without surface reflections:
-// AccumulateForwardPhiAse / ForwardAseCellPolicy
+// Follows the reference's present forward path: AccumulateForwardPhiAse /
+// ForwardAseCellPolicy
 // (include/kernels/forward/accumulation.hpp:300-405), the walk and its tie
 // and recovery rules (policyRay.hpp:379-465, rayWalk.hpp:106-208,
 // rayTransition.hpp:17-188), midpoint vertex weights (barycentric.hpp:69-99),
 // stratified beta*V sources and wavelengths (volumeSampling.hpp:19-137),
 // face planes and lumped vertex volumes (include/core/mesh.hpp:630-725,
 // include/kernels/vertexAccumulation.hpp:26-39), FinalizeForwardVolumePhiAse
-// (include/kernels/forwardPhiAseMapping.hpp:47-147) and the adaptive loop
-// (include/core/forwardPhiAseEvaluator.hpp:114-178, types.hpp:333-396).
+// (include/kernels/forwardPhiAseMapping.hpp:47-147), the adaptive loop
+// (include/core/forwardPhiAseEvaluator.hpp:114-178, types.hpp:333-396) and
+// the surface reservoir method for reflections (include/core/forwardSrm.hpp:
+// 120-366, accumulation.hpp:407-834): rays leaving through a reflective face
+// leave their reflected direction in that face's weighted reservoir (one per
+// RSE batch), and passes restart rays from face centroids until the
+// remaining weight is below the tolerance, stops changing or keeps growing.
 //
 // Layout in HBM (C cells, Np points): face planes 4 x double4 per cell
 // (128 B, one cache line), a 64-B cell record (neighbour links: neighbour * 4
@@ -57,7 +62,8 @@ struct CellAux {
     uint32_t verts[4]; // point ids of the 4 vertices
     double beta;
     uint32_t clad;
-    uint32_t pad[5];
+    int32_t bface[4];  // reflective boundary face index of each face, -1 if none
+    uint32_t pad;
 };
 static_assert(sizeof(CellAux) == 64, "cell record is one half cache line");
 
@@ -96,6 +102,38 @@ struct Cross {
     uint32_t cell;
     int forbidden;
     int status;
+    int face;          // boundary face when status == kBoundary
+};
+
+// Reflective boundary face: centroid (where reflected rays restart), outward
+// unit normal and surface data (include/kernels/forward/rayWalk.hpp:42-84,
+// reflection.hpp:21-47).
+struct BFace {
+    double c[3];
+    double n[3];
+    double refl, n_in, n_out;
+    uint32_t face;     // cell * 4 + local face
+    uint32_t pad;
+};
+
+struct Slot {
+    double d[3];
+    double w;
+    uint32_t sigma;
+    uint32_t pad;
+};
+
+// One side (input or output) of the surface reservoirs of all eight batches.
+struct Reservoir {
+    uint32_t* count;        // [8][nb]
+    double* face_weight;    // [8][nb]
+    Slot* slots;            // [8][nb][S]
+};
+
+struct SrmArgs {
+    const BFace* faces;
+    uint32_t nb, slots, pass;
+    Reservoir out;
 };
 
 __device__ __forceinline__ double plane_at(double4 pl, double x, double y, double z)
@@ -145,7 +183,7 @@ __device__ __forceinline__ Hit next_face(const TetDev& m, uint32_t cell, const d
 __device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const double p[3], const double u[3])
 {
 #pragma clang fp contract(off)
-    Cross r{cell, -1, kEntered};
+    Cross r{cell, -1, kEntered, -1};
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     uint32_t recent0 = cell, recent1 = kNone, recent2 = kNone;
     for (uint32_t step = 0; step < kMaxRecovery; ++step) {
@@ -153,6 +191,7 @@ __device__ Cross recover(const TetDev& m, uint32_t cell, int exit_face, const do
         if (link < 0) {
             r.cell = cell;
             r.status = kBoundary;
+            r.face = exit_face;
             return r;
         }
         cell = (uint32_t)link >> 2;
@@ -187,8 +226,8 @@ __device__ __forceinline__ Cross cross(const TetDev& m, uint32_t cell, Hit h, co
 {
     if (h.tied & (h.tied - 1u)) return recover(m, cell, h.face, p, u);
     const int32_t link = m.aux[cell].link[h.face];
-    if (link < 0) return Cross{cell, -1, kBoundary};
-    return Cross{(uint32_t)link >> 2, link & 3, kEntered};
+    if (link < 0) return Cross{cell, -1, kBoundary, h.face};
+    return Cross{(uint32_t)link >> 2, link & 3, kEntered, -1};
 }
 
 // Start of ray r of a launch: contiguous RSE batch, stratified beta*V cell,
@@ -259,21 +298,24 @@ __device__ __forceinline__ void ray_start(const TraceArgs& a, uint32_t r, uint32
     lambda = a.n_lambda ? (uint32_t)(((uint64_t)i * a.n_lambda / count + a.phase[b]) % a.n_lambda) : 0u;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceArgs a)
+struct Exit {
+    uint32_t cell;
+    int face;          // boundary face the walk left through, -1 if it did not
+    double weight;     // source weight times accumulated gain there
+};
+
+// Walks one ray from `p` in cell `cell` (entered through `forbidden`),
+// scoring weight * gain * track integral into the vertices of every cell.
+__device__ void walk(const TraceArgs& a, uint32_t cell, double p[3], const double u[3], int forbidden,
+                     double weight, uint32_t batch, double sa, double se, uint32_t& segments, Exit& ex)
 {
 #pragma clang fp contract(off)
-    const uint32_t r = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
-    if (r >= a.rays) return;
     const TetDev& m = a.m;
-    uint32_t batch, cell, lambda;
-    double p[3], u[3];
-    ray_start(a, r, batch, cell, p, u, lambda);
-    const double sa = a.sigma_a[lambda], se = a.sigma_e[lambda];
     double* scores = a.scores + (size_t)batch * 2u * m.n_points;
-    int forbidden = -1;
     double gain = 1.0;
-    uint32_t segments = 0;
     bool dropped = true;
+    ex.face = -1;
+    ex.weight = 0.0;
     // vertex shares of the last scored cell, not yet added to `scores`
     uint32_t pend_v[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     double pend_s[4] = {0.0, 0.0, 0.0, 0.0};
@@ -298,15 +340,14 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
             }
             if (face < 0) break;
             const Cross t = recover(m, cell, face, p, u);
-            if (t.status == kFailed) {
-                cell = t.cell;
-                break;
-            }
+            cell = t.cell;
+            if (t.status == kFailed) break;
             if (t.status == kBoundary) {
                 dropped = false;
+                ex.face = t.face;
+                ex.weight = weight * gain;
                 break;
             }
-            cell = t.cell;
             forbidden = t.forbidden;
             continue;
         }
@@ -317,7 +358,7 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
         const double gl = g * h.length;
         // one expm1 gives the track integral and the gain factor
         const double em1 = expm1(gl);
-        const double contribution = gain * (gl == 0.0 ? h.length : em1 / g);
+        const double contribution = weight * gain * (gl == 0.0 ? h.length : em1 / g);
         if (isfinite(contribution)) {
             atomicAdd(&a.visits[cell], 1u);
             ++segments;
@@ -360,22 +401,206 @@ __global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceAr
         p[1] = p[1] + u[1] * h.length;
         p[2] = p[2] + u[2] * h.length;
         const Cross t = cross(m, cell, h, p, u);
-        if (t.status == kFailed) {
-            cell = t.cell;
-            break;
-        }
+        cell = t.cell;
+        if (t.status == kFailed) break;
         if (t.status == kBoundary) {
             dropped = false;
+            ex.face = t.face;
+            ex.weight = weight * gain;
             break;
         }
-        cell = t.cell;
         forbidden = t.forbidden;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (pend_s[k] != 0.0) unsafeAtomicAdd(&scores[pend_v[k]], pend_s[k]);
     if (dropped) atomicAdd(&a.dropped[cell], 1u);
+    ex.cell = cell;
+}
+
+__device__ __forceinline__ void unit_vector(double v[3])
+{
+#pragma clang fp contract(off)
+    const double len = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len <= DBL_EPSILON) {
+        v[0] = v[1] = v[2] = 0.0;
+        return;
+    }
+    const double inv = 1.0 / len;
+    v[0] = v[0] * inv;
+    v[1] = v[1] * inv;
+    v[2] = v[2] * inv;
+}
+
+// depositReflection (accumulation.hpp:409-455): weighted reservoir sampling
+// of the reflected direction at reflective boundary face f of `batch`.
+__device__ void deposit(const SrmArgs& s, uint32_t batch, uint32_t f, const double dir[3], double w, uint32_t sigma,
+                        uint32_t seed, uint32_t launch, uint32_t ray)
+{
+#pragma clang fp contract(off)
+    if (s.slots == 0 || !(w > 0.0) || !isfinite(w)) return;
+    const BFace bf = s.faces[f];
+    double reflectance = fmax(0.0, bf.refl);
+    if (bf.n_in > 0.0 && bf.n_out > 0.0) {
+        double d[3] = {dir[0], dir[1], dir[2]};
+        unit_vector(d);
+        const double cosi = fmin(1.0, fabs(d[0] * bf.n[0] + d[1] * bf.n[1] + d[2] * bf.n[2]));
+        const double sin2 = fmax(0.0, 1.0 - cosi * cosi);
+        const double ratio = bf.n_in / bf.n_out;
+        if (ratio * ratio * sin2 > 1.0) reflectance = 1.0;
+    }
+    const double rw = w * reflectance;
+    if (!(rw > 0.0) || !isfinite(rw)) return;
+    const size_t fi = (size_t)batch * s.nb + f;
+    const uint32_t seen = atomicAdd(&s.out.count[fi], 1u);
+    const double prior = atomicAdd(&s.out.face_weight[fi], rw);
+    uint32_t slot = seen;
+    if (slot >= s.slots) {
+        uint32_t o[4];
+        philox4x32_10(ray, launch, 0x10000u + s.pass, 1u, seed, kKeyTag, o);
+        const double keep = fmin(1.0, (double)s.slots * rw / (prior + rw));
+        if ((double)u32_to_unit(o[0]) >= keep) return;
+        slot = (uint32_t)((double)u32_to_unit(o[1]) * (double)s.slots);
+        if (slot >= s.slots) slot = s.slots - 1;
+    }
+    const double dn = 2.0 * (dir[0] * bf.n[0] + dir[1] * bf.n[1] + dir[2] * bf.n[2]);
+    double out[3] = {dir[0] - bf.n[0] * dn, dir[1] - bf.n[1] * dn, dir[2] - bf.n[2] * dn};
+    unit_vector(out);
+    Slot& sl = s.out.slots[fi * s.slots + slot];
+    sl.d[0] = out[0];
+    sl.d[1] = out[1];
+    sl.d[2] = out[2];
+    sl.w = rw;
+    sl.sigma = sigma;
+}
+
+// Primary rays (AccumulateForwardPhiAse / AccumulateForwardPhiAseReservoir):
+// with kReflect a ray leaving through a reflective face deposits there.
+template <bool kReflect>
+__global__ void __launch_bounds__(kTraceBlock) hase_forward_trace_kernel(TraceArgs a, SrmArgs s)
+{
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
+    if (r >= a.rays) return;
+    uint32_t batch, cell, lambda, segments = 0;
+    double p[3], u[3];
+    ray_start(a, r, batch, cell, p, u, lambda);
+    Exit ex;
+    walk(a, cell, p, u, -1, 1.0, batch, a.sigma_a[lambda], a.sigma_e[lambda], segments, ex);
+    if (kReflect && ex.face >= 0) {
+        const int32_t f = a.m.aux[ex.cell].bface[ex.face];
+        if (f >= 0) deposit(s, batch, (uint32_t)f, u, ex.weight, lambda, a.seed, a.launch, r);
+    }
     atomicAdd(a.segments, (unsigned long long)segments);
+}
+
+struct PassArgs {
+    Reservoir in;
+    const double* cdf;          // [8][nb] normalised inclusive prefix of the input face weights
+    double source[kBatches];    // weight of each reflected ray of a batch, 0 when the batch has stopped
+    double shift[kBatches];     // stratification offset of the face choice
+};
+
+// One reflected pass (AccumulateReflectedForwardPhiAse,
+// accumulation.hpp:679-832): a face by reservoir weight (stratified when
+// every face can get a ray), a slot by slot weight, a walk from the face
+// centroid away from the face, deposits into the output reservoir.
+__global__ void __launch_bounds__(kTraceBlock) hase_forward_reflect_kernel(TraceArgs a, SrmArgs s, PassArgs q)
+{
+#pragma clang fp contract(off)
+    const uint32_t r = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x;
+    if (r >= a.rays || s.nb == 0) return;
+    const uint32_t base = a.rays / kBatches, rem = a.rays % kBatches;
+    const uint32_t split = rem * (base + 1u);
+    const uint32_t b = r < split ? r / (base + 1u) : rem + (r - split) / base;
+    const uint32_t count = base + (b < rem ? 1u : 0u);
+    const uint32_t i = r - (b * base + (b < rem ? b : rem));
+    if (!(q.source[b] > 0.0)) return;
+    uint32_t o[4];
+    philox4x32_10(r, a.launch, 0x10000u + s.pass, 2u, a.seed, kKeyTag, o);
+    const double target =
+        s.nb <= count ? ((double)i + q.shift[b]) / (double)count : (double)u32_to_unit(o[0]);
+    const double* cdf = q.cdf + (size_t)b * s.nb;
+    uint32_t lo = 0, hi = s.nb;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (cdf[mid] <= target) lo = mid + 1u;
+        else hi = mid;
+    }
+    const uint32_t f = lo < s.nb ? lo : s.nb - 1u;
+    const size_t fi = (size_t)b * s.nb + f;
+    const uint32_t filled = min(q.in.count[fi], s.slots);
+    if (filled == 0) return;
+    const Slot* sl = q.in.slots + fi * s.slots;
+    double slot_weight = 0.0;
+    for (uint32_t k = 0; k < filled; ++k) slot_weight += sl[k].w;
+    if (!(slot_weight > 0.0)) return;
+    const double t2 = (double)u32_to_unit(o[1]) * slot_weight;
+    double cum = 0.0;
+    uint32_t pick = filled - 1u;
+    for (uint32_t k = 0; k < filled; ++k) {
+        cum += sl[k].w;
+        if (cum >= t2) {
+            pick = k;
+            break;
+        }
+    }
+    double dir[3] = {sl[pick].d[0], sl[pick].d[1], sl[pick].d[2]};
+    unit_vector(dir);
+    const uint32_t lam = sl[pick].sigma;
+    const BFace bf = s.faces[f];
+    double p[3] = {bf.c[0], bf.c[1], bf.c[2]};
+    uint32_t segments = 0;
+    Exit ex;
+    walk(a, bf.face >> 2, p, dir, (int)(bf.face & 3u), q.source[b], b, a.sigma_a[lam], a.sigma_e[lam], segments,
+         ex);
+    if (ex.face >= 0) {
+        const int32_t g = a.m.aux[ex.cell].bface[ex.face];
+        if (g >= 0) deposit(s, b, (uint32_t)g, dir, ex.weight, lam, a.seed, a.launch, r);
+    }
+    atomicAdd(a.segments, (unsigned long long)segments);
+}
+
+// Per batch (one work-group each): inclusive prefix of the face weights of a
+// reservoir side, normalised by its total; totals[b] = the total.
+__global__ void __launch_bounds__(1024) hase_forward_face_cdf_kernel(const double* face_weight, uint32_t nb,
+                                                                     double* cdf, double* totals)
+{
+    __shared__ double wave_sums[16];
+    __shared__ double carry_s;
+    const uint32_t b = blockIdx.x, t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    const double* w = face_weight + (size_t)b * nb;
+    double* out = cdf + (size_t)b * nb;
+    if (t == 0) carry_s = 0.0;
+    __syncthreads();
+    for (uint32_t start = 0; start < nb; start += 1024u) {
+        const uint32_t i = start + t;
+        double v = i < nb ? w[i] : 0.0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double x = __shfl_up(v, d, 64);
+            if ((int)lane >= d) v += x;
+        }
+        if (lane == 63u) wave_sums[wv] = v;
+        __syncthreads();
+        double before = carry_s;
+        for (uint32_t k = 0; k < wv; ++k) before += wave_sums[k];
+        if (i < nb) out[i] = before + v;
+        __syncthreads();
+        if (t == 1023u) carry_s = before + v;
+        __syncthreads();
+    }
+    const double total = carry_s;
+    for (uint32_t i = t; i < nb; i += 1024u) out[i] = total > 0.0 ? out[i] / total : 0.0;
+    if (t == 0) totals[b] = total;
+}
+
+__global__ void hase_forward_clear_reservoir_kernel(uint32_t* count, double* face_weight, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        count[i] = 0u;
+        face_weight[i] = 0.0;
+    }
 }
 
 struct FinalizeArgs {
@@ -487,6 +712,18 @@ struct hase_forward {
     hase::DevBuf<float> phi;
     hase::DevBuf<double> se, rse, dndt;
     hase_forward_stats_t stats{};
+    // surface reservoir method (reflections)
+    bool reflect = false;                       // experiment.use_reflections
+    uint32_t nb = 0, slots = 32, max_passes = 40, streak = 3;
+    double tolerance = 1.0e-4;
+    hase::DevBuf<hase::fwd::BFace> bfaces;
+    hase::DevBuf<uint32_t> res_count[2];
+    hase::DevBuf<double> res_weight[2];
+    hase::DevBuf<hase::fwd::Slot> res_slots[2];
+    hase::DevBuf<double> face_cdf, face_totals;
+    uint32_t srm_status[hase::fwd::kBatches] = {0};
+    uint32_t srm_passes[hase::fwd::kBatches] = {0};
+    double srm_remaining[hase::fwd::kBatches] = {0.0};
 
     ~hase_forward()
     {
@@ -536,10 +773,6 @@ static void validate(const hase_tet_mesh_t* mesh, const hase_experiment_t* exp, 
         if (!std::isfinite(exp->sigma_a[w]) || !std::isfinite(exp->sigma_e[w]) || exp->sigma_a[w] < 0.0 ||
             exp->sigma_e[w] < 0.0)
             throw HaseError(HASE_E_INVALID, "cross sections must be finite and >= 0");
-    if (exp->use_reflections)
-        throw HaseError(HASE_E_INVALID,
-                        "forward estimator: surface reflections (the surface reservoir) are not built; "
-                        "set use_reflections = 0");
 }
 
 // Face planes, neighbours, volumes and lumped vertex volumes on the host,
@@ -621,6 +854,7 @@ static void prepare(hase_forward* f, const hase_tet_mesh_t* mesh)
             ax.verts[k] = f->verts[4 * (size_t)c + k];
         }
         ax.clad = f->clad[c];
+        for (uint32_t k = 0; k < 4; ++k) ax.bface[k] = -1;
     }
     f->px.upload(px, np, f->stream);
     f->py.upload(py, np, f->stream);
@@ -666,6 +900,7 @@ static std::unique_ptr<hase_forward> create(const hase_tet_mesh_t* mesh, const h
     f->n_tot = mesh->n_tot;
     f->tau = mesh->crystal_t_fluo;
     f->clad_alpha = mesh->cladding_absorption;
+    f->reflect = exp->use_reflections != 0;
     prepare(f.get(), mesh);
     set_beta(f.get(), mesh->beta_volume);
     f->sigma_a.upload(exp->sigma_a, f->n_lambda, f->stream);
@@ -715,24 +950,221 @@ static TraceArgs trace_args(hase_forward* f, uint32_t seed, uint32_t launch, uin
 }
 
 // One launch of `rays` rays; returns its device time in ms.
+static SrmArgs srm_args(hase_forward* f, int out_side, uint32_t pass)
+{
+    SrmArgs s{};
+    s.faces = f->bfaces.p;
+    s.nb = f->nb;
+    s.slots = f->slots;
+    s.pass = pass;
+    s.out = Reservoir{f->res_count[out_side].p, f->res_weight[out_side].p, f->res_slots[out_side].p};
+    return s;
+}
+
+static void clear_side(hase_forward* f, int side)
+{
+    const size_t n = (size_t)kBatches * f->nb;
+    hipLaunchKernelGGL(hase_forward_clear_reservoir_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       f->stream, f->res_count[side].p, f->res_weight[side].p, n);
+    HIP_CHECK(hipGetLastError());
+}
+
+// Normalised face prefix of a reservoir side per batch; returns the totals.
+static void face_totals(hase_forward* f, int side, double totals[kBatches])
+{
+    hipLaunchKernelGGL(hase_forward_face_cdf_kernel, dim3(kBatches), dim3(1024), 0, f->stream,
+                       f->res_weight[side].p, f->nb, f->face_cdf.p, f->face_totals.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(totals, f->face_totals.p, kBatches * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIP_CHECK(hipStreamSynchronize(f->stream));
+}
+
+// One launch of `rays` rays (with reflected passes when reflections are on);
+// returns the device time of its ray kernels in ms.
 static double launch_trace(hase_forward* f, uint32_t seed, uint32_t launch, uint32_t rays)
 {
     if (rays == 0 || !(f->prefix.back() > 0.0)) return 0.0;
     const TraceArgs a = trace_args(f, seed, launch, rays);
+    const bool srm = f->reflect && f->nb > 0 && f->slots > 0;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
-    HIP_CHECK(hipEventRecord(e0, f->stream));
     const unsigned blocks = (unsigned)(((uint64_t)rays + kTraceBlock - 1) / kTraceBlock);
-    hipLaunchKernelGGL(hase_forward_trace_kernel, dim3(blocks), dim3(kTraceBlock), 0, f->stream, a);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(e1, f->stream));
-    HIP_CHECK(hipEventSynchronize(e1));
-    float ms = 0.0f;
-    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    double ms_total = 0.0;
+    auto timed = [&](auto&& enqueue) {
+        HIP_CHECK(hipEventRecord(e0, f->stream));
+        enqueue();
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(e1, f->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        ms_total += ms;
+    };
+    if (!srm) {
+        timed([&] {
+            hipLaunchKernelGGL(hase_forward_trace_kernel<false>, dim3(blocks), dim3(kTraceBlock), 0, f->stream, a,
+                               SrmArgs{});
+        });
+    } else {
+        // forwardSrm.hpp:120-366, per RSE batch: primary rays deposit into
+        // side 0, then passes alternate sides until every batch has stopped
+        clear_side(f, 0);
+        timed([&] {
+            hipLaunchKernelGGL(hase_forward_trace_kernel<true>, dim3(blocks), dim3(kTraceBlock), 0, f->stream, a,
+                               srm_args(f, 0, 0));
+        });
+        double initial[kBatches], previous[kBatches], current[kBatches];
+        uint32_t grow[kBatches] = {0};
+        bool active[kBatches];
+        face_totals(f, 0, initial);
+        for (uint32_t b = 0; b < kBatches; ++b) {
+            previous[b] = initial[b];
+            active[b] = initial[b] > 0.0;
+            f->srm_status[b] = active[b] ? HASE_SRM_MAX_PASSES : HASE_SRM_CONVERGED;
+            f->srm_passes[b] = 0;
+            f->srm_remaining[b] = active[b] ? 1.0 : 0.0;
+        }
+        int in = 0;
+        for (uint32_t pass = 1; pass <= f->max_passes; ++pass) {
+            bool any = false;
+            for (uint32_t b = 0; b < kBatches; ++b) any |= active[b];
+            if (!any) break;
+            const int out = 1 - in;
+            clear_side(f, out);
+            PassArgs q{};
+            q.in = Reservoir{f->res_count[in].p, f->res_weight[in].p, f->res_slots[in].p};
+            q.cdf = f->face_cdf.p;
+            for (uint32_t b = 0; b < kBatches; ++b) {
+                const uint32_t count = rays / kBatches + (b < rays % kBatches ? 1u : 0u);
+                q.source[b] = active[b] && count ? previous[b] / (double)count : 0.0;
+                uint32_t o[4];
+                philox4x32_10(0xFFFFFFFEu, b, launch, pass, seed, kKeyTag, o);
+                q.shift[b] = (double)u32_to_unit(o[0]);
+            }
+            timed([&] {
+                hipLaunchKernelGGL(hase_forward_reflect_kernel, dim3(blocks), dim3(kTraceBlock), 0, f->stream, a,
+                                   srm_args(f, out, pass), q);
+            });
+            face_totals(f, out, current);
+            // stop rules (forwardSrm.hpp:322-361)
+            for (uint32_t b = 0; b < kBatches; ++b) {
+                if (!active[b]) continue;
+                const double w = current[b];
+                f->srm_passes[b] = pass;
+                f->srm_remaining[b] = w / initial[b];
+                if (w > previous[b]) {
+                    if (++grow[b] >= f->streak) {
+                        f->srm_status[b] = HASE_SRM_DIVERGED;
+                        active[b] = false;
+                        continue;
+                    }
+                } else {
+                    grow[b] = 0;
+                    if (std::fabs(w - previous[b]) / std::max(w, 1.0e-30) < f->tolerance) {
+                        f->srm_status[b] = HASE_SRM_STABLE;
+                        active[b] = false;
+                        continue;
+                    }
+                }
+                if (f->srm_remaining[b] < f->tolerance) {
+                    f->srm_status[b] = HASE_SRM_CONVERGED;
+                    active[b] = false;
+                    continue;
+                }
+                previous[b] = w;
+            }
+            in = out;
+        }
+        for (uint32_t b = 0; b < kBatches; ++b) {
+            f->stats.srm_passes = std::max(f->stats.srm_passes, f->srm_passes[b]);
+            f->stats.srm_remaining = std::max(f->stats.srm_remaining, f->srm_remaining[b]);
+            f->stats.srm_status = std::max(f->stats.srm_status, (uint32_t)f->srm_status[b]);
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return ms;
+    return ms_total;
+}
+
+// Reflective boundary faces and their reservoirs (hase_forward_set_reflections).
+static void set_reflections(hase_forward* f, const hase_forward_reflections_t* r)
+{
+    if (!r) throw HaseError(HASE_E_INVALID, "reflections must not be NULL");
+    if (!r->face_boundaries) throw HaseError(HASE_E_INVALID, "face_boundaries must not be NULL");
+    if (r->number_of_surfaces > 0 && !r->reflectivity)
+        throw HaseError(HASE_E_INVALID, "reflectivity is required for the surfaces");
+    for (uint32_t i = 0; i < r->number_of_surfaces; ++i) {
+        if (r->reflectivity && (!std::isfinite(r->reflectivity[i]) || r->reflectivity[i] < 0.0f ||
+                                r->reflectivity[i] > 1.0f))
+            throw HaseError(HASE_E_INVALID, "surface reflectivities must lie in [0, 1]");
+        if ((r->n_inside && !(r->n_inside[i] >= 0.0f)) || (r->n_outside && !(r->n_outside[i] >= 0.0f)))
+            throw HaseError(HASE_E_INVALID, "refractive indices must be >= 0");
+    }
+    if (!(r->tolerance > 0.0) || r->max_passes == 0 || r->divergence_streak == 0)
+        throw HaseError(HASE_E_INVALID, "tolerance must be > 0, max_passes and divergence_streak >= 1");
+    const uint32_t n = f->n_cells;
+    const uint32_t* cells = f->verts.data();
+    std::vector<double> hx(f->n_points), hy(f->n_points), hz(f->n_points);
+    HIP_CHECK(hipMemcpy(hx.data(), f->px.p, f->n_points * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(hy.data(), f->py.p, f->n_points * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(hz.data(), f->pz.p, f->n_points * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<BFace> faces;
+    for (uint32_t c = 0; c < n; ++c) {
+        CellAux& ax = f->aux_host[c];
+        for (uint32_t k = 0; k < 4; ++k) {
+            ax.bface[k] = -1;
+            const int32_t id = r->face_boundaries[4 * (size_t)c + k];
+            if (ax.link[k] >= 0 || id <= 0 || (uint32_t)id >= r->number_of_surfaces) continue;
+            const uint32_t* cv = cells + 4 * (size_t)c;
+            uint32_t fv[3], j = 0;
+            for (uint32_t q = 0; q < 4; ++q)
+                if (q != k) fv[j++] = cv[q];
+            std::sort(fv, fv + 3);
+            BFace bf{};
+            // face centroid and outward normal (rayWalk.hpp:42-84)
+            bf.c[0] = (hx[fv[0]] + hx[fv[1]] + hx[fv[2]]) * (1.0 / 3.0);
+            bf.c[1] = (hy[fv[0]] + hy[fv[1]] + hy[fv[2]]) * (1.0 / 3.0);
+            bf.c[2] = (hz[fv[0]] + hz[fv[1]] + hz[fv[2]]) * (1.0 / 3.0);
+            const double ctr[3] = {(hx[cv[0]] + hx[cv[1]] + hx[cv[2]] + hx[cv[3]]) * 0.25,
+                                   (hy[cv[0]] + hy[cv[1]] + hy[cv[2]] + hy[cv[3]]) * 0.25,
+                                   (hz[cv[0]] + hz[cv[1]] + hz[cv[2]] + hz[cv[3]]) * 0.25};
+            const double bx = hx[fv[1]] - hx[fv[0]], by = hy[fv[1]] - hy[fv[0]], bz = hz[fv[1]] - hz[fv[0]];
+            const double cx = hx[fv[2]] - hx[fv[0]], cy = hy[fv[2]] - hy[fv[0]], cz = hz[fv[2]] - hz[fv[0]];
+            double nr[3] = {by * cz - bz * cy, bz * cx - bx * cz, bx * cy - by * cx};
+            const double len = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+            const double inv = len <= DBL_EPSILON ? 0.0 : 1.0 / len;
+            for (int d = 0; d < 3; ++d) nr[d] = nr[d] * inv;
+            if (nr[0] * (ctr[0] - bf.c[0]) + nr[1] * (ctr[1] - bf.c[1]) + nr[2] * (ctr[2] - bf.c[2]) > 0.0)
+                for (int d = 0; d < 3; ++d) nr[d] = -nr[d];
+            for (int d = 0; d < 3; ++d) bf.n[d] = nr[d];
+            bf.refl = r->reflectivity ? (double)r->reflectivity[id] : 0.0;
+            bf.n_in = r->n_inside ? (double)r->n_inside[id] : 0.0;
+            bf.n_out = r->n_outside ? (double)r->n_outside[id] : 0.0;
+            bf.face = 4 * c + k;
+            ax.bface[k] = (int32_t)faces.size();
+            faces.push_back(bf);
+        }
+    }
+    f->nb = (uint32_t)faces.size();
+    f->slots = r->reservoir_slots;
+    f->max_passes = r->max_passes;
+    f->streak = r->divergence_streak;
+    f->tolerance = r->tolerance;
+    if (f->nb && (size_t)kBatches * f->nb * f->slots > ((size_t)1 << 31))
+        throw HaseError(HASE_E_INVALID, "reservoir too large: reduce reservoir_slots");
+    f->aux.upload(f->aux_host.data(), n, f->stream);
+    if (f->nb) {
+        f->bfaces.upload(faces.data(), faces.size(), f->stream);
+        for (int side = 0; side < 2; ++side) {
+            f->res_count[side].alloc((size_t)kBatches * f->nb);
+            f->res_weight[side].alloc((size_t)kBatches * f->nb);
+            f->res_slots[side].alloc(std::max<size_t>(1, (size_t)kBatches * f->nb * f->slots));
+        }
+        f->face_cdf.alloc((size_t)kBatches * f->nb);
+        f->face_totals.alloc(kBatches);
+    }
+    HIP_CHECK(hipStreamSynchronize(f->stream));
 }
 
 static uint32_t adaptive_target(uint32_t min_rays, uint32_t max_rays, uint32_t steps, uint32_t k)
@@ -864,6 +1296,7 @@ int hase_forward_trace(hase_forward_t* f, uint32_t seed, uint32_t launch, uint32
             throw hase::HaseError(HASE_E_INVALID, "forward, scores, ray_visits and dropped_rays must not be NULL");
         HIP_CHECK(hipSetDevice(f->device));
         hase::fwd::clear(f);
+        f->stats = hase_forward_stats_t{};
         f->stats.trace_ms = hase::fwd::launch_trace(f, seed, launch, rays);
         const size_t n = f->n_cells;
         HIP_CHECK(hipMemcpyAsync(scores, f->scores.p, (size_t)hase::fwd::kBatches * 2u * f->n_points * sizeof(double),
@@ -875,6 +1308,15 @@ int hase_forward_trace(hase_forward_t* f, uint32_t seed, uint32_t launch, uint32
         HIP_CHECK(hipMemcpyAsync(&seg, f->segments.p, sizeof(seg), hipMemcpyDeviceToHost, f->stream));
         HIP_CHECK(hipStreamSynchronize(f->stream));
         if (segments) *segments = seg;
+    });
+}
+
+int hase_forward_set_reflections(hase_forward_t* f, const hase_forward_reflections_t* reflections)
+{
+    return hase::guarded([&] {
+        if (!f) throw hase::HaseError(HASE_E_INVALID, "forward must not be NULL");
+        HIP_CHECK(hipSetDevice(f->device));
+        hase::fwd::set_reflections(f, reflections);
     });
 }
 

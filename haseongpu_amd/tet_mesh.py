@@ -67,6 +67,20 @@ class TetMesh:
     def centroids(self):
         return self.points[self.cells].mean(axis=1)
 
+    def faceSurfacesByPlane(self, planes, *, atol=1e-12):
+        """Surface ids of the cell faces (C, 4), face k opposite vertex k: a
+        face whose three vertices lie on the plane z = z0 gets planes[z0]
+        (e.g. {0.0: 1, thickness_total: 2} for the bottom and top of an
+        extruded mesh), every other face 0 (no surface)."""
+        out = np.zeros((self.n_cells, 4), dtype=np.int32)
+        z = self.points[self.cells][:, :, 2]                     # (C, 4)
+        for z0, sid in planes.items():
+            on = np.abs(z - float(z0)) <= atol                   # vertex on the plane
+            for k in range(4):
+                others = [j for j in range(4) if j != k]
+                out[:, k] = np.where(on[:, others].all(axis=1), int(sid), out[:, k])
+        return out
+
     @classmethod
     def fromPrisms(cls, topology, beta_volume, *, cladding_types=None, cladding_number=3, cladding_absorption=0.0,
                    n_tot=2.76e20, tau=9.5e-4):

@@ -84,7 +84,7 @@ typedef struct hase_experiment {
     const double* sigma_e;          /* [W] emission cross sections, cm^2 */
     double max_sigma_a;             /* pair used by the importance map and dndtAse */
     double max_sigma_e;
-    double mse_threshold;           /* stop when the relative standard error <= this */
+    double mse_threshold;           /* stop when mse (the squared relative standard error) <= this */
     int32_t use_reflections;        /* top/bottom reflections by mirrored images */
     uint32_t max_reflections;       /* reflection order cap when use_reflections (0 -> 1) */
     double defensive_fraction;      /* share of the proposal drawn from beta*V, in [0, 1) */
@@ -434,8 +434,7 @@ void hase_simulation_destroy(hase_simulation_t* sim);
 
 /* ---------------------------------------------------------------------
  * Forward Tet4 estimator: the reference's present method
- * (include/core/calcForwardPhiAse.hpp:60-300, ForwardPhiAseContext), without
- * surface reflections.  Rays start at beta*V-stratified points of a
+ * (include/core/calcForwardPhiAse.hpp:60-300, ForwardPhiAseContext).  Rays start at beta*V-stratified points of a
  * tetrahedral gain medium, travel in isotropic directions until they leave
  * it, and score the track-length integral of their gain into the vertices of
  * every traversed cell; a cell's phiAse is the mean lumped-vertex density of
@@ -447,8 +446,10 @@ void hase_simulation_destroy(hase_simulation_t* sim);
  * evaluation, grown geometrically over compute.adaptive_steps as in
  * include/core/types.hpp:333-354), the wavelength tables (each ray takes a
  * stratified wavelength), max_sigma_a/e (dndtAse) and mse_threshold (stop
- * when every cell has relative_standard_error^2 <= it); use_reflections must
- * be 0.  Of hase_compute_t: adaptive_steps, rng_seed, device. */
+ * when every cell has relative_standard_error^2 <= it) and use_reflections
+ * (surface reservoir passes over the surfaces of
+ * hase_forward_set_reflections).  Of hase_compute_t: adaptive_steps,
+ * rng_seed, device. */
 typedef struct hase_tet_mesh {
     uint32_t number_of_points;
     uint32_t number_of_cells;
@@ -477,15 +478,46 @@ typedef struct hase_forward_result {
     uint32_t* convergence_rays;       /* ray target at which the cell first met the threshold, 0 = never */
 } hase_forward_result_t;
 
+/* Surface-reservoir pass outcome (SrmStatus, include/core/types.hpp:174-181). */
+enum hase_srm_status {
+    HASE_SRM_DISABLED = 0,
+    HASE_SRM_CONVERGED = 1,    /* remaining reflected weight < tolerance */
+    HASE_SRM_STABLE = 2,       /* remaining weight changed by < tolerance */
+    HASE_SRM_MAX_PASSES = 3,   /* pass limit reached */
+    HASE_SRM_DIVERGED = 4      /* weight grew divergence_streak passes in a row */
+};
+
 typedef struct hase_forward_stats {
-    double trace_ms;                /* summed device time of the trace kernel */
+    double trace_ms;                /* summed device time of the ray kernels */
     double total_ms;                /* wall time of the evaluation */
     uint64_t rays;
-    uint64_t segments;              /* scored cell segments */
+    uint64_t segments;              /* scored cell segments (primary and reflected rays) */
     uint64_t dropped_rays;
     uint32_t launches;              /* adaptive launches */
     uint32_t open_cells;            /* cells short of the threshold at the end */
+    uint32_t srm_status;            /* worst hase_srm_status over batches and launches */
+    uint32_t srm_passes;            /* most reflected passes of a batch */
+    double srm_remaining;           /* largest remaining weight fraction */
 } hase_forward_stats_t;
+
+/* Reflective surfaces of the Tet4 mesh for the surface reservoir method
+ * (include/core/forwardSrm.hpp, the mesh's cellFaceBoundaries and surface
+ * tables, include/core/mesh.hpp:300-332; controls of ExperimentParameters,
+ * types.hpp:328-330).  A boundary face (no neighbour) whose surface id is in
+ * (0, number_of_surfaces) reflects with that surface's reflectivity, or
+ * totally when (n_inside / n_outside)^2 sin^2(theta) > 1; other boundary faces
+ * absorb.  Used when the experiment's use_reflections is set. */
+typedef struct hase_forward_reflections {
+    const int32_t* face_boundaries; /* [4C] surface id of face k (opposite vertex k) of each cell */
+    uint32_t number_of_surfaces;    /* entries of the tables below (id 0 unused) */
+    uint32_t reservoir_slots;       /* surfaceReservoirSize, 32 */
+    const float* reflectivity;      /* [number_of_surfaces] */
+    const float* n_inside;          /* [number_of_surfaces] or NULL (no total reflection) */
+    const float* n_outside;         /* [number_of_surfaces] or NULL */
+    uint32_t max_passes;            /* reflectionMaxIterations, 40 */
+    uint32_t divergence_streak;     /* 3 */
+    double tolerance;               /* reflectionTolerance, 1e-4 */
+} hase_forward_reflections_t;
 
 typedef struct hase_forward hase_forward_t;
 
@@ -506,6 +538,9 @@ int hase_forward_evaluate(hase_forward_t* forward, hase_forward_result_t* result
  * cladding copies), visits and dropped [C], scored segments. */
 int hase_forward_trace(hase_forward_t* forward, uint32_t seed, uint32_t launch, uint32_t rays, double* scores,
                        uint32_t* ray_visits, uint32_t* dropped_rays, uint64_t* segments);
+
+/* Sets the reflective surfaces (the whole table is replaced). */
+int hase_forward_set_reflections(hase_forward_t* forward, const hase_forward_reflections_t* reflections);
 
 int hase_forward_stats(const hase_forward_t* forward, hase_forward_stats_t* stats);
 int hase_forward_destroy(hase_forward_t* forward);

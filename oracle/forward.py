@@ -31,6 +31,17 @@ class _Mesh(C.Structure):
     ]
 
 
+class _Surfaces(C.Structure):
+    _fields_ = [("n_faces", C.c_uint32), ("face", _u32p), ("face_index", _i32p), ("centroid", _f64p),
+                ("normal", _f64p), ("reflectivity", _f64p), ("n_in", _f64p), ("n_out", _f64p)]
+
+
+class Srm(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_passes", C.c_uint32), ("divergence_streak", C.c_uint32),
+                ("tolerance", C.c_double), ("status", C.c_int32 * BATCHES), ("passes", C.c_uint32 * BATCHES),
+                ("remaining", C.c_double * BATCHES)]
+
+
 class Hit(C.Structure):
     _fields_ = [("face", C.c_int32), ("length", C.c_double), ("tied_mask", C.c_uint32)]
 
@@ -78,6 +89,21 @@ def _load():
                                      C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, _f32p, _f64p, _f64p, _f64p,
                                      _u32p, _u32p, _u32p, _u64p, C.c_int]
         lib.orf_evaluate.restype = C.c_int
+        _f32p_ = C.POINTER(C.c_float)
+        lib.orf_surfaces_build.argtypes = [C.POINTER(_Mesh), _i32p, C.c_uint32, _f32p_, _f32p_, _f32p_,
+                                           C.POINTER(_Surfaces)]
+        lib.orf_surfaces_build.restype = C.c_int
+        lib.orf_surfaces_free.argtypes = [C.POINTER(_Surfaces)]
+        lib.orf_reflectance.argtypes = [C.POINTER(_Surfaces), C.c_uint32, _f64p]
+        lib.orf_reflectance.restype = C.c_double
+        lib.orf_trace_srm.argtypes = [C.POINTER(_Mesh), C.POINTER(_Surfaces), C.POINTER(Srm), _f64p, _f64p,
+                                      C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f64p, _u32p, _u32p, _u32p,
+                                      _u64p]
+        lib.orf_evaluate_srm.argtypes = [C.POINTER(_Mesh), C.POINTER(_Surfaces), C.POINTER(Srm), _f64p, _f64p,
+                                         C.c_uint32, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_double, C.c_uint32, _f32p, _f64p, _f64p, _f64p, _u32p, _u32p, _u32p,
+                                         _u64p]
+        lib.orf_evaluate_srm.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -190,6 +216,78 @@ class ForwardOracle:
                                         _p(conv, _u32p), C.byref(total), threads)
         return dict(phiAse=phi, standardError=err, relativeStandardError=rse, dndtAse=dndt, rayVisits=visits,
                     droppedRays=dropped, convergenceRays=conv, rays=int(total.value), launches=launches)
+
+
+class _Reflecting:
+    """Surfaces and reservoir controls of the oracle's surface reservoir method."""
+
+    def __init__(self, oracle, face_surface, reflectivity, n_in=None, n_out=None, *, slots=32, max_passes=40,
+                 divergence_streak=3, tolerance=1e-4):
+        self._fs = np.ascontiguousarray(face_surface, dtype=np.int32).reshape(-1)
+        self._refl = np.ascontiguousarray(reflectivity, dtype=np.float32)
+        self._nin = None if n_in is None else np.ascontiguousarray(n_in, dtype=np.float32)
+        self._nout = None if n_out is None else np.ascontiguousarray(n_out, dtype=np.float32)
+        self.surfaces = _Surfaces()
+        rc = _load().orf_surfaces_build(C.byref(oracle._m), _p(self._fs, _i32p), self._refl.size,
+                                        _p(self._refl, C.POINTER(C.c_float)), _p(self._nin, C.POINTER(C.c_float)),
+                                        _p(self._nout, C.POINTER(C.c_float)), C.byref(self.surfaces))
+        if rc != 0:
+            raise MemoryError("surface table")
+        self.srm = Srm()
+        self.srm.slots, self.srm.max_passes = int(slots), int(max_passes)
+        self.srm.divergence_streak, self.srm.tolerance = int(divergence_streak), float(tolerance)
+
+    @property
+    def n_faces(self):
+        return int(self.surfaces.n_faces)
+
+    def status(self):
+        return (list(self.srm.status), list(self.srm.passes), list(self.srm.remaining))
+
+    def __del__(self):
+        try:
+            _load().orf_surfaces_free(C.byref(self.surfaces))
+        except Exception:
+            pass
+
+
+def reflecting(oracle, face_surface, reflectivity, n_in=None, n_out=None, **controls):
+    return _Reflecting(oracle, face_surface, reflectivity, n_in, n_out, **controls)
+
+
+def trace_srm(oracle, refl, sigma_a, sigma_e, seed, launch, rays):
+    """Raw accumulators of one launch with reflections (single-threaded)."""
+    sa = np.ascontiguousarray(sigma_a, dtype=np.float64)
+    se = np.ascontiguousarray(sigma_e, dtype=np.float64)
+    mesh = oracle.mesh
+    scores = np.zeros(BATCHES * 2 * mesh.n_points)
+    visits = np.zeros(mesh.n_cells, np.uint32)
+    dropped = np.zeros(mesh.n_cells, np.uint32)
+    counts = np.zeros(BATCHES, np.uint32)
+    seg = C.c_uint64(0)
+    _load().orf_trace_srm(C.byref(oracle._m), C.byref(refl.surfaces), C.byref(refl.srm), _p(sa, _f64p),
+                          _p(se, _f64p), sa.size, seed, launch, rays, _p(scores, _f64p), _p(visits, _u32p),
+                          _p(dropped, _u32p), _p(counts, _u32p), C.byref(seg))
+    return scores.reshape(BATCHES, -1), visits, dropped, counts, int(seg.value)
+
+
+def evaluate_srm(oracle, refl, sigma_a, sigma_e, *, max_sigma_a=None, max_sigma_e=None, min_rays, max_rays, steps=4,
+                 threshold=0.005, seed=5489):
+    sa = np.ascontiguousarray(sigma_a, dtype=np.float64).reshape(-1)
+    se = np.ascontiguousarray(sigma_e, dtype=np.float64).reshape(-1)
+    msa = float(sa.max() if max_sigma_a is None else max_sigma_a)
+    mse = float(se.max() if max_sigma_e is None else max_sigma_e)
+    n = oracle.mesh.n_cells
+    phi, err, rse, dndt = np.zeros(n, np.float32), np.zeros(n), np.zeros(n), np.zeros(n)
+    visits, dropped, conv = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    total = C.c_uint64(0)
+    launches = _load().orf_evaluate_srm(C.byref(oracle._m), C.byref(refl.surfaces), C.byref(refl.srm),
+                                        _p(sa, _f64p), _p(se, _f64p), sa.size, msa, mse, min_rays, max_rays, steps,
+                                        threshold, seed, _p(phi, _f32p), _p(err, _f64p), _p(rse, _f64p),
+                                        _p(dndt, _f64p), _p(visits, _u32p), _p(dropped, _u32p), _p(conv, _u32p),
+                                        C.byref(total))
+    return dict(phiAse=phi, standardError=err, relativeStandardError=rse, dndtAse=dndt, rayVisits=visits,
+                droppedRays=dropped, convergenceRays=conv, rays=int(total.value), launches=launches)
 
 
 def point_in_tet(u3):

@@ -233,12 +233,17 @@ static inline double orf_gain_coefficient(const orf_mesh* m, uint32_t cell, doub
     return m->n_tot * (m->beta[cell] * (se + sa) - sa);
 }
 
-int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
-             const double u[3], double* scores, uint32_t* visits, uint32_t* dropped, uint64_t* segments)
+int orf_walk_from(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
+                  const double u[3], int32_t forbidden, double weight, double* scores, uint32_t* visits,
+                  uint32_t* dropped, uint64_t* segments, orf_exit* out)
 {
     double p[3] = {p0[0], p0[1], p0[2]};
-    int32_t forbidden = -1;
     double gain = 1.0;
+    if (out) {
+        out->cell = cell;
+        out->face = -1;
+        out->weight = 0.0;
+    }
     for (uint32_t step = 0; step < ORF_MAX_STEPS; ++step) {
         orf_hit h = orf_next_face(m, cell, p, u, forbidden);
         if (h.face < 0) {
@@ -260,7 +265,14 @@ int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, c
             if (face < 0) { dropped[cell] += 1; return 1; }
             orf_transition t = orf_recover(m, cell, face, p, u);
             if (t.status == ORF_FAILED) { dropped[t.cell] += 1; return 1; }
-            if (t.status == ORF_BOUNDARY) return 0;
+            if (t.status == ORF_BOUNDARY) {
+                if (out) {
+                    out->cell = t.cell;
+                    out->face = t.boundary_face;
+                    out->weight = weight * gain;
+                }
+                return 0;
+            }
             cell = t.cell;
             forbidden = t.forbidden;
             continue;
@@ -271,7 +283,7 @@ int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, c
         const double g = orf_gain_coefficient(m, cell, sigma_a, sigma_e);
         const double gl = g * h.length;
         const double em1 = expm1(gl); /* track integral and gain factor */
-        const double contribution = gain * (gl == 0.0 ? h.length : em1 / g);
+        const double contribution = weight * gain * (gl == 0.0 ? h.length : em1 / g);
         if (isfinite(contribution)) {
             visits[cell] += 1;
             *segments += 1;
@@ -300,12 +312,25 @@ int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, c
         p[2] = p[2] + u[2] * h.length;
         orf_transition t = orf_cross(m, cell, h, p, u);
         if (t.status == ORF_FAILED) { dropped[t.cell] += 1; return 1; }
-        if (t.status == ORF_BOUNDARY) return 0;
+        if (t.status == ORF_BOUNDARY) {
+            if (out) {
+                out->cell = t.cell;
+                out->face = t.boundary_face;
+                out->weight = weight * gain;
+            }
+            return 0;
+        }
         cell = t.cell;
         forbidden = t.forbidden;
     }
     dropped[cell] += 1;
     return 1;
+}
+
+int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
+             const double u[3], double* scores, uint32_t* visits, uint32_t* dropped, uint64_t* segments)
+{
+    return orf_walk_from(m, sigma_a, sigma_e, cell, p0, u, -1, 1.0, scores, visits, dropped, segments, NULL);
 }
 
 /* ------------------------------------------------------------ ray starts */
@@ -520,10 +545,38 @@ void orf_finalize(const orf_mesh* m, const double* scores, const uint32_t* batch
     }
 }
 
+static int orf_evaluate_impl(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                             const double* sigma_e, uint32_t n_lambda, double max_sigma_a, double max_sigma_e,
+                             uint32_t min_rays, uint32_t max_rays, uint32_t steps, double threshold, uint32_t seed,
+                             float* phi, double* se, double* rse, double* dndt, uint32_t* visits, uint32_t* dropped,
+                             uint32_t* convergence_rays, uint64_t* total_rays, int threads);
+
 int orf_evaluate(const orf_mesh* m, const double* sigma_a, const double* sigma_e, uint32_t n_lambda,
                  double max_sigma_a, double max_sigma_e, uint32_t min_rays, uint32_t max_rays, uint32_t steps,
                  double threshold, uint32_t seed, float* phi, double* se, double* rse, double* dndt,
                  uint32_t* visits, uint32_t* dropped, uint32_t* convergence_rays, uint64_t* total_rays, int threads)
+{
+    return orf_evaluate_impl(m, NULL, NULL, sigma_a, sigma_e, n_lambda, max_sigma_a, max_sigma_e, min_rays,
+                             max_rays, steps, threshold, seed, phi, se, rse, dndt, visits, dropped,
+                             convergence_rays, total_rays, threads);
+}
+
+int orf_evaluate_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                     const double* sigma_e, uint32_t n_lambda, double max_sigma_a, double max_sigma_e,
+                     uint32_t min_rays, uint32_t max_rays, uint32_t steps, double threshold, uint32_t seed,
+                     float* phi, double* se, double* rse, double* dndt, uint32_t* visits, uint32_t* dropped,
+                     uint32_t* convergence_rays, uint64_t* total_rays)
+{
+    return orf_evaluate_impl(m, sf, srm, sigma_a, sigma_e, n_lambda, max_sigma_a, max_sigma_e, min_rays, max_rays,
+                             steps, threshold, seed, phi, se, rse, dndt, visits, dropped, convergence_rays,
+                             total_rays, 1);
+}
+
+static int orf_evaluate_impl(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                             const double* sigma_e, uint32_t n_lambda, double max_sigma_a, double max_sigma_e,
+                             uint32_t min_rays, uint32_t max_rays, uint32_t steps, double threshold, uint32_t seed,
+                             float* phi, double* se, double* rse, double* dndt, uint32_t* visits, uint32_t* dropped,
+                             uint32_t* convergence_rays, uint64_t* total_rays, int threads)
 {
     const size_t nv = 2 * (size_t)m->n_points;
     double* scores = (double*)calloc(ORF_BATCHES * nv, sizeof(double));
@@ -537,8 +590,12 @@ int orf_evaluate(const orf_mesh* m, const double* sigma_a, const double* sigma_e
     for (uint32_t k = 0;; ++k) {
         const uint32_t target = orc_adaptive_target(min_rays, max_rays, steps, k);
         const uint32_t launch_rays = (uint32_t)(target - done);
-        orf_trace(m, sigma_a, sigma_e, n_lambda, seed, (uint32_t)launches, launch_rays, scores, visits, dropped,
-                  counts, NULL, threads);
+        if (sf)
+            orf_trace_srm(m, sf, srm, sigma_a, sigma_e, n_lambda, seed, (uint32_t)launches, launch_rays, scores,
+                          visits, dropped, counts, NULL);
+        else
+            orf_trace(m, sigma_a, sigma_e, n_lambda, seed, (uint32_t)launches, launch_rays, scores, visits, dropped,
+                      counts, NULL, threads);
         done = target;
         ++launches;
         orf_finalize(m, scores, counts, done, dropped, max_sigma_a, max_sigma_e, phi, se, rse, dndt);
@@ -553,4 +610,339 @@ int orf_evaluate(const orf_mesh* m, const double* sigma_a, const double* sigma_e
     *total_rays = done;
     free(scores);
     return launches;
+}
+
+/* ------------------------------------------------------------ reflections */
+static void orf_normalize(double v[3])
+{
+    const double len = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len <= DBL_EPSILON) {
+        v[0] = v[1] = v[2] = 0.0;
+        return;
+    }
+    const double inv = 1.0 / len;
+    v[0] = v[0] * inv;
+    v[1] = v[1] * inv;
+    v[2] = v[2] * inv;
+}
+
+int orf_surfaces_build(const orf_mesh* m, const int32_t* face_surface, uint32_t n_surfaces,
+                       const float* reflectivity, const float* n_in, const float* n_out, orf_surfaces* out)
+{
+    memset(out, 0, sizeof(*out));
+    const uint32_t n = m->n_cells;
+    out->face_index = (int32_t*)malloc(4 * (size_t)n * sizeof(int32_t));
+    if (!out->face_index) return -3;
+    uint32_t count = 0;
+    for (size_t i = 0; i < 4 * (size_t)n; ++i) {
+        const int32_t id = face_surface ? face_surface[i] : 0;
+        const int ok = m->nbr[i] < 0 && id > 0 && (uint32_t)id < n_surfaces;
+        out->face_index[i] = ok ? (int32_t)count++ : -1;
+    }
+    out->n_faces = count;
+    out->face = (uint32_t*)malloc((count ? count : 1) * sizeof(uint32_t));
+    out->centroid = (double*)malloc((count ? count : 1) * 3 * sizeof(double));
+    out->normal = (double*)malloc((count ? count : 1) * 3 * sizeof(double));
+    out->reflectivity = (double*)malloc((count ? count : 1) * sizeof(double));
+    out->n_in = (double*)malloc((count ? count : 1) * sizeof(double));
+    out->n_out = (double*)malloc((count ? count : 1) * sizeof(double));
+    if (!out->face || !out->centroid || !out->normal || !out->reflectivity || !out->n_in || !out->n_out) return -3;
+    for (size_t i = 0; i < 4 * (size_t)n; ++i) {
+        const int32_t f = out->face_index[i];
+        if (f < 0) continue;
+        const uint32_t c = (uint32_t)(i / 4), k = (uint32_t)(i % 4);
+        const uint32_t* cv = m->cells + 4 * (size_t)c;
+        uint32_t fv[3], j = 0;
+        for (uint32_t q = 0; q < 4; ++q)
+            if (q != k) fv[j++] = cv[q];
+        orf_sort3(fv);
+        /* face centroid and outward normal (rayWalk.hpp:42-84) */
+        double cen[3], ctr[3];
+        cen[0] = (m->px[fv[0]] + m->px[fv[1]] + m->px[fv[2]]) * (1.0 / 3.0);
+        cen[1] = (m->py[fv[0]] + m->py[fv[1]] + m->py[fv[2]]) * (1.0 / 3.0);
+        cen[2] = (m->pz[fv[0]] + m->pz[fv[1]] + m->pz[fv[2]]) * (1.0 / 3.0);
+        ctr[0] = (m->px[cv[0]] + m->px[cv[1]] + m->px[cv[2]] + m->px[cv[3]]) * 0.25;
+        ctr[1] = (m->py[cv[0]] + m->py[cv[1]] + m->py[cv[2]] + m->py[cv[3]]) * 0.25;
+        ctr[2] = (m->pz[cv[0]] + m->pz[cv[1]] + m->pz[cv[2]] + m->pz[cv[3]]) * 0.25;
+        const double bx = m->px[fv[1]] - m->px[fv[0]], by = m->py[fv[1]] - m->py[fv[0]],
+                     bz = m->pz[fv[1]] - m->pz[fv[0]];
+        const double cx = m->px[fv[2]] - m->px[fv[0]], cy = m->py[fv[2]] - m->py[fv[0]],
+                     cz = m->pz[fv[2]] - m->pz[fv[0]];
+        double nrm[3] = {by * cz - bz * cy, bz * cx - bx * cz, bx * cy - by * cx};
+        orf_normalize(nrm);
+        if (nrm[0] * (ctr[0] - cen[0]) + nrm[1] * (ctr[1] - cen[1]) + nrm[2] * (ctr[2] - cen[2]) > 0.0) {
+            nrm[0] = -nrm[0];
+            nrm[1] = -nrm[1];
+            nrm[2] = -nrm[2];
+        }
+        const int32_t id = face_surface[i];
+        out->face[f] = (uint32_t)i;
+        memcpy(out->centroid + 3 * (size_t)f, cen, sizeof(cen));
+        memcpy(out->normal + 3 * (size_t)f, nrm, sizeof(nrm));
+        out->reflectivity[f] = reflectivity ? (double)reflectivity[id] : 0.0;
+        out->n_in[f] = n_in ? (double)n_in[id] : 0.0;
+        out->n_out[f] = n_out ? (double)n_out[id] : 0.0;
+    }
+    return 0;
+}
+
+void orf_surfaces_free(orf_surfaces* s)
+{
+    free(s->face);
+    free(s->face_index);
+    free(s->centroid);
+    free(s->normal);
+    free(s->reflectivity);
+    free(s->n_in);
+    free(s->n_out);
+    memset(s, 0, sizeof(*s));
+}
+
+/* boundaryReflectance (include/kernels/reflection.hpp:21-47): total internal
+ * reflection when (n_in / n_out)^2 sin^2(theta) > 1, else the reflectivity. */
+double orf_reflectance(const orf_surfaces* s, uint32_t f, const double dir[3])
+{
+    const double nin = s->n_in[f], nout = s->n_out[f];
+    if (nin > 0.0 && nout > 0.0) {
+        double d[3] = {dir[0], dir[1], dir[2]};
+        orf_normalize(d);
+        const double* nr = s->normal + 3 * (size_t)f;
+        const double cosi = fmin(1.0, fabs(d[0] * nr[0] + d[1] * nr[1] + d[2] * nr[2]));
+        const double sin2 = fmax(0.0, 1.0 - cosi * cosi);
+        const double ratio = nin / nout;
+        if (ratio * ratio * sin2 > 1.0) return 1.0;
+    }
+    return fmax(0.0, s->reflectivity[f]);
+}
+
+typedef struct {
+    uint32_t* count;       /* per face */
+    double* face_weight;   /* per face */
+    double* dir;           /* 3 per slot */
+    double* weight;        /* per slot */
+    uint32_t* sigma;       /* per slot */
+} orf_reservoir;
+
+static int orf_reservoir_alloc(orf_reservoir* r, size_t faces, size_t slots)
+{
+    const size_t fs = faces * slots > 0 ? faces * slots : 1, nf = faces > 0 ? faces : 1;
+    r->count = (uint32_t*)calloc(nf, sizeof(uint32_t));
+    r->face_weight = (double*)calloc(nf, sizeof(double));
+    r->dir = (double*)calloc(fs * 3, sizeof(double));
+    r->weight = (double*)calloc(fs, sizeof(double));
+    r->sigma = (uint32_t*)calloc(fs, sizeof(uint32_t));
+    return (r->count && r->face_weight && r->dir && r->weight && r->sigma) ? 0 : -3;
+}
+
+static void orf_reservoir_free(orf_reservoir* r)
+{
+    free(r->count);
+    free(r->face_weight);
+    free(r->dir);
+    free(r->weight);
+    free(r->sigma);
+}
+
+/* depositReflection (accumulation.hpp:409-455): weighted reservoir sampling
+ * of the reflected direction; u0 decides a replacement, u1 picks the slot. */
+static void orf_deposit(const orf_surfaces* sf, orf_reservoir* r, uint32_t slots, uint32_t f, const double dir[3],
+                        double w, uint32_t sigma, double u0, double u1)
+{
+    if (slots == 0 || !(w > 0.0) || !isfinite(w)) return;
+    const double rw = w * orf_reflectance(sf, f, dir);
+    if (!(rw > 0.0) || !isfinite(rw)) return;
+    const uint32_t seen = r->count[f]++;
+    const double prior = r->face_weight[f];
+    r->face_weight[f] = prior + rw;
+    uint32_t slot = seen;
+    if (slot >= slots) {
+        const double keep = fmin(1.0, (double)slots * rw / (prior + rw));
+        if (u0 >= keep) return;
+        slot = (uint32_t)(u1 * (double)slots);
+        if (slot >= slots) slot = slots - 1;
+    }
+    const size_t idx = (size_t)f * slots + slot;
+    const double* nr = sf->normal + 3 * (size_t)f;
+    const double dn = 2.0 * (dir[0] * nr[0] + dir[1] * nr[1] + dir[2] * nr[2]);
+    double out[3] = {dir[0] - nr[0] * dn, dir[1] - nr[1] * dn, dir[2] - nr[2] * dn};
+    orf_normalize(out);
+    r->dir[3 * idx] = out[0];
+    r->dir[3 * idx + 1] = out[1];
+    r->dir[3 * idx + 2] = out[2];
+    r->weight[idx] = rw;
+    r->sigma[idx] = sigma;
+}
+
+static void orf_srm_uniforms(uint32_t seed, uint32_t launch, uint32_t ray, uint32_t pass, uint32_t use,
+                             double* u0, double* u1)
+{
+    const uint32_t key[2] = {seed, ORF_KEY_TAG};
+    const uint32_t ctr[4] = {ray, launch, 0x10000u + pass, use};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    *u0 = orc_u32_to_unit(o[0]);
+    *u1 = orc_u32_to_unit(o[1]);
+}
+
+static double orf_srm_shift(uint32_t seed, uint32_t launch, uint32_t batch, uint32_t pass)
+{
+    const uint32_t key[2] = {seed, ORF_KEY_TAG};
+    const uint32_t ctr[4] = {0xFFFFFFFEu, batch, launch, pass};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    return orc_u32_to_unit(o[0]);
+}
+
+void orf_trace_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                   const double* sigma_e, uint32_t n_lambda, uint32_t seed, uint32_t launch, uint32_t rays,
+                   double* scores, uint32_t* visits, uint32_t* dropped, uint32_t* batch_counts, uint64_t* segments)
+{
+    const size_t nv = 2 * (size_t)m->n_points, nb = sf->n_faces, S = srm->slots;
+    for (uint32_t b = 0; b < ORF_BATCHES; ++b) {
+        uint32_t first, count;
+        orf_batch_range(rays, b, &first, &count);
+        batch_counts[b] += count;
+        srm->status[b] = ORF_SRM_CONVERGED;
+        srm->passes[b] = 0;
+        srm->remaining[b] = 0.0;
+    }
+    if (rays == 0 || m->bv_prefix[m->n_cells - 1] <= 0.0) return;
+    orf_reservoir res[2][ORF_BATCHES];
+    for (int side = 0; side < 2; ++side)
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) orf_reservoir_alloc(&res[side][b], nb, S);
+    uint64_t seg = 0;
+    /* pass 0: primary rays, reflections into side 0 */
+    for (uint32_t r = 0; r < rays; ++r) {
+        orf_ray ray;
+        orf_ray_start(m, seed, launch, rays, r, n_lambda, &ray);
+        orf_exit ex;
+        orf_walk_from(m, sigma_a[ray.lambda], sigma_e[ray.lambda], ray.cell, ray.p, ray.u, -1, 1.0,
+                      scores + ray.batch * nv, visits, dropped, &seg, &ex);
+        if (ex.face >= 0) {
+            const int32_t f = sf->face_index[4 * (size_t)ex.cell + ex.face];
+            if (f >= 0) {
+                double u0, u1;
+                orf_srm_uniforms(seed, launch, r, 0, 1, &u0, &u1);
+                orf_deposit(sf, &res[0][ray.batch], (uint32_t)S, (uint32_t)f, ray.u, ex.weight, ray.lambda, u0, u1);
+            }
+        }
+    }
+    double initial[ORF_BATCHES], previous[ORF_BATCHES];
+    uint32_t grow[ORF_BATCHES];
+    int active[ORF_BATCHES];
+    for (uint32_t b = 0; b < ORF_BATCHES; ++b) {
+        double w = 0.0;
+        for (size_t f = 0; f < nb; ++f) w += res[0][b].face_weight[f];
+        initial[b] = previous[b] = w;
+        grow[b] = 0;
+        active[b] = w > 0.0;
+        srm->status[b] = active[b] ? ORF_SRM_MAX_PASSES : ORF_SRM_CONVERGED;
+        srm->remaining[b] = active[b] ? 1.0 : 0.0;
+    }
+    double* cdf = (double*)malloc((nb ? nb : 1) * sizeof(double));
+    int in = 0;
+    for (uint32_t pass = 1; pass <= srm->max_passes; ++pass) {
+        int any = 0;
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) any |= active[b];
+        if (!any) break;
+        const int out = 1 - in;
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) {
+            memset(res[out][b].count, 0, (nb ? nb : 1) * sizeof(uint32_t));
+            memset(res[out][b].face_weight, 0, (nb ? nb : 1) * sizeof(double));
+        }
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) {
+            if (!active[b]) continue;
+            uint32_t first, count;
+            orf_batch_range(rays, b, &first, &count);
+            if (count == 0) continue;
+            const orf_reservoir* src = &res[in][b];
+            double run = 0.0;
+            for (size_t f = 0; f < nb; ++f) {
+                run += src->face_weight[f];
+                cdf[f] = run;
+            }
+            for (size_t f = 0; f < nb; ++f) cdf[f] = run > 0.0 ? cdf[f] / run : 0.0;
+            const double source = previous[b] / (double)count;
+            const double shift = orf_srm_shift(seed, launch, b, pass);
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t r = first + i;
+                double u0, u1;
+                orf_srm_uniforms(seed, launch, r, pass, 2, &u0, &u1);
+                /* face by reservoir weight, stratified when every face can get a ray
+                 * (accumulation.hpp:706-746) */
+                const double target = nb <= count ? ((double)i + shift) / (double)count : u0;
+                uint32_t lo = 0, hi = (uint32_t)nb;
+                while (lo < hi) {
+                    const uint32_t mid = lo + (hi - lo) / 2u;
+                    if (cdf[mid] <= target) lo = mid + 1u;
+                    else hi = mid;
+                }
+                const uint32_t f = lo < nb ? lo : (uint32_t)nb - 1u;
+                const uint32_t filled = src->count[f] < S ? src->count[f] : (uint32_t)S;
+                if (filled == 0) continue;
+                double slot_weight = 0.0;
+                for (uint32_t k = 0; k < filled; ++k) slot_weight += src->weight[(size_t)f * S + k];
+                if (!(slot_weight > 0.0)) continue;
+                const double t2 = u1 * slot_weight;
+                double cum = 0.0;
+                uint32_t pick = filled - 1u;
+                for (uint32_t k = 0; k < filled; ++k) {
+                    cum += src->weight[(size_t)f * S + k];
+                    if (cum >= t2) {
+                        pick = k;
+                        break;
+                    }
+                }
+                const size_t idx = (size_t)f * S + pick;
+                double dir[3] = {src->dir[3 * idx], src->dir[3 * idx + 1], src->dir[3 * idx + 2]};
+                orf_normalize(dir);
+                const uint32_t lam = src->sigma[idx];
+                const uint32_t face = sf->face[f];
+                orf_exit ex;
+                orf_walk_from(m, sigma_a[lam], sigma_e[lam], face / 4, sf->centroid + 3 * (size_t)f, dir,
+                              (int32_t)(face % 4), source, scores + b * nv, visits, dropped, &seg, &ex);
+                if (ex.face >= 0) {
+                    const int32_t g = sf->face_index[4 * (size_t)ex.cell + ex.face];
+                    if (g >= 0) {
+                        double v0, v1;
+                        orf_srm_uniforms(seed, launch, r, pass, 1, &v0, &v1);
+                        orf_deposit(sf, &res[out][b], (uint32_t)S, (uint32_t)g, dir, ex.weight, lam, v0, v1);
+                    }
+                }
+            }
+        }
+        /* stop rules (forwardSrm.hpp:322-361) */
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) {
+            if (!active[b]) continue;
+            double w = 0.0;
+            for (size_t f = 0; f < nb; ++f) w += res[out][b].face_weight[f];
+            srm->passes[b] = pass;
+            srm->remaining[b] = w / initial[b];
+            if (w > previous[b]) {
+                if (++grow[b] >= srm->divergence_streak) {
+                    srm->status[b] = ORF_SRM_DIVERGED;
+                    active[b] = 0;
+                    continue;
+                }
+            } else {
+                grow[b] = 0;
+                if (fabs(w - previous[b]) / fmax(w, 1.0e-30) < srm->tolerance) {
+                    srm->status[b] = ORF_SRM_STABLE;
+                    active[b] = 0;
+                    continue;
+                }
+            }
+            if (srm->remaining[b] < srm->tolerance) {
+                srm->status[b] = ORF_SRM_CONVERGED;
+                active[b] = 0;
+                continue;
+            }
+            previous[b] = w;
+        }
+        in = out;
+    }
+    free(cdf);
+    for (int side = 0; side < 2; ++side)
+        for (uint32_t b = 0; b < ORF_BATCHES; ++b) orf_reservoir_free(&res[side][b]);
+    if (segments) *segments += seg;
 }

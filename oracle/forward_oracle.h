@@ -85,6 +85,12 @@ typedef struct {
     uint32_t batch_count;
 } orf_ray;
 
+typedef struct {
+    uint32_t cell;
+    int32_t face;      /* boundary face the walk left through, -1 if it did not */
+    double weight;     /* source weight times accumulated gain there */
+} orf_exit;
+
 int orf_prepare(orf_mesh* m);
 void orf_set_beta(orf_mesh* m, const double* beta);
 
@@ -106,6 +112,12 @@ void orf_ray_start(const orf_mesh* m, uint32_t seed, uint32_t launch, uint32_t r
 int orf_walk(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
              const double u[3], double* scores, uint32_t* visits, uint32_t* dropped, uint64_t* segments);
 
+/* Walk from a point with a forbidden entry face and a source weight; on
+ * leaving the mesh `out` names the boundary face and the incident weight. */
+int orf_walk_from(const orf_mesh* m, double sigma_a, double sigma_e, uint32_t cell, const double p0[3],
+                  const double u[3], int32_t forbidden, double weight, double* scores, uint32_t* visits,
+                  uint32_t* dropped, uint64_t* segments, orf_exit* out);
+
 /* One launch of `rays` rays: accumulates into scores [8 * 2 * n_points],
  * visits and dropped [n_cells]; batch_counts [8] gain each batch's rays. */
 void orf_trace(const orf_mesh* m, const double* sigma_a, const double* sigma_e, uint32_t n_lambda, uint32_t seed,
@@ -123,5 +135,55 @@ int orf_evaluate(const orf_mesh* m, const double* sigma_a, const double* sigma_e
                  double threshold, uint32_t seed, float* phi, double* se, double* rse, double* dndt,
                  uint32_t* visits, uint32_t* dropped, uint32_t* convergence_rays, uint64_t* total_rays,
                  int threads);
+
+/* ---------------------------------------------------------- reflections
+ * Surface reservoir method (include/core/forwardSrm.hpp:120-366,
+ * accumulation.hpp:407-834): rays that leave through a reflective boundary
+ * face leave their reflected direction and weight in that face's weighted
+ * reservoir (per RSE batch); each later pass draws faces by reservoir weight
+ * and slots by slot weight and restarts from the face centroid, until the
+ * remaining weight falls below the tolerance, stops changing or grows
+ * `divergence_streak` passes in a row. */
+enum { ORF_SRM_DISABLED = 0, ORF_SRM_CONVERGED = 1, ORF_SRM_STABLE = 2, ORF_SRM_MAX_PASSES = 3,
+       ORF_SRM_DIVERGED = 4 };
+
+typedef struct {
+    uint32_t n_faces;          /* reflective boundary faces (surface id > 0, no neighbour) */
+    uint32_t* face;            /* cell * 4 + local face */
+    int32_t* face_index;       /* 4 per cell: reflective face index or -1 */
+    double* centroid;          /* 3 per face */
+    double* normal;            /* 3 per face, outward unit */
+    double* reflectivity;      /* per face */
+    double* n_in;              /* per face, 0 = no total-internal-reflection test */
+    double* n_out;
+} orf_surfaces;
+
+typedef struct {
+    uint32_t slots;            /* reservoir slots per face */
+    uint32_t max_passes;
+    uint32_t divergence_streak;
+    double tolerance;
+    /* results of the last launch, per batch */
+    int32_t status[ORF_BATCHES];
+    uint32_t passes[ORF_BATCHES];
+    double remaining[ORF_BATCHES];
+} orf_srm;
+
+int orf_surfaces_build(const orf_mesh* m, const int32_t* face_surface, uint32_t n_surfaces,
+                       const float* reflectivity, const float* n_in, const float* n_out, orf_surfaces* out);
+void orf_surfaces_free(orf_surfaces* s);
+double orf_reflectance(const orf_surfaces* s, uint32_t f, const double dir[3]);
+
+/* One launch with reflections (single-threaded, rays in order): primary rays,
+ * then reflected passes per batch. */
+void orf_trace_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                   const double* sigma_e, uint32_t n_lambda, uint32_t seed, uint32_t launch, uint32_t rays,
+                   double* scores, uint32_t* visits, uint32_t* dropped, uint32_t* batch_counts, uint64_t* segments);
+
+int orf_evaluate_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
+                     const double* sigma_e, uint32_t n_lambda, double max_sigma_a, double max_sigma_e,
+                     uint32_t min_rays, uint32_t max_rays, uint32_t steps, double threshold, uint32_t seed,
+                     float* phi, double* se, double* rse, double* dndt, uint32_t* visits, uint32_t* dropped,
+                     uint32_t* convergence_rays, uint64_t* total_rays);
 
 #endif

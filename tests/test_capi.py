@@ -40,7 +40,8 @@ def test_struct_layouts_match_header(tmp_path):
                "hase_proposal_stats_t": _native.ProposalStats, "hase_tuning_t": _native.Tuning,
                "hase_run_control_t": _native.RunControl, "hase_time_state_t": _native.TimeState,
                "hase_tet_mesh_t": _native.TetMeshC, "hase_forward_result_t": _native.ForwardResultC,
-               "hase_forward_stats_t": _native.ForwardStats}
+               "hase_forward_stats_t": _native.ForwardStats,
+               "hase_forward_reflections_t": _native.ForwardReflections}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():
@@ -208,9 +209,6 @@ def test_forward_host_rejections():
     pts = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
     mesh = TetMesh(points=pts, cells=[[0, 1, 2, 3]], beta=[0.1])
     comp = _native.make_compute(n_samples=1)
-    refl = _native.NativeExperiment(sigma_a=[1e-21], sigma_e=[2e-20], min_rays=10, max_rays=10, use_reflections=True)
-    with pytest.raises(_native.HaseError, match="surface reservoir"):
-        _native.ForwardContext(mesh, refl, comp)
     ok = _native.NativeExperiment(sigma_a=[1e-21], sigma_e=[2e-20], min_rays=10, max_rays=10)
     bad = _native.NativeTetMesh(mesh)
     bad.cells[3] = 9

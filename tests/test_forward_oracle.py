@@ -220,3 +220,78 @@ def test_adaptive_loop_targets_and_convergence_record():
     # a fixed ray count (min == max) is a single launch
     one = o.evaluate([SA], [SE], min_rays=3000, max_rays=3000, steps=4, seed=1)
     assert one["rays"] == 3000 and one["launches"] == 1
+
+
+# --------------------------------------------------------------- reflections
+def _slab(nx=8, levels=9, z=0.5, beta=SA / (SA + SE)):
+    mesh, topo = _box(nx=nx, levels=levels, z=z, beta=beta)
+    return mesh, mesh.faceSurfacesByPlane({0.0: 1, z: 2})
+
+
+def test_zero_reflectivity_reservoir_equals_plain():
+    mesh, fs = _slab()
+    o = F.ForwardOracle(mesh)
+    refl = F.reflecting(o, fs, [0.0, 0.0, 0.0])
+    a = F.trace_srm(o, refl, [SA], [SE], 3, 0, 40_000)
+    b = o.trace([SA], [SE], 3, 0, 40_000, threads=1)
+    assert refl.n_faces == 2 * 2 * 8 * 8              # two triangles per grid cell, top and bottom
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[0], b[0])
+    assert refl.status()[0] == [1] * 8 and refl.status()[1] == [0] * 8   # converged without a pass
+
+
+def test_first_order_reflections_match_the_image_method():
+    """One reservoir pass with reflectivity 0.4 (bottom) and 0.2 (top), zero
+    gain: the centre cells carry the first-order image-method flux of the
+    angular quadrature averaged by the vertex hat functions (the reflected
+    rays restart at face centroids, which the tolerance allows for)."""
+    z, beta = 0.5, SA / (SA + SE)
+    mesh, fs = _slab(z=z, beta=beta)
+    o = F.ForwardOracle(mesh)
+    refl = F.reflecting(o, fs, [0.0, 0.4, 0.2], max_passes=1)
+    res = F.evaluate_srm(o, refl, [SA], [SE], min_rays=400_000, max_rays=400_000, steps=0, seed=3)
+    assert refl.status()[0] == [3] * 8 and refl.status()[1] == [1] * 8     # stopped at the pass limit
+    cen = mesh.centroids()
+    near = np.nonzero(np.linalg.norm((cen - [0.5, 0.5, z / 2]) / [1, 1, z], axis=1) < 0.12)[0]
+
+    def flux(p):
+        return box_point_flux(p, (0.0, 1.0, 0.0, 1.0), z, beta, 0.0, NTOT / TAU, reflect=True, refl_bottom=0.4,
+                              refl_top=0.2, n_mu=200, n_phi=400)
+
+    with np.errstate(invalid="ignore"):
+        expect = hat_smoothed_cell_flux(mesh.points, mesh.cells, o.volume, near, flux)
+    phi = res["phiAse"][near].astype(np.float64)
+    assert abs(phi.sum() / expect.sum() - 1.0) < 0.008, phi.sum() / expect.sum()
+
+
+def test_reflection_passes_converge_and_add_flux():
+    mesh, fs = _slab(nx=4, levels=5, z=0.4)
+    o = F.ForwardOracle(mesh)
+    kw = dict(min_rays=40_000, max_rays=40_000, steps=0, seed=5)
+    plain = o.evaluate([SA], [SE], **kw)["phiAse"].astype(np.float64)
+    one = F.evaluate_srm(o, F.reflecting(o, fs, [0.0, 0.6, 0.6], max_passes=1), [SA], [SE], **kw)
+    many_r = F.reflecting(o, fs, [0.0, 0.6, 0.6], max_passes=40, tolerance=1e-3)
+    many = F.evaluate_srm(o, many_r, [SA], [SE], **kw)
+    status, passes, remaining = many_r.status()
+    assert all(s in (1, 2) for s in status) and min(passes) > 1
+    assert all(r < 1e-3 for s, r in zip(status, remaining) if s == 1)
+    assert one["phiAse"].sum() > plain.sum() and many["phiAse"].sum() > one["phiAse"].sum()
+    # the primary rays are the plain run's: their visits are all there
+    assert np.all(many["rayVisits"] >= o.evaluate([SA], [SE], **kw)["rayVisits"])
+
+
+def test_total_internal_reflection_only_beyond_the_critical_angle():
+    mesh, fs = _slab(nx=2, levels=3, z=0.2)
+    o = F.ForwardOracle(mesh)
+    refl = F.reflecting(o, fs, [0.0, 0.0, 0.0], n_in=[1.0, 1.83, 1.83], n_out=[1.0, 1.0, 1.0])
+    crit = math.asin(1.0 / 1.83)
+    f = 0
+    normal = refl.surfaces.normal[0:3]
+    for theta, expect in ((crit - 0.05, 0.0), (crit + 0.05, 1.0), (0.0, 0.0), (1.5, 1.0)):
+        # a direction at angle theta to the outward normal of face 0
+        n = np.array(normal)
+        t = np.cross(n, [1.0, 0.0, 0.0] if abs(n[0]) < 0.9 else [0.0, 1.0, 0.0])
+        t /= np.linalg.norm(t)
+        d = np.ascontiguousarray(math.cos(theta) * n + math.sin(theta) * t)
+        got = F._load().orf_reflectance(F.C.byref(refl.surfaces), f, d.ctypes.data_as(F._f64p))
+        assert got == expect, (theta, got)

@@ -9,7 +9,11 @@ a box with 16 wavelengths and random beta; a disk split from prisms with a
 cladding ring (alpha 5.5 / cm); the adaptive evaluation end to end; the
 box-centre flux against its expectation from the angular quadrature (the
 point flux averaged by the vertex hat functions) at 4e6 rays, for zero gain
-and 1.47 / cm; beta refresh; host-side rejections.
+and 1.47 / cm; beta refresh; host-side rejections.  Reflections (surface
+reservoir passes): zero reflectivity leaves the plain result; with gain and
+reflecting top and bottom faces the device agrees with the oracle
+statistically (its reservoirs fill in atomic order); one pass matches the
+first-order image method.
 """
 import numpy as np
 import pytest
@@ -51,9 +55,9 @@ def _spectra(n, seed=2):
     return SA * rng.uniform(0.5, 1.5, n), SE * rng.uniform(0.5, 1.5, n)
 
 
-def _context(mesh, sa, se, *, min_rays, max_rays, steps=0, threshold=0.005, seed=5489):
+def _context(mesh, sa, se, *, min_rays, max_rays, steps=0, threshold=0.005, seed=5489, reflect=False):
     exp = _native.NativeExperiment(sigma_a=sa, sigma_e=se, min_rays=min_rays, max_rays=max_rays,
-                                   mse_threshold=threshold)
+                                   mse_threshold=threshold, use_reflections=reflect)
     comp = _native.make_compute(n_samples=1, adaptive_steps=steps, rng_seed=seed)
     return _native.ForwardContext(mesh, exp, comp)
 
@@ -145,6 +149,79 @@ def test_rejections():
     flat = TetMesh(points=mesh.points * [1, 1, 0], cells=mesh.cells, beta=mesh.beta)
     with pytest.raises(_native.HaseError, match="degenerate"):
         _context(flat, [SA], [SE], min_rays=10, max_rays=10)
-    exp = _native.NativeExperiment(sigma_a=[SA], sigma_e=[SE], min_rays=10, max_rays=10, use_reflections=True)
-    with pytest.raises(_native.HaseError, match="surface"):
-        _native.ForwardContext(mesh, exp, _native.make_compute(n_samples=1))
+    with _context(mesh, [SA], [SE], min_rays=10, max_rays=10, reflect=True) as ctx:
+        fs = mesh.faceSurfacesByPlane({0.0: 1, 0.2: 2})
+        with pytest.raises(_native.HaseError, match="reflectivities"):
+            ctx.set_reflections(fs, [0.0, 1.5, 0.2])
+        with pytest.raises(_native.HaseError, match="tolerance"):
+            ctx.set_reflections(fs, [0.0, 0.5, 0.2], tolerance=0.0)
+
+
+# --------------------------------------------------------------- reflections
+def _slab(nx=8, levels=9, z=0.5, beta=SA / (SA + SE)):
+    mesh = _box(nx=nx, levels=levels, z=z, beta=beta)
+    return mesh, mesh.faceSurfacesByPlane({0.0: 1, z: 2})
+
+
+def test_zero_reflectivity_reservoir_equals_plain():
+    mesh, fs = _slab(beta=0.2)
+    with _context(mesh, [SA], [SE], min_rays=100_000, max_rays=100_000) as ctx:
+        a = ctx.trace(9, 0, 100_000)
+    with _context(mesh, [SA], [SE], min_rays=100_000, max_rays=100_000, reflect=True) as ctx:
+        ctx.set_reflections(fs, [0.0, 0.0, 0.0])
+        b = ctx.trace(9, 0, 100_000)
+        st = ctx.stats()
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-9, atol=1e-12 * np.abs(a[0]).max())
+    assert st["srm_status_name"] == "CONVERGED" and st["srm_passes"] == 0
+
+
+def test_reflections_agree_with_the_oracle():
+    """Gain 1.47 / cm, reflectivity 0.5 (bottom) and 0.3 (top), passes to
+    convergence: the device and the oracle draw the same primary rays, but
+    their reservoirs fill in different orders (atomics), so the reflected
+    contributions agree statistically."""
+    z = 0.4
+    mesh, fs = _slab(nx=6, levels=7, z=z, beta=0.25)
+    kw = dict(min_rays=120_000, max_rays=120_000, steps=0, seed=21)
+    o = F.ForwardOracle(mesh)
+    refl = F.reflecting(o, fs, [0.0, 0.5, 0.3], tolerance=1e-4)
+    print("[forward] reflections: oracle", flush=True)
+    ref = F.evaluate_srm(o, refl, [SA], [SE], **kw)
+    with _context(mesh, [SA], [SE], min_rays=kw["min_rays"], max_rays=kw["max_rays"], seed=kw["seed"],
+                  reflect=True) as ctx:
+        ctx.set_reflections(fs, [0.0, 0.5, 0.3], tolerance=1e-4)
+        res = ctx.evaluate()
+        st = ctx.stats()
+    status, passes, _ = refl.status()
+    print(f"[forward] reflections: oracle passes {passes} status {status}; device {st['srm_passes']} "
+          f"{st['srm_status_name']}", flush=True)
+    assert st["srm_status_name"] in ("CONVERGED", "STABLE") and abs(st["srm_passes"] - max(passes)) <= 2
+    zs = (res.phiAse.astype(np.float64) - ref["phiAse"]) / np.hypot(res.standardError, ref["standardError"])
+    assert np.mean(np.abs(zs) > 4) <= 0.01 and abs(zs.mean()) < 0.3, (np.mean(np.abs(zs) > 4), zs.mean())
+    vol = mesh.volumes()
+    total = float(np.sum(res.phiAse * vol) / np.sum(ref["phiAse"] * vol))
+    assert abs(total - 1.0) < 0.01, total
+
+
+def test_first_order_reflections_against_the_image_method():
+    z, beta = 0.5, SA / (SA + SE)
+    mesh, fs = _slab(z=z, beta=beta)
+    print("[forward] first-order reflections, 2e6 rays", flush=True)
+    with _context(mesh, [SA], [SE], min_rays=2_000_000, max_rays=2_000_000, reflect=True) as ctx:
+        ctx.set_reflections(fs, [0.0, 0.4, 0.2], max_passes=1)
+        res = ctx.evaluate()
+        st = ctx.stats()
+    assert st["srm_status_name"] == "MAX_PASSES" and st["srm_passes"] == 1
+    cen = mesh.centroids()
+    near = np.nonzero(np.linalg.norm((cen - [0.5, 0.5, z / 2]) / [1, 1, z], axis=1) < 0.12)[0]
+
+    def flux(p):
+        return box_point_flux(p, (0.0, 1.0, 0.0, 1.0), z, beta, 0.0, NTOT / TAU, reflect=True, refl_bottom=0.4,
+                              refl_top=0.2, n_mu=200, n_phi=400)
+
+    with np.errstate(invalid="ignore"):
+        expect = hat_smoothed_cell_flux(mesh.points, mesh.cells, mesh.volumes(), near, flux)
+    ratio = res.phiAse[near].astype(np.float64).sum() / expect.sum()
+    print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}", flush=True)
+    assert abs(ratio - 1.0) < 0.006, ratio
