@@ -222,7 +222,24 @@ def forward_bench(args):
                                    max_sigma_e=mse_)
     comp = _native.make_compute(n_samples=1, adaptive_steps=0, rng_seed=5489)
     log(f"forward: {mesh.n_cells} cells, {mesh.n_points} points, {rays} rays per evaluation")
+    if args.forward_reflect:
+        # bottom and top faces keep their triangle's reflectivity and the
+        # slab's refractive indices: surface 1 + t (bottom), 1 + T + t (top)
+        T, L = topo.numberOfTriangles, int(topo.levels)
+        z_top = float(topo.levelCoordinates()[-1])
+        fs = mesh.faceSurfacesByPlane({0.0: -1, z_top: -2})
+        tri = (np.arange(mesh.n_cells) // 3) % T
+        fs = np.where(fs == -1, 1 + tri[:, None], np.where(fs == -2, 1 + T + tri[:, None], fs))
+        refl = np.asarray(w.medium.value("reflectivities"), dtype=np.float64).reshape(-1)
+        nidx = np.asarray(w.medium.value("refractiveIndices"), dtype=np.float64).reshape(-1)
+        r_tab = np.concatenate([[0.0], refl[0::2], refl[1::2]])
+        nin = np.concatenate([[0.0], np.full(T, nidx[0]), np.full(T, nidx[2])])
+        nout = np.concatenate([[0.0], np.full(T, nidx[1]), np.full(T, nidx[3])])
+        exp = _native.NativeExperiment(sigma_a=sa, sigma_e=se, min_rays=rays, max_rays=rays, max_sigma_a=msa,
+                                       max_sigma_e=mse_, use_reflections=True)
     ctx = _native.ForwardContext(mesh, exp, comp)
+    if args.forward_reflect:
+        ctx.set_reflections(fs, r_tab, nin, nout)
     ctx.evaluate()
     walls, traces, segs = [], [], 0
     for _ in range(max(1, args.steps)):
@@ -241,6 +258,8 @@ def forward_bench(args):
            "rays_per_s": rays / wall, "trace_rays_per_s": rays / trace, "segments_per_s": segs / trace,
            "bytes_per_segment": bytes_per_segment, "achieved_GBps": segs * bytes_per_segment / trace / 1e9,
            "dropped_rays": int(res.droppedRays.sum()), "dtype": "f64"}
+    if args.forward_reflect:
+        out["reflections"] = {k: st[k] for k in ("srm_status_name", "srm_passes", "srm_remaining")}
     if not args.no_cpu_baseline:
         from oracle import forward as F
 
@@ -399,6 +418,8 @@ def main():
                     help="time integrator of --time-steps")
     ap.add_argument("--forward", type=float, default=0, metavar="RAYS",
                     help="forward Tet4 estimator on the production disk with this many rays per evaluation")
+    ap.add_argument("--forward-reflect", action="store_true",
+                    help="with --forward: reflecting top and bottom faces (surface reservoir passes)")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
