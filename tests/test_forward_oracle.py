@@ -295,3 +295,28 @@ def test_total_internal_reflection_only_beyond_the_critical_angle():
         d = np.ascontiguousarray(math.cos(theta) * n + math.sin(theta) * t)
         got = F._load().orf_reflectance(F.C.byref(refl.surfaces), f, d.ctypes.data_as(F._f64p))
         assert got == expect, (theta, got)
+
+
+def test_single_tetrahedron_flux_is_the_mean_exit_distance():
+    """One tetrahedron, zero gain: each ray scores its exit distance and the
+    lumped vertex volumes of a single cell add up to its volume, so the cell
+    flux is (nTot/tau) * beta * (mean exit distance of the rays), the exit
+    distances computed here by brute force from the oracle's own ray starts
+    (ray-plane distances to the four faces)."""
+    pts = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    beta = SA / (SA + SE)
+    mesh = TetMesh(points=pts, cells=[[0, 1, 2, 3]], beta=[beta], n_tot=NTOT, tau=TAU)
+    o = F.ForwardOracle(mesh)
+    rays = 4000
+    res = o.evaluate([SA], [SE], min_rays=rays, max_rays=rays, steps=0, seed=8)
+    n = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    total = 0.0
+    for r in range(rays):
+        ray = o.ray_start(8, 0, rays, r, 1)
+        p, u = np.array(ray.p), np.array(ray.u)
+        ts = [-p[k] / u[k] for k in range(3) if u[k] < 0]      # planes x_k = 0
+        if u @ n > 0:                                          # plane x + y + z = 1
+            ts.append((1.0 / np.sqrt(3.0) - p @ n) / (u @ n))
+        total += min(ts)
+    expect = (NTOT / TAU) * beta * total / rays
+    assert res["phiAse"][0] == pytest.approx(expect, rel=1e-6)

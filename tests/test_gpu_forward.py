@@ -230,3 +230,30 @@ def test_first_order_reflections_against_the_image_method():
     ratio = res.phiAse[near].astype(np.float64).sum() / expect.sum()
     print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}", flush=True)
     assert abs(ratio - 1.0) < 0.006, ratio
+
+
+def test_edge_cases_zero_beta_few_rays_single_cell():
+    # no excited ions: nothing is emitted, every cell reads zero
+    mesh = _box(nx=2, levels=3, z=0.2, beta=0.0)
+    with _context(mesh, [SA], [SE], min_rays=1000, max_rays=1000) as ctx:
+        res = ctx.evaluate()
+        st = ctx.stats()
+    assert st["segments"] == 0 and np.all(res.phiAse == 0.0) and np.all(res.rayVisits == 0)
+    # fewer rays than RSE batches: the oracle's accumulators, batch by batch
+    mesh = _box(nx=2, levels=3, z=0.2)
+    o = F.ForwardOracle(mesh)
+    for rays in (1, 5, 8, 9):
+        ref_scores, ref_visits, ref_dropped, counts, ref_seg = o.trace([SA], [SE], 4, 0, rays)
+        with _context(mesh, [SA], [SE], min_rays=rays, max_rays=rays) as ctx:
+            scores, visits, dropped, seg = ctx.trace(4, 0, rays)
+        np.testing.assert_array_equal(visits, ref_visits)
+        np.testing.assert_allclose(scores, ref_scores, rtol=1e-9, atol=1e-300)
+        assert seg == ref_seg
+    # one tetrahedron: every ray leaves through one of its faces after one segment
+    one = TetMesh(points=[[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], cells=[[0, 1, 2, 3]], beta=[0.2],
+                  n_tot=NTOT, tau=TAU)
+    with _context(one, [SA], [SE], min_rays=10_000, max_rays=10_000) as ctx:
+        res = ctx.evaluate()
+        st = ctx.stats()
+    assert st["segments"] == 10_000 and res.rayVisits[0] == 10_000 and res.droppedRays[0] == 0
+    assert res.phiAse[0] > 0.0 and np.isfinite(res.relativeStandardError[0])
