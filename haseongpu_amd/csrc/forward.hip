@@ -22,8 +22,9 @@
 // Layout in HBM (C cells, Np points): face planes 4 x double4 per cell
 // (128 B, one cache line), a 64-B cell record (neighbour links: neighbour * 4
 // + its local face, -1 at the boundary; vertex ids; beta; cladding flag),
-// beta*V prefix f64[C], lumped vertex volumes f64[2 Np], scores f64[8 x 2 Np]
-// (eight RSE batches x gain | cladding vertex copies), visits and dropped
+// beta*V prefix f64[C], lumped vertex volumes f64[2 Np], scores f64[8 x 8 x 2 Np]
+// (eight atomic copies x eight RSE batches x gain | cladding vertex copies),
+// visits and dropped
 // u32[C].  One thread walks one ray in FP64 with contraction off, so a ray
 // takes exactly the cells the FP64 oracle takes.  Vertex scores are FP64
 // hardware atomics, which bound the kernel; since consecutive cells of a walk
@@ -46,6 +47,10 @@ namespace hase {
 namespace fwd {
 
 constexpr uint32_t kBatches = 8;
+// copies of the vertex scores: work-group b adds into copy b % kCopies, which
+// spreads the FP64 atomics of concurrent work-groups over more addresses (the
+// copies are folded into copy 0 before the scores are read)
+constexpr uint32_t kCopies = 8;
 constexpr uint32_t kKeyTag = 0x46574431u;
 constexpr uint32_t kMaxRecovery = 32;
 constexpr uint32_t kDirectionCalls = 32;
@@ -311,7 +316,7 @@ __device__ void walk(const TraceArgs& a, uint32_t cell, double p[3], const doubl
 {
 #pragma clang fp contract(off)
     const TetDev& m = a.m;
-    double* scores = a.scores + (size_t)batch * 2u * m.n_points;
+    double* scores = a.scores + ((size_t)(blockIdx.x % kCopies) * kBatches + batch) * 2u * m.n_points;
     double gain = 1.0;
     bool dropped = true;
     ex.face = -1;
@@ -681,6 +686,20 @@ __global__ void __launch_bounds__(256) hase_forward_finalize_kernel(FinalizeArgs
     if (!ok) atomicAdd(a.open_cells, 1u);
 }
 
+// copy 0 += copies 1..kCopies-1, which are cleared (n = one copy's length)
+__global__ void hase_forward_fold_kernel(double* scores, size_t n)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = scores[i];
+#pragma unroll
+    for (uint32_t c = 1; c < kCopies; ++c) {
+        s += scores[c * n + i];
+        scores[c * n + i] = 0.0;
+    }
+    scores[i] = s;
+}
+
 __global__ void hase_forward_clear_kernel(double* scores, size_t n_scores, uint32_t* counters, size_t n_counters)
 {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -905,7 +924,7 @@ static std::unique_ptr<hase_forward> create(const hase_tet_mesh_t* mesh, const h
     set_beta(f.get(), mesh->beta_volume);
     f->sigma_a.upload(exp->sigma_a, f->n_lambda, f->stream);
     f->sigma_e.upload(exp->sigma_e, f->n_lambda, f->stream);
-    f->scores.alloc((size_t)kBatches * 2u * f->n_points);
+    f->scores.alloc((size_t)kCopies * kBatches * 2u * f->n_points);
     f->counters.alloc(3 * (size_t)f->n_cells + 1);
     f->segments.alloc(1);
     f->phi.alloc(f->n_cells);
@@ -916,9 +935,17 @@ static std::unique_ptr<hase_forward> create(const hase_tet_mesh_t* mesh, const h
     return f;
 }
 
+static void fold(hase_forward* f)
+{
+    const size_t n = (size_t)kBatches * 2u * f->n_points;
+    hipLaunchKernelGGL(hase_forward_fold_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, f->stream,
+                       f->scores.p, n);
+    HIP_CHECK(hipGetLastError());
+}
+
 static void clear(hase_forward* f)
 {
-    const size_t ns = (size_t)kBatches * 2u * f->n_points, nc = 3 * (size_t)f->n_cells + 1;
+    const size_t ns = (size_t)kCopies * kBatches * 2u * f->n_points, nc = 3 * (size_t)f->n_cells + 1;
     const size_t n = std::max(ns, nc);
     hipLaunchKernelGGL(hase_forward_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, f->stream,
                        f->scores.p, ns, f->counters.p, nc);
@@ -1194,6 +1221,7 @@ static void evaluate(hase_forward* f, hase_forward_result_t* out)
         f->stats.trace_ms += launch_trace(f, f->seed, launches, rays);
         done = target;
         ++launches;
+        fold(f);
         FinalizeArgs fa{};
         fa.m = f->view();
         fa.volume = f->volume_dev.p;
@@ -1298,6 +1326,7 @@ int hase_forward_trace(hase_forward_t* f, uint32_t seed, uint32_t launch, uint32
         hase::fwd::clear(f);
         f->stats = hase_forward_stats_t{};
         f->stats.trace_ms = hase::fwd::launch_trace(f, seed, launch, rays);
+        hase::fwd::fold(f);
         const size_t n = f->n_cells;
         HIP_CHECK(hipMemcpyAsync(scores, f->scores.p, (size_t)hase::fwd::kBatches * 2u * f->n_points * sizeof(double),
                                  hipMemcpyDeviceToHost, f->stream));
