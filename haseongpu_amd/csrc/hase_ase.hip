@@ -63,6 +63,34 @@ struct EntryIndex {
     }
 };
 
+// Entry e = k * P + l * T + t of loops that visit e, e + step, e + 2 step, ...:
+// decoded once, then advanced without an integer division per entry.
+struct EntryCursor {
+    EntryIndex x;
+    uint32_t T, layers;
+    __device__ __forceinline__ void start(uint32_t e, uint32_t P, uint32_t T_)
+    {
+        T = T_;
+        layers = P / T_;
+        x.init(e, P, T_);
+    }
+    __device__ __forceinline__ void advance(uint32_t step)
+    {
+        uint32_t t = x.t + step, l = x.l, k = x.k;
+        while (t >= T) {
+            t -= T;
+            if (++l == layers) {
+                l = 0;
+                ++k;
+            }
+        }
+        x.t = t;
+        x.l = l;
+        x.k = k;
+        x.j = t + l * T;
+    }
+};
+
 // LDS for one pass of mixture_prefix: kBlock * kScanPer doubles, one pad
 // slot per kScanPer so a thread's consecutive run is bank-conflict free.
 constexpr int kScanLds = kBlock * kScanPer + kBlock * kScanPer / kScanPer;
@@ -100,17 +128,17 @@ __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_
     const bool use_imp = I > 0.0 && isfinite(I);
     const double wI = use_imp ? (1.0 - eps) / I : 0.0;
     const double wB = (use_imp ? eps : 1.0) / (Bsum * (double)R);
+    // entries base + i * kBlock + tid: one cursor stepping by kBlock
+    EntryCursor cur;
+    cur.start((uint32_t)tid, P, T);
     for (uint32_t base = 0; base < n; base += (uint32_t)kBlock * kScanPer) {
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) {
             const uint32_t e = base + (uint32_t)(i * kBlock + tid);
             double q = 0.0;
-            if (e < n) {
-                EntryIndex x;
-                x.init(e, P, T);
-                q = wI * (double)value(x, e) + wB * (double)bv[x.j];
-            }
+            if (e < n) q = wI * (double)value(cur.x, e) + wB * (double)bv[cur.x.j];
             buf[scan_slot(i * kBlock + tid)] = q;
+            cur.advance((uint32_t)kBlock);
         }
         __syncthreads();
         double v[kScanPer];
@@ -462,10 +490,11 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
     __shared__ double red[kBlock / 64 + 2];
     __shared__ double buf[kScanLds];
     double local = 0.0;
+    EntryCursor cur;
+    cur.start(threadIdx.x, P, T);
     for (uint32_t e = threadIdx.x; e < n; e += kBlock) {
-        EntryIndex x;
-        x.init(e, P, T);
-        local += (double)value(x, e);
+        local += (double)value(cur.x, e);
+        cur.advance((uint32_t)kBlock);
     }
     mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, value, p.prefix + (size_t)slot * n, p.Q + slot, red,
                    buf, p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
