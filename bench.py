@@ -348,6 +348,125 @@ def time_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def combined_bench(args):
+    """Combined case at scale, one share: the 20,682-triangle, 30-level disk
+    (599,778 prisms, 315,300 sample points, 128 wavelengths, reflections,
+    cladding, adaptive 1e5 -> 1e8 rays, mse 0.005) with a seeded smooth
+    pump-depletion beta update before every evaluation, for
+    ``--combined`` time steps.  One GPU evaluates one share of the sample
+    points (``--share`` contiguous points of the mid-slab level), the unit
+    the path shards by; the mesh and beta are whole and resident.  Both
+    forms are timed (walked map; network trained on this mesh's own random
+    pump fields, with the walked-map fallback), and the last step's
+    estimates are checked against the FP64 oracle on a few points."""
+    from haseongpu_amd import _native
+    from haseongpu_amd import proposal as prop
+    from haseongpu_amd.geometry import GainMedium
+    from haseongpu_amd.workloads import Workload, combined_disk, depletion_field
+
+    w = combined_disk()
+    topo = w.topology
+    Np, L = topo.numberOfPoints, int(topo.levels)
+    first = (L // 2) * Np
+    last = first + min(args.share, Np) - 1
+    beta0 = np.asarray(w.medium.value("betaVolume"), dtype=np.float64)
+    factor = depletion_field(topo)
+    n = args.combined
+    log(f"combined: {topo.numberOfTriangles} triangles, {L} levels, {topo.numberOfPrisms} prisms, {w.samples} "
+        f"samples; share {first}..{last}; {n} steps")
+    forms, results = {}, {}
+    for form in (("plain",) if args.plain_only else ("plain", "learned")):
+        t0 = time.perf_counter()
+        ctx, spectra = build_context(w, min_sample_range=first, max_sample_range=last)
+        setup = time.perf_counter() - t0
+        train_s = 0.0
+        if form == "learned":
+            ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+            t0 = time.perf_counter()
+            if args.load_proposal:
+                net = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
+            else:
+                net, _ = prop.train_on_workload(w, fields=args.fields, samples_per_field=48, rows_per_sample=1024,
+                                                seed=1, epochs=12, batch=4096, cluster_triangles=ct,
+                                                cluster_levels=cl)
+            train_s = time.perf_counter() - t0
+            log(f"network trained in {train_s:.1f}s")
+            ctx.set_proposal(net)
+        best = ctx.autotune(256)
+        beta = beta0.copy()
+        ctx.update_beta(beta)
+        ctx.evaluate()                                  # warm evaluation at beta_0
+        step_s, rays, conv, max_mse = [], 0, True, 0.0
+        for k in range(n):
+            beta = beta * factor
+            t0 = time.perf_counter()
+            ctx.update_beta(beta)
+            res = ctx.evaluate()
+            step_s.append(time.perf_counter() - t0)
+            part = slice(first, last + 1)
+            rays += int(res.totalRays[part].astype(np.int64).sum())
+            conv &= bool(np.all(res.mse[part] <= w.settings["mseThreshold"]))
+            max_mse = max(max_mse, float(np.nanmax(res.mse[part])))
+            st = ctx.stats()
+            log(f"{form} step {k + 1}: {step_s[-1]:.3f}s rays {int(res.totalRays[part].sum())} "
+                f"max mse {np.nanmax(res.mse[part]):.2e} importance {st['importance_ms']:.0f}ms "
+                f"trace {st['trace_ms']:.0f}ms")
+        total = float(np.sum(step_s))
+        forms[form] = {"s_per_step": total / n, "rays_per_s": rays / total, "all_converged": conv,
+                       "max_mse": max_mse, "context_s": setup, "training_s": train_s, "tuning": best}
+        if form == "learned":
+            forms[form]["fallback_samples_last_step"] = ctx.proposal_stats()["fallback_samples"]
+        results[form] = res
+        ctx.close()
+    part = slice(first, last + 1)
+    out = {"measurement": "combined at scale (one share)", "workload": "combined_disk + depletion field",
+           "triangles": topo.numberOfTriangles, "levels": L, "prisms": topo.numberOfPrisms,
+           "sample_points": w.samples, "share": [first, last], "wavelengths": int(spectra[0].size),
+           "rays_per_sample": [w.settings["minRaysPerSample"], w.settings["maxRaysPerSample"]],
+           "steps": n, "beta_max_last": float(beta.max())}
+    out.update(forms)
+    if "learned" in forms:
+        a, b = results["plain"], results["learned"]
+        z = (b.phiAse[part].astype(np.float64) - a.phiAse[part]) / np.sqrt(b.standardError[part] ** 2
+                                                                           + a.standardError[part] ** 2)
+        out["speedup_vs_plain"] = forms["plain"]["s_per_step"] / forms["learned"]["s_per_step"]
+        out["learned_vs_plain"] = {"mean_z": float(np.mean(z)), "max_abs_z": float(np.max(np.abs(z)))}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        phys = dict(w.medium.physical)
+        phys["betaVolume"] = beta
+        wl = Workload(w.name, topo, GainMedium(topo, phys), w.spectra, w.settings)
+        sa, se, msa, mse_ = spectra
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        count = max(8, min(threads, 16))
+        s = w.settings
+        op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_,
+                            n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")),
+                            beta_cells=w.medium.value("betaCells"), min_rays=s["minRaysPerSample"],
+                            max_rays=s["maxRaysPerSample"], mse_threshold=s["mseThreshold"],
+                            use_reflections=s["useReflections"], max_reflections=s.get("maxReflections", 1),
+                            adaptive_steps=s["adaptiveSteps"], seed=s["rngSeed"], sample_min=first,
+                            sample_max=first + count - 1, threads=threads, n_samples=w.samples)
+        t0 = time.perf_counter()
+        ref = O.calc_phi_ase(O.OracleMesh(topo, wl.medium), op)
+        dt = time.perf_counter() - t0
+        sl = slice(first, first + count)
+        check = {"samples": count, "oracle_s": dt, "oracle_threads": threads}
+        for form, r in results.items():
+            se_ref = np.sqrt(ref["mse"][sl]) * ref["phiAse"][sl]
+            zz = (r.phiAse[sl].astype(np.float64) - ref["phiAse"][sl]) / np.sqrt(r.standardError[sl] ** 2 + se_ref ** 2)
+            rel = np.abs(r.phiAse[sl].astype(np.float64) / ref["phiAse"][sl] - 1.0)
+            check[form] = {"max_abs_z": float(np.max(np.abs(zz))), "median_rel": float(np.median(rel)),
+                           "max_rel": float(np.max(rel))}
+        out["oracle_check"] = check
+        out["cpu_baseline"] = {"value": float(ref["totalRays"][sl].astype(np.int64).sum()) / dt, "unit": "rays/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{count} sample points of the share at the last step's beta"}
+    print(json.dumps(out), flush=True)
+
+
 def traffic_pass(w, args):
     """Kernels for the FETCH_SIZE / WRITE_SIZE counter runs: the learned
     evaluation (its hase_trace_kernel launches are the bench line's), a plain
@@ -420,6 +539,10 @@ def main():
                     help="forward Tet4 estimator on the production disk with this many rays per evaluation")
     ap.add_argument("--forward-reflect", action="store_true",
                     help="with --forward: reflecting top and bottom faces (surface reservoir passes)")
+    ap.add_argument("--combined", type=int, default=0, metavar="STEPS",
+                    help="separate measurement: the combined case at scale (599,778 prisms, 128 wavelengths, "
+                         "1e5 -> 1e8 rays), one share of its sample points, STEPS depletion steps, both forms")
+    ap.add_argument("--share", type=int, default=1024, help="sample points of the --combined share")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
@@ -441,6 +564,9 @@ def main():
         return
     if args.forward:
         forward_bench(args)
+        return
+    if args.combined:
+        combined_bench(args)
         return
     w = production_disk(levels=args.levels)
     if args.traffic_pass:

@@ -17,6 +17,11 @@
   bottom reflectivity ~U(0, 0.1), the outer ring of triangles (those with an
   exterior edge) as cladding with absorption 5.5 cm^-1 and beta 0, adaptive
   rays 1e5 -> 1e7 per sample with mse threshold 0.005.
+* ``combined_disk`` - the same construction at scale: radius 5 cm, ~10.5k
+  points, 20,682 triangles, 30 levels over 1.0 cm (599,778 prisms, 315,300
+  sample points), 128 wavelengths, reflections, cladding ring, adaptive
+  rays 1e5 -> 1e8; ``depletion_field`` gives its seeded smooth
+  pump-depletion factors, one beta update per time step.
 """
 from __future__ import annotations
 
@@ -118,6 +123,36 @@ def production_disk(seed=2, levels=20, n_points=2060, wavelengths=64, min_rays=1
                     repetitions=1, adaptiveSteps=4, useReflections=True, maxReflections=max_reflections,
                     monochromatic=False, rngSeed=seed)
     return Workload("production_disk", topo, medium, gaussian_spectra(wavelengths), settings)
+
+
+def combined_disk(seed=5, levels=30, n_points=10_400, wavelengths=128, min_rays=100_000, max_rays=100_000_000,
+                  mse_threshold=0.005, radius=5.0, thickness_total=1.0, max_reflections=1):
+    """Combined case at scale (production-disk construction, radius 5 cm,
+    ~20.7k triangles, 30 levels over 1 cm, 128 wavelengths, 1e5 -> 1e8 rays)."""
+    w = production_disk(seed=seed, levels=levels, n_points=n_points, wavelengths=wavelengths, min_rays=min_rays,
+                        max_rays=max_rays, mse_threshold=mse_threshold, radius=radius,
+                        thickness_total=thickness_total, max_reflections=max_reflections)
+    w.name = "combined_disk"
+    return w
+
+
+def depletion_field(topology, *, seed=5, spots=3, rate=(0.004, 0.012), widths=(1.0, 3.0), absorption=0.8):
+    """Seeded smooth pump-depletion factors per prism, applied once per time
+    step (beta_{k+1} = beta_k * factor): 1 - sum of Gaussian spots, each with
+    a depletion rate per step drawn from `rate`, a width from `widths` (cm)
+    and a centre within 60% of the disk radius, decaying along z with
+    coefficient `absorption` (cm^-1)."""
+    rng = np.random.default_rng(seed)
+    cent = topology.prismCentroids()
+    r_max = float(np.hypot(topology.points[:, 0], topology.points[:, 1]).max())
+    loss = np.zeros(cent.shape[0])
+    for _ in range(int(spots)):
+        rr = 0.6 * r_max * np.sqrt(rng.uniform())
+        ang = rng.uniform(0.0, 2.0 * np.pi)
+        w = rng.uniform(*widths)
+        d2 = (cent[:, 0] - rr * np.cos(ang)) ** 2 + (cent[:, 1] - rr * np.sin(ang)) ** 2
+        loss += rng.uniform(*rate) * np.exp(-d2 / (2.0 * w * w))
+    return np.clip(1.0 - loss * np.exp(-absorption * cent[:, 2]), 0.0, 1.0)
 
 
 def batch_problems(n=64, seed=0, triangles=(200, 8000), levels=(5, 40), wavelengths=(1, 16), min_rays=10_000,
