@@ -367,15 +367,19 @@ def combined_bench(args):
     w = combined_disk()
     topo = w.topology
     Np, L = topo.numberOfPoints, int(topo.levels)
-    first = (L // 2) * Np
-    last = first + min(args.share, Np) - 1
+    if args.share >= w.samples:                         # every sample point: one whole step, no warm evaluation
+        first, last = 0, w.samples - 1
+    else:
+        first = (L // 2) * Np
+        last = first + min(args.share, Np) - 1
+    whole = first == 0 and last == w.samples - 1
     beta0 = np.asarray(w.medium.value("betaVolume"), dtype=np.float64)
     factor = depletion_field(topo)
     n = args.combined
     log(f"combined: {topo.numberOfTriangles} triangles, {L} levels, {topo.numberOfPrisms} prisms, {w.samples} "
         f"samples; share {first}..{last}; {n} steps")
     forms, results = {}, {}
-    for form in (("plain",) if args.plain_only else ("plain", "learned")):
+    for form in (("plain",) if args.plain_only else ("learned",) if args.learned_only else ("plain", "learned")):
         t0 = time.perf_counter()
         ctx, spectra = build_context(w, min_sample_range=first, max_sample_range=last)
         setup = time.perf_counter() - t0
@@ -395,7 +399,8 @@ def combined_bench(args):
         best = ctx.autotune(256)
         beta = beta0.copy()
         ctx.update_beta(beta)
-        ctx.evaluate()                                  # warm evaluation at beta_0
+        if not whole:
+            ctx.evaluate()                              # warm evaluation at beta_0
         step_s, rays, conv, max_mse = [], 0, True, 0.0
         for k in range(n):
             beta = beta * factor
@@ -419,13 +424,14 @@ def combined_bench(args):
         results[form] = res
         ctx.close()
     part = slice(first, last + 1)
-    out = {"measurement": "combined at scale (one share)", "workload": "combined_disk + depletion field",
+    out = {"measurement": "combined at scale (every sample point)" if whole else "combined at scale (one share)",
+           "workload": "combined_disk + depletion field",
            "triangles": topo.numberOfTriangles, "levels": L, "prisms": topo.numberOfPrisms,
            "sample_points": w.samples, "share": [first, last], "wavelengths": int(spectra[0].size),
            "rays_per_sample": [w.settings["minRaysPerSample"], w.settings["maxRaysPerSample"]],
-           "steps": n, "beta_max_last": float(beta.max())}
+           "steps": n, "warm_evaluation_first": not whole, "beta_max_last": float(beta.max())}
     out.update(forms)
-    if "learned" in forms:
+    if "learned" in forms and "plain" in forms:
         a, b = results["plain"], results["learned"]
         z = (b.phiAse[part].astype(np.float64) - a.phiAse[part]) / np.sqrt(b.standardError[part] ** 2
                                                                            + a.standardError[part] ** 2)
@@ -521,6 +527,7 @@ def main():
     ap.add_argument("--fields", type=int, default=16, help="random pump fields the network trains on")
     ap.add_argument("--clusters", default="16x4", help="triangles x layers per network evaluation")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--learned-only", action="store_true", help="with --combined: time the learned form only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default queue launch shape")
     ap.add_argument("--rounds", action="store_true", help="one trace launch per adaptive round (no work queue)")
     ap.add_argument("--save-proposal", default=None, help="write the trained network (.npz)")
@@ -542,7 +549,8 @@ def main():
     ap.add_argument("--combined", type=int, default=0, metavar="STEPS",
                     help="separate measurement: the combined case at scale (599,778 prisms, 128 wavelengths, "
                          "1e5 -> 1e8 rays), one share of its sample points, STEPS depletion steps, both forms")
-    ap.add_argument("--share", type=int, default=1024, help="sample points of the --combined share")
+    ap.add_argument("--share", type=int, default=1024,
+                    help="sample points of the --combined share (>= 315300: all of them, no warm evaluation)")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
