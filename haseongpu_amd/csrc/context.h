@@ -76,6 +76,7 @@ struct hase_context {
     hase::DevBuf<uint32_t> q_slot_problem;                                      // zeros
     hase::GraphCache q_graph;
     uint64_t q_generation = 1;                                                  // bumped when launch inputs may change
+    size_t batch_budget = 0;                                                    // bytes of batch buffers, fixed at first use
 
     // device copy of the flux of every sample, kept when phi_keep is set
     // (the time-stepped loop reads it without a host round trip)

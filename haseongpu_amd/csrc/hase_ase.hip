@@ -1037,10 +1037,21 @@ struct BatchTimers {
 };
 
 // Device buffers for batches of up to `want` resident samples; returns the
-// batch size (the importance prefix and scratch of a batch stay within ~6 GiB
-// unless the caller fixed sample_batch).
-uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = (size_t)6 << 30, bool proposal_only = false)
+// batch size.  Unless the caller fixed sample_batch, the importance prefix and
+// scratch of a batch stay within `budget` bytes; budget 0 takes the context's
+// own, a quarter of the device memory free when it is first needed, between
+// 6 and 48 GiB (large meshes need ~22 MB per sample, and batches of a few
+// hundred samples leave most CUs idle in the per-sample proposal kernels).
+uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool proposal_only = false)
 {
+    if (budget == 0) {
+        if (ctx->batch_budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+            ctx->batch_budget = std::min(std::max(free_b / 4u, (size_t)6 << 30), (size_t)48 << 30);
+        }
+        budget = ctx->batch_budget;
+    }
     const hase_compute_t& c = ctx->comp;
     const hase_experiment_t& e = ctx->exp;
     const uint32_t R = slices(ctx);
