@@ -119,3 +119,22 @@ def test_autotune_picks_a_valid_setting_and_keeps_results():
         assert ctx.tuning() == best
         got = ctx.evaluate()
     _same(ref, got)
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_results_do_not_depend_on_the_batch_size(learned):
+    # sample_batch 0 sizes the batch from the free device memory (every sample point of this
+    # disk in one batch); 97 splits the same evaluation into 25 batches
+    w = production_disk(levels=6, n_points=400, wavelengths=8)
+    out = []
+    for sb in (0, 97):
+        ctx, _ = _context(w, min_rays=4096, max_rays=16384, adaptive_steps=2, use_reflections=True,
+                          mse_threshold=2e-3, rng_seed=8, sample_batch=sb)
+        with ctx:
+            if learned:
+                ctx.set_proposal(_proposal_for(w, _random_net(8, 0.5), 4, 2, fallback=True))
+            out.append(ctx.evaluate())
+            if learned:
+                assert ctx.proposal_stats()["fallback_samples"] > 0
+    assert w.samples > 97 * 20
+    _same(*out)
