@@ -182,10 +182,12 @@ def batch_bench(args):
         out = batch.evaluate()
         return time.perf_counter() - t0, out
 
-    alone()
-    together()
+    log(f"warm-up alone: {alone()[0]:.2f}s")
+    log(f"warm-up together: {together()[0]:.2f}s")
     ta, ra = alone()
+    log(f"alone: {ta:.2f}s")
     tb, rb = together()
+    log(f"together: {tb:.2f}s")
     st = batch.stats()
     rays_total = sum(int(r.totalRays.astype(np.int64).sum()) for r in rb)
     same = all(np.array_equal(a.totalRays, b.totalRays) for a, b in zip(ra, rb))

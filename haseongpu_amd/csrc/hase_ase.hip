@@ -1039,19 +1039,14 @@ struct BatchTimers {
 // Device buffers for batches of up to `want` resident samples; returns the
 // batch size.  Unless the caller fixed sample_batch, the importance prefix and
 // scratch of a batch stay within `budget` bytes; budget 0 takes the context's
-// own, a quarter of the device memory free when it is first needed, between
-// 6 and 48 GiB (large meshes need ~22 MB per sample, and batches of a few
-// hundred samples leave most CUs idle in the per-sample proposal kernels).
+// own, fixed at its first batch: 6 GiB, or for a mesh whose entries need more
+// than that for 2048 samples (~22 MB per sample at 600k prisms, where a few
+// hundred samples per batch leave most CUs idle in the per-sample proposal
+// kernels) up to those 2048 samples' bytes, within a quarter of the device
+// memory then free and 48 GiB.  Every context keeps its buffers, so a
+// greedier default would starve the contexts created after it.
 uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool proposal_only = false)
 {
-    if (budget == 0) {
-        if (ctx->batch_budget == 0) {
-            size_t free_b = 0, total_b = 0;
-            HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-            ctx->batch_budget = std::min(std::max(free_b / 4u, (size_t)6 << 30), (size_t)48 << 30);
-        }
-        budget = ctx->batch_budget;
-    }
     const hase_compute_t& c = ctx->comp;
     const hase_experiment_t& e = ctx->exp;
     const uint32_t R = slices(ctx);
@@ -1069,6 +1064,18 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
                                       : max_chunks;
         size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + kPrefixBuckets * 4u +
                             (learned ? (size_t)R * ctx->C * 4u : 0u);
+        if (budget == 0) {
+            if (ctx->batch_budget == 0) {
+                const size_t floor_b = (size_t)6 << 30, want_b = per_sample * 2048u;
+                ctx->batch_budget = floor_b;
+                if (want_b > floor_b) {
+                    size_t free_b = 0, total_b = 0;
+                    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+                    ctx->batch_budget = std::max(floor_b, std::min({want_b, free_b / 4u, (size_t)48 << 30}));
+                }
+            }
+            budget = ctx->batch_budget;
+        }
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
     batch = std::min(batch, want);

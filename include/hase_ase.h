@@ -100,8 +100,9 @@ typedef struct hase_compute {
     int32_t device;                 /* HIP device ordinal */
     uint32_t rays_per_thread;       /* tuning: rays a lane takes per work item (0 = auto) */
     uint32_t block_size;            /* tuning: threads per ray work-group (0 = auto) */
-    uint32_t sample_batch;          /* tuning: samples resident at once (0 = auto: up to 8192, within a quarter
-                                       of the free device memory, 6-48 GiB, at the first evaluation) */
+    uint32_t sample_batch;          /* tuning: samples resident at once (0 = auto: up to 8192 in 6 GiB, or up
+                                       to 2048 for meshes too large for that, within a quarter of the free
+                                       device memory and 48 GiB) */
 } hase_compute_t;
 
 /* Per-sample outputs, caller-owned, each with S entries.  Samples outside
