@@ -182,8 +182,8 @@ def batch_bench(args):
         out = batch.evaluate()
         return time.perf_counter() - t0, out
 
+    log(f"warm-up together (contexts never evaluated): {together()[0]:.2f}s")
     log(f"warm-up alone: {alone()[0]:.2f}s")
-    log(f"warm-up together: {together()[0]:.2f}s")
     ta, ra = alone()
     log(f"alone: {ta:.2f}s")
     tb, rb = together()

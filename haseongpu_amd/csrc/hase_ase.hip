@@ -1498,7 +1498,34 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
     check_tuning(tu);
     b->stats = hase_stats_t{};
     b->generation++;
-    const size_t budget = std::min<size_t>((size_t)6 << 30, std::max<size_t>((size_t)512 << 20, ((size_t)24 << 30) / n));
+    // Batch memory per problem: what it would take evaluated alone (all its
+    // samples, at most 8192 within 6 GiB), scaled down evenly when the
+    // problems together would need more than half the free device memory
+    // (each at least one sample's bytes); a context that already holds
+    // larger buffers keeps using them.  Small shares leave the largest
+    // problems many small waves after the others have finished.
+    std::vector<size_t> budget(n, 0);
+    {
+        size_t free_b = 0, total_b = 0;
+        HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        std::vector<size_t> unit(n), held(n);
+        size_t want_sum = 0;
+        for (size_t p = 0; p < n; ++p) {
+            const hase_context* ctx = b->ctx[p];
+            const size_t count = std::min<size_t>(8192, ctx->comp.max_sample_range - ctx->comp.min_sample_range + 1u);
+            unit[p] = (size_t)slices(ctx) * ctx->P * 12u + (size_t)queue_items_per_slot(ctx, tu) * 12u + 96u +
+                      kPrefixBuckets * 4u + (ctx->proposal_kind == HASE_PROPOSAL_LEARNED
+                                                 ? (size_t)slices(ctx) * ctx->C * 4u : 0u);
+            held[p] = ctx->imp.n * sizeof(float) + ctx->prefix.n * sizeof(double);
+            budget[p] = std::min(unit[p] * count, (size_t)6 << 30);
+            want_sum += budget[p] > held[p] ? budget[p] - held[p] : 0u;
+        }
+        const double scale = want_sum > free_b / 2u ? (double)(free_b / 2u) / (double)want_sum : 1.0;
+        for (size_t p = 0; p < n; ++p) {
+            const size_t extra = budget[p] > held[p] ? budget[p] - held[p] : 0u;
+            budget[p] = std::max({held[p] + (size_t)(scale * (double)extra), unit[p], held[p]});
+        }
+    }
     std::vector<uint32_t> batch(n, 0), next(n), last(n);
     std::vector<std::vector<uint32_t>> deferred(n);
     std::vector<BatchTimers> tm(n);
@@ -1524,7 +1551,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
             next[p] = last[p] + 1u;
             continue;
         }
-        batch[p] = reserve_batch(ctx, last[p] - next[p] + 1u, budget, true);
+        batch[p] = reserve_batch(ctx, last[p] - next[p] + 1u, budget[p], true);
         HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
     }
     b->counters.alloc(4);
@@ -1648,7 +1675,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
     for (size_t p = 0; p < n; ++p) {
         if (deferred[p].empty()) continue;
         hase_context* ctx = b->ctx[p];
-        const uint32_t bs = reserve_batch(ctx, (uint32_t)deferred[p].size(), budget);
+        const uint32_t bs = reserve_batch(ctx, (uint32_t)deferred[p].size(), budget[p]);
         for (size_t d0 = 0; d0 < deferred[p].size(); d0 += bs) {
             const uint32_t sl = (uint32_t)std::min<size_t>(bs, deferred[p].size() - d0);
             run_batch(ctx, results[p], 0u, deferred[p].data() + d0, sl, false, 1u, nullptr, tm[p]);
