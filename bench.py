@@ -356,7 +356,7 @@ def combined_bench(args):
     cladding, adaptive 1e5 -> 1e8 rays, mse 0.005) with a seeded smooth
     pump-depletion beta update before every evaluation, for
     ``--combined`` time steps.  One GPU evaluates one share of the sample
-    points (``--share`` contiguous points of the mid-slab level), the unit
+    points (``--share`` contiguous points from the mid-slab level on), the unit
     the path shards by; the mesh and beta are whole and resident.  Both
     forms are timed (walked map; network trained on this mesh's own random
     pump fields, with the walked-map fallback), and the last step's
@@ -373,7 +373,7 @@ def combined_bench(args):
         first, last = 0, w.samples - 1
     else:
         first = (L // 2) * Np
-        last = first + min(args.share, Np) - 1
+        last = min(first + args.share, w.samples) - 1
     whole = first == 0 and last == w.samples - 1
     beta0 = np.asarray(w.medium.value("betaVolume"), dtype=np.float64)
     factor = depletion_field(topo)
