@@ -4,7 +4,7 @@ import pytest
 
 from haseongpu_amd import CrossSectionData, GainMedium, Grid, MeshTopology, PhiASE
 from haseongpu_amd.laser import interpolateLinear
-from haseongpu_amd.workloads import disk_points, plain_small, production_disk
+from haseongpu_amd.workloads import combined_disk, depletion_field, disk_points, plain_small, production_disk
 
 
 def _check_topology(topo, area):
@@ -80,6 +80,29 @@ def test_production_disk_workload():
     assert 0.0 <= beta.min() and beta.max() <= 1.0
     sa, se, msa, mse = w.spectra.spectralPairs(False)
     assert sa.size == 64 and 0.9e-21 < msa < 1.2e-21 and 2.4e-20 < mse < 2.7e-20
+
+
+def test_combined_disk_workload_and_depletion_field():
+    w = combined_disk()
+    topo = w.topology
+    assert topo.numberOfTriangles == 20682 and topo.levels == 30
+    assert topo.numberOfPrisms == 599778 and topo.numberOfSamples == 315300
+    r = np.hypot(topo.points[:, 0], topo.points[:, 1])
+    assert abs(r.max() - 5.0) < 1e-9 and abs(topo.thickness * (topo.levels - 1) - 1.0) < 1e-12
+    sa, se, msa, mse = w.spectra.spectralPairs(False)
+    assert sa.size == 128
+    s = w.settings
+    assert (s["minRaysPerSample"], s["maxRaysPerSample"], s["mseThreshold"]) == (100_000, 100_000_000, 0.005)
+    assert s["useReflections"] and float(w.medium.value("claddingAbsorption")) == 5.5
+    beta = w.medium.value("betaVolume").reshape(topo.levels - 1, -1)
+    assert np.all(beta[:, topo.boundaryTriangles()] == 0.0)
+    f = depletion_field(topo)
+    # seeded, smooth, a depletion (factors in (0, 1]) strongest near the spots and weaker deeper in z
+    np.testing.assert_array_equal(f, depletion_field(topo))
+    assert not np.array_equal(f, depletion_field(topo, seed=6))
+    assert f.shape == (topo.numberOfPrisms,) and np.all((f > 0.95) & (f <= 1.0)) and f.min() < 0.995
+    layers = f.reshape(topo.levels - 1, -1)
+    assert np.all(layers[0] <= layers[-1] + 1e-15)
 
 
 def test_gain_medium_layouts():
