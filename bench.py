@@ -198,7 +198,9 @@ def batch_bench(args):
            "triangles": [int(w.topology.numberOfTriangles) for w in ws], "levels": [int(w.topology.levels) for w in ws],
            "rays_per_sample": list(rays), "alone_s": ta, "batched_s": tb, "speedup": ta / tb,
            "batched_rays_per_s": rays_total / tb, "rays": rays_total, "same_ray_counts": same,
-           "all_converged": bool(all(np.all(r.mse <= 0.005) for r in rb)), "importance_ms": st["importance_ms"],
+           "all_converged": bool(all(np.all(r.mse <= 0.005) for r in rb)),
+           "samples_at_ray_cap": int(sum(np.count_nonzero(~(r.mse <= 0.005) & (r.totalRays >= rays[1])) for r in rb)),
+           "importance_ms": st["importance_ms"],
            "trace_ms": st["trace_ms"], "tuning": tuning}
     print(json.dumps(out), flush=True)
 
