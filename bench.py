@@ -77,6 +77,7 @@ def cpu_baseline(w, spectra, seconds=15.0):
     first = (int(w.topology.levels) // 2) * Np
     block = max(threads, 8)
     done_rays, done_samples, elapsed = 0, 0, 0.0
+    picked, phi, mse = [], [], []
     while elapsed < seconds and done_samples < Np:
         op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_,
                             n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")),
@@ -90,12 +91,17 @@ def cpu_baseline(w, spectra, seconds=15.0):
         r = O.calc_phi_ase(om, op)
         elapsed += time.perf_counter() - t0
         done_rays += int(r["totalRays"].astype(np.int64).sum())
+        sl = slice(first + done_samples, first + done_samples + block)
+        picked.append(np.arange(sl.start, sl.stop))
+        phi.append(r["phiAse"][sl])
+        mse.append(r["mse"][sl])
         done_samples += block
         log(f"cpu baseline: {done_samples} samples, {done_rays} rays, {elapsed:.1f}s")
     return {"value": done_rays / elapsed, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{done_samples} sample points (contiguous, mid-slab level) of the same workload, "
                       f"full importance map + adaptive rays, {elapsed:.1f}s on {threads} threads",
-            "samples_per_s": done_samples / elapsed}
+            "samples_per_s": done_samples / elapsed,
+            "_oracle": (np.concatenate(picked), np.concatenate(phi), np.concatenate(mse))}
 
 
 def timed_steps(ctx, w, label, steps, warmup):
@@ -698,6 +704,17 @@ def main():
     ctx.close()
     if rank == 0 and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, spectra, args.cpu_seconds)
+        # the GPU estimates of the oracle's samples against the FP64 oracle's
+        idx, phi_o, mse_o = out["cpu_baseline"].pop("_oracle")
+        se_o = np.sqrt(mse_o) * phi_o
+        agree = {"samples": int(idx.size), "rule": "z = (phi_gpu - phi_oracle) / sqrt(se_gpu^2 + se_oracle^2)"}
+        for form, r in (("plain", p_res), ("learned", res if learned is not None else None)):
+            if r is None:
+                continue
+            z = (r.phiAse[idx].astype(np.float64) - phi_o) / np.sqrt(r.standardError[idx] ** 2 + se_o ** 2)
+            agree[form] = {"mean_z": float(np.mean(z)), "max_abs_z": float(np.max(np.abs(z))),
+                           "frac_abs_z_gt_3": float(np.mean(np.abs(z) > 3))}
+        out["oracle_agreement"] = agree
     else:
         out["cpu_baseline"] = None
     if rank == 0:
