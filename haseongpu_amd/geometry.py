@@ -25,51 +25,45 @@ __all__ = ["Grid", "MeshTopology", "GainMedium", "PropertyHandle"]
 
 
 class Grid:
-    """Regular rectangular helper that generates a 2D triangular base mesh.
+    """Axis-aligned box sampled by a regular lattice: a (nx x ny) point grid in
+    x and y, split two triangles per cell, extruded over the level planes of
+    the z axis.  The API is that of pyInclude/geometry/core.py:788-863, so a
+    Grid built with the same arguments yields the same mesh."""
 
-    ``xExtent``/``yExtent`` give the transverse footprint, ``zExtent`` and
-    ``tileSizeZ`` the sampled levels (pyInclude/geometry/core.py:788-863).
-    """
+    # spellings the reference also accepts, mapped onto the canonical keywords
+    _KEYWORD_SPELLINGS = {"x": "xExtent", "y": "yExtent", "z": "zExtent",
+                          "tilesSizeX": "tileSizeX", "tilesSizeY": "tileSizeY", "tilesSizeZ": "tileSizeZ"}
 
     def __init__(self, xExtent=None, yExtent=None, zExtent=None, tileSizeX=1.0,
-                 tileSizeY=None, tileSizeZ=None, origin=(0.0, 0.0), **aliases):
-        for short, name in (("x", "xExtent"), ("y", "yExtent"), ("z", "zExtent")):
-            if short in aliases:
-                value = aliases.pop(short)
-                if name == "xExtent" and xExtent is None:
-                    xExtent = value
-                elif name == "yExtent" and yExtent is None:
-                    yExtent = value
-                elif name == "zExtent" and zExtent is None:
-                    zExtent = value
-        for legacy, name in (("tilesSizeX", "x"), ("tilesSizeY", "y"), ("tilesSizeZ", "z")):
-            if legacy in aliases:
-                value = aliases.pop(legacy)
-                if name == "x":
-                    tileSizeX = value
-                elif name == "y":
-                    tileSizeY = value
-                else:
-                    tileSizeZ = value
-        if aliases:
-            raise TypeError(f"unknown Grid arguments: {', '.join(aliases)}")
-        if xExtent is None or yExtent is None or zExtent is None:
-            raise TypeError("Grid requires xExtent, yExtent and zExtent")
-        self.xExtent = float(xExtent)
-        self.yExtent = float(yExtent)
-        self.zExtent = float(zExtent)
-        self.tileSizeX = float(tileSizeX)
-        self.tileSizeY = float(self.tileSizeX if tileSizeY is None else tileSizeY)
-        self.tileSizeZ = float(self.tileSizeX if tileSizeZ is None else tileSizeZ)
+                 tileSizeY=None, tileSizeZ=None, origin=(0.0, 0.0), **extra):
+        given = dict(xExtent=xExtent, yExtent=yExtent, zExtent=zExtent, tileSizeX=tileSizeX,
+                     tileSizeY=tileSizeY, tileSizeZ=tileSizeZ)
+        # a short extent name may only fill an extent left unset; the old
+        # tile-size spellings always replace the tile size
+        stray = [k for k in extra if k not in self._KEYWORD_SPELLINGS
+                 or (self._KEYWORD_SPELLINGS[k].endswith("Extent") and given[self._KEYWORD_SPELLINGS[k]] is not None)]
+        if stray:
+            raise TypeError(f"Grid() got unexpected keyword(s): {', '.join(stray)}")
+        for spelling, value in extra.items():
+            given[self._KEYWORD_SPELLINGS[spelling]] = value
+        missing = [k for k in ("xExtent", "yExtent", "zExtent") if given[k] is None]
+        if missing:
+            raise TypeError(f"Grid() needs an extent for every axis; missing {', '.join(missing)}")
+        self.xExtent, self.yExtent, self.zExtent = (float(given[k]) for k in ("xExtent", "yExtent", "zExtent"))
+        self.tileSizeX = float(given["tileSizeX"])
+        self.tileSizeY = float(self.tileSizeX if given["tileSizeY"] is None else given["tileSizeY"])
+        self.tileSizeZ = float(self.tileSizeX if given["tileSizeZ"] is None else given["tileSizeZ"])
         self.origin = tuple(float(v) for v in origin)
-        if min(self.xExtent, self.yExtent, self.zExtent) <= 0.0:
-            raise ValueError("grid dimensions x, y and z must be positive")
-        if min(self.tileSizeX, self.tileSizeY, self.tileSizeZ) <= 0.0:
-            raise ValueError("tile sizes must be positive")
+        if not min(self.xExtent, self.yExtent, self.zExtent) > 0.0:
+            raise ValueError(f"Grid extents must be > 0, got {self.xExtent}, {self.yExtent}, {self.zExtent}")
+        if not min(self.tileSizeX, self.tileSizeY, self.tileSizeZ) > 0.0:
+            raise ValueError(f"Grid tile sizes must be > 0, got {self.tileSizeX}, {self.tileSizeY}, {self.tileSizeZ}")
 
     @staticmethod
     def _axisValues(extent, tileSize):
-        # half a step of slack so exactly divisible extents keep their end point
+        # lattice 0, h, 2h, ... up to the extent; the arange bound reaches half
+        # a tile past it so that an extent that is a whole number of tiles
+        # keeps its last plane despite rounding in the multiples of h
         values = np.arange(0.0, extent + tileSize * 0.5, tileSize)
         return values[values <= extent + 1e-12]
 
@@ -102,6 +96,86 @@ def _grid_triangles(nx, ny):
     second = np.stack([p01, p10, p11], axis=-1)
     rows = np.concatenate([first, second], axis=1)
     return rows.reshape(-1, 3).astype(np.uint32)
+
+
+def _distinct_points(points):
+    """(N, 2) float64 points with repeated coordinates removed, each kept at
+    its first position (core.py:555-562)."""
+    arr = np.asarray(points, dtype=np.float64)
+    if arr.ndim != 2 or arr.shape[1] != 2:
+        raise ValueError(f"points must have shape (N, 2), got {arr.shape}")
+    if arr.shape[0] < 3:
+        raise ValueError("at least three points are required")
+    seen, keep = set(), []
+    for i, (x, y) in enumerate(arr.tolist()):
+        # -0.0 and 0.0 hash and compare equal, as they do for the rows of np.unique
+        if (x, y) not in seen:
+            seen.add((x, y))
+            keep.append(i)
+    return arr[np.asarray(keep, dtype=np.int64)]
+
+
+def bowyerWatson(points):
+    """Incremental Delaunay triangulation with the insertion rules of
+    core.py:583-627, vectorised over the current triangle list.
+
+    The result depends on more than the point set: triangles whose
+    circumcircle holds a new point are removed from the list in place, the
+    cavity's boundary edges (counted in the order the removed triangles list
+    them) are re-joined to the point as (low id, high id, point), swapped to
+    counter-clockwise, and appended; the three enclosing vertices start as
+    one triangle in the iteration order of the set of their ids.  Every
+    in-circle decision uses the reference's determinant expression and its
+    1e-12 margin on float64 values, term for term, so the decisions - and
+    with them triangle order and rotation - are identical.
+    Returns (points without duplicates, (T, 3) uint32 triangles)."""
+    pts = _distinct_points(points)
+    n = pts.shape[0]
+    lo, hi = pts.min(axis=0), pts.max(axis=0)
+    span = max(hi - lo)
+    if span <= 0.0:
+        raise ValueError("points must span a two-dimensional area")
+    mid = (lo + hi) / 2.0
+    box = np.array([[mid[0] - 20.0 * span, mid[1] - span],
+                    [mid[0], mid[1] + 20.0 * span],
+                    [mid[0] + 20.0 * span, mid[1] - span]], dtype=np.float64)
+    allp = np.vstack([pts, box])
+    X, Y = allp[:, 0], allp[:, 1]
+    # the first triangle takes its vertices in the set's iteration order
+    tri = np.asarray([tuple({n, n + 1, n + 2})], dtype=np.int64)
+
+    def orientation(t):
+        ax, ay = X[t[:, 0]], Y[t[:, 0]]
+        return (X[t[:, 1]] - ax) * (Y[t[:, 2]] - ay) - (Y[t[:, 1]] - ay) * (X[t[:, 2]] - ax)
+
+    ccw = orientation(tri) > 0.0
+    for p in range(n):
+        px, py = X[p], Y[p]
+        ax, ay = X[tri[:, 0]] - px, Y[tri[:, 0]] - py
+        bx, by = X[tri[:, 1]] - px, Y[tri[:, 1]] - py
+        cx, cy = X[tri[:, 2]] - px, Y[tri[:, 2]] - py
+        det = ((ax * ax + ay * ay) * (bx * cy - cx * by)
+               - (bx * bx + by * by) * (ax * cy - cx * ay)
+               + (cx * cx + cy * cy) * (ax * by - bx * ay))
+        inside = np.where(ccw, det > 1e-12, det < -1e-12)
+        cavity = tri[inside]
+        count = {}
+        for a, b, c in cavity.tolist():
+            for e in ((a, b), (b, c), (c, a)):
+                key = (e[0], e[1]) if e[0] < e[1] else (e[1], e[0])
+                count[key] = count.get(key, 0) + 1
+        rim = [key for key, k in count.items() if k == 1]
+        new = np.empty((len(rim), 3), dtype=np.int64)
+        for i, (a, b) in enumerate(rim):
+            # counter-clockwise test on the candidate (a, b, p)
+            turn = (X[b] - X[a]) * (py - Y[a]) - (Y[b] - Y[a]) * (px - X[a])
+            new[i] = (b, a, p) if turn < 0.0 else (a, b, p)
+        tri = np.concatenate([tri[~inside], new])
+        ccw = np.concatenate([ccw[~inside], orientation(new) > 0.0])
+    keep = np.all(tri < n, axis=1)
+    if not np.any(keep):
+        raise ValueError("could not construct triangles; points may be collinear")
+    return pts, tri[keep].astype(np.uint32)
 
 
 def _orient_ccw(points, triangles):
@@ -192,14 +266,13 @@ class MeshTopology:
 
     @classmethod
     def fromPoints(cls, points, numberOfLevels=None, thickness=None):
-        """Delaunay-triangulate transverse points (core.py:583-627).  The
-        triangulation is computed with scipy's Qhull Delaunay; for points in
-        general position the Delaunay triangulation is unique, so it equals the
-        reference's Bowyer-Watson result up to triangle order."""
-        from scipy.spatial import Delaunay
-
-        pts = np.asarray(points, dtype=np.float64)
-        tri = Delaunay(pts).simplices
+        """Delaunay-triangulate transverse points by incremental insertion
+        (pyInclude/geometry/core.py:583-627, called at core.py:886-889).
+        Duplicate points are dropped (first occurrence kept); the triangle
+        list, its order and each triangle's vertex rotation are those of the
+        reference's insertion, so per-triangle arrays built against a
+        reference topology index the same triangles here."""
+        pts, tri = bowyerWatson(points)
         return cls(pts, tri, levels=numberOfLevels, thickness=thickness,
                    metadata={"source": "delaunay"})
 

@@ -41,17 +41,48 @@ class Result:
 
 
 _CONFIG_KEYS = {
-    "minRaysPerSample": "minRaysPerSample", "minRays": "minRaysPerSample", "min_rays": "minRaysPerSample",
-    "maxRaysPerSample": "maxRaysPerSample", "maxRays": "maxRaysPerSample", "max_rays": "maxRaysPerSample",
+    # legacy backward-estimator controls (per sample point)
+    "minRaysPerSample": "minRaysPerSample", "min_rays_per_sample": "minRaysPerSample",
+    "maxRaysPerSample": "maxRaysPerSample", "max_rays_per_sample": "maxRaysPerSample",
     "mseThreshold": "mseThreshold", "mse_threshold": "mseThreshold",
+    # the 2.2.0 names (pyInclude/simulation.py:130-175, 296-339)
+    "minRays": "minRays", "min_rays": "minRays", "maxRays": "maxRays", "max_rays": "maxRays",
+    "relativeStandardErrorThreshold": "relativeStandardErrorThreshold",
+    "relative_standard_error_threshold": "relativeStandardErrorThreshold",
+    "forwardRayCount": "forwardRayCount", "forward_ray_count": "forwardRayCount",
+    "propagationMode": "propagationMode", "propagation_mode": "propagationMode",
+    "reflectionMaxIterations": "reflectionMaxIterations", "reflection_max_iterations": "reflectionMaxIterations",
+    "reflectionTolerance": "reflectionTolerance", "reflection_tolerance": "reflectionTolerance",
+    "surfaceReservoirSize": "surfaceReservoirSize", "surface_reservoir_size": "surfaceReservoirSize",
+    "backend": "backend", "openpmdBackend": "openpmdBackend", "openpmd_backend": "openpmdBackend",
+    "parallelMode": "parallelMode", "parallel_mode": "parallelMode",
+    "numDevices": "numDevices", "num_devices": "numDevices", "max_gpus": "numDevices",
+    "nPerNode": "nPerNode", "n_per_node": "nPerNode", "devices": "devices",
+    # shared
     "repetitions": "repetitions", "adaptiveSteps": "adaptiveSteps", "adaptive_steps": "adaptiveSteps",
     "useReflections": "useReflections", "use_reflections": "useReflections",
     "maxReflections": "maxReflections", "monochromatic": "monochromatic",
     "rngSeed": "rngSeed", "rng_seed": "rngSeed", "minSampleRange": "minSampleRange",
-    "maxSampleRange": "maxSampleRange", "device": "device", "defensiveFraction": "defensiveFraction",
+    "min_sample_range": "minSampleRange", "maxSampleRange": "maxSampleRange",
+    "max_sample_range": "maxSampleRange", "device": "device", "defensiveFraction": "defensiveFraction",
     "raysPerThread": "raysPerThread", "sampleBatch": "sampleBatch",
     "writeVtk": "writeVtk", "write_vtk": "writeVtk", "outputPath": "outputPath", "output_path": "outputPath",
 }
+
+_DEFAULT_RAYS = 100000
+_DEFAULT_MSE = 0.005
+
+
+@dataclass
+class RunControls:
+    """The controls one evaluation runs with, after reconciling the legacy
+    per-sample names with the 2.2.0 names of ``PhiASE``."""
+
+    min_rays: int
+    max_rays: int
+    mse_threshold: float
+    adaptive_steps: int
+    device: int
 
 
 @dataclass
@@ -60,9 +91,10 @@ class PhiASE:
     crossSections: CrossSectionData | None = None
     spectralProperties: CrossSectionData | None = None
     gainMedium: GainMedium | None = None
-    minRaysPerSample: int = 100000
-    maxRaysPerSample: int = 100000
-    mseThreshold: float = 0.005
+    # legacy per-sample controls; None: take the 2.2.0 name below, else the default
+    minRaysPerSample: int | None = None
+    maxRaysPerSample: int | None = None
+    mseThreshold: float | None = None
     repetitions: int = 1
     adaptiveSteps: int = 4
     useReflections: bool = False
@@ -79,6 +111,26 @@ class PhiASE:
     ase_steps: int | None = None     # time-stepped runs: ASE in steps < ase_steps (None counts as 0)
     writeVtk: bool = False           # run(): per-sample results as VTK files under <outputPath>/vtk
     outputPath: object = "."
+    # 2.2.0 keyword surface (pyInclude/simulation.py:130-175).  minRays / maxRays
+    # are the ray counts of each sample point here (the reference's migration
+    # maps minRaysPerSample onto minRays, simulation.py:318-323);
+    # relativeStandardErrorThreshold r stops a sample at mse = RSE^2 <= r^2;
+    # forwardRayCount fixes the rays of every sample (no escalation).
+    minRays: int | None = None
+    maxRays: int | None = None
+    relativeStandardErrorThreshold: float | None = None
+    forwardRayCount: int | None = None
+    propagationMode: str | None = None   # None / "backward": this estimator; "forward" is ForwardContext
+    reflectionMaxIterations: int = 40    # surface-reservoir controls of the forward estimator; the
+    reflectionTolerance: float = 1e-4    # importance-sampled path reflects by mirror images up to
+    surfaceReservoirSize: int = 32       # maxReflections and does not read them
+    laserProperties: object | None = None
+    backend: str | None = None           # None, or a HIP / AMD GPU backend name
+    openpmdBackend: str | None = "auto"  # no openPMD transport: the library is called in-process
+    parallelMode: str = "single"
+    numDevices: int = 1
+    nPerNode: int = 1
+    devices: list = field(default_factory=list)
 
     _result: Result | None = field(default=None, init=False, repr=False)
 
@@ -89,8 +141,13 @@ class PhiASE:
             self._applyConfig(self.config)
         if self.crossSections is None and self.spectralProperties is not None:
             self.crossSections = self.spectralProperties
+        if self.crossSections is None and self.laserProperties is not None:
+            lp = self.laserProperties.toDict()
+            self.crossSections = CrossSectionData(lp["l_abs"], lp["s_abs"], lp["l_ems"], lp["s_ems"],
+                                                  resolution=lp["l_res"])
         if self.spectralProperties is None:
             self.spectralProperties = self.crossSections
+        self.controls()   # reject contradictory or unsupported settings at construction
         if self.ase_steps is not None:
             if isinstance(self.ase_steps, bool) or not isinstance(self.ase_steps, (int, np.integer)):
                 raise TypeError("PhiASE.ase_steps must be an integer or None")
@@ -122,17 +179,57 @@ class PhiASE:
             setattr(obj, name, value)
         return obj
 
+    def controls(self):
+        """Effective ray counts, threshold, schedule and device.
+
+        Each quantity may be given under its legacy name or its 2.2.0 name;
+        giving both with different meanings is an error, as are the 2.2.0
+        settings this one-GPU, importance-sampled path cannot honour (the
+        forward propagation mode, MPI, several devices, a CPU backend)."""
+        def pick(legacy_name, legacy, modern_name, modern, default, same=lambda a, b: a == b):
+            if legacy is not None and modern is not None and not same(legacy, modern):
+                raise ValueError(f"PhiASE: {legacy_name}={legacy} contradicts {modern_name}={modern}")
+            return legacy if legacy is not None else modern if modern is not None else default
+
+        mode = None if self.propagationMode is None else str(self.propagationMode).strip().lower()
+        if mode not in (None, "backward", "importance"):
+            raise ValueError(f"PhiASE runs the importance-sampled (backward) estimator; propagationMode="
+                             f"{self.propagationMode!r} is the Tet4 forward method, available as "
+                             f"haseongpu_amd.ForwardContext on a TetMesh")
+        if str(self.parallelMode).strip().lower() != "single":
+            raise ValueError(f"PhiASE schedules every sample point of a run on one GPU; parallelMode="
+                             f"{self.parallelMode!r} is not available (split the sample range instead)")
+        if int(self.numDevices) != 1 or len(self.devices) > 1:
+            raise ValueError("PhiASE runs on one GPU: numDevices must be 1 and devices may name one device")
+        if self.backend is not None and not any(k in str(self.backend).lower() for k in ("hip", "amd", "gpu")):
+            raise ValueError(f"backend {self.backend!r} is not available: this library runs on the HIP device")
+        lo = pick("minRaysPerSample", self.minRaysPerSample, "minRays", self.minRays, _DEFAULT_RAYS)
+        hi = pick("maxRaysPerSample", self.maxRaysPerSample, "maxRays", self.maxRays, max(int(lo), _DEFAULT_RAYS))
+        steps = int(self.adaptiveSteps)
+        if self.forwardRayCount is not None and int(self.forwardRayCount) > 0:
+            lo = hi = int(self.forwardRayCount)
+        rse = self.relativeStandardErrorThreshold
+        mse = pick("mseThreshold", self.mseThreshold, "relativeStandardErrorThreshold^2",
+                   None if rse is None else float(rse) ** 2, _DEFAULT_MSE,
+                   same=lambda a, b: abs(float(a) - float(b)) <= 1e-12 * max(abs(float(a)), abs(float(b))))
+        if rse is not None and float(rse) < 0.0:
+            raise ValueError("relativeStandardErrorThreshold must be non-negative")
+        device = int(self.devices[0]) if len(self.devices) == 1 else int(self.device)
+        if int(lo) <= 0:
+            raise ValueError("minRays (minRaysPerSample) must be positive")
+        if int(hi) < int(lo):
+            raise ValueError("maxRays (maxRaysPerSample) must be >= minRays (minRaysPerSample)")
+        if float(mse) < 0.0:
+            raise ValueError("mseThreshold must be non-negative")
+        return RunControls(min_rays=int(lo), max_rays=int(hi), mse_threshold=float(mse), adaptive_steps=steps,
+                           device=device)
+
     def _validate(self, medium, spectra):
         if not isinstance(medium, GainMedium):
             raise TypeError("run() needs a GainMedium")
         if spectra is None:
             raise ValueError("PhiASE needs crossSections")
-        if int(self.minRaysPerSample) <= 0:
-            raise ValueError("minRaysPerSample must be positive")
-        if int(self.maxRaysPerSample) < int(self.minRaysPerSample):
-            raise ValueError("maxRaysPerSample must be >= minRaysPerSample")
-        if float(self.mseThreshold) < 0.0:
-            raise ValueError("mseThreshold must be non-negative")
+        self.controls()
         if int(self.repetitions) < 1:
             raise ValueError("repetitions must be >= 1")
         if int(self.adaptiveSteps) < 0:
@@ -149,19 +246,20 @@ class PhiASE:
         topo = medium.topology
         S = topo.numberOfSamples
         sa, se, msa, mse_ = spectra.spectralPairs(bool(self.monochromatic))
+        ctl = self.controls()
         mesh = _native.NativeMesh(topo, medium)
-        exp = _native.NativeExperiment(sigma_a=sa, sigma_e=se, min_rays=int(self.minRaysPerSample),
-                                       max_rays=int(self.maxRaysPerSample), max_sigma_a=msa, max_sigma_e=mse_,
-                                       mse_threshold=float(self.mseThreshold),
+        exp = _native.NativeExperiment(sigma_a=sa, sigma_e=se, min_rays=ctl.min_rays,
+                                       max_rays=ctl.max_rays, max_sigma_a=msa, max_sigma_e=mse_,
+                                       mse_threshold=ctl.mse_threshold,
                                        use_reflections=bool(self.useReflections),
                                        max_reflections=int(self.maxReflections),
                                        defensive_fraction=float(self.defensiveFraction))
         seed = 5489 if self.rngSeed is None else int(self.rngSeed)
         comp = _native.make_compute(n_samples=S, max_repetitions=int(self.repetitions),
-                                    adaptive_steps=int(self.adaptiveSteps),
+                                    adaptive_steps=ctl.adaptive_steps,
                                     min_sample_range=0 if self.minSampleRange is None else int(self.minSampleRange),
                                     max_sample_range=None if self.maxSampleRange is None else int(self.maxSampleRange),
-                                    rng_seed=seed, device=int(self.device), rays_per_thread=int(self.raysPerThread),
+                                    rng_seed=seed, device=ctl.device, rays_per_thread=int(self.raysPerThread),
                                     sample_batch=int(self.sampleBatch))
         return mesh, exp, comp
 
@@ -190,12 +288,13 @@ class PhiASE:
         from .vtk_wedge import writePointsToVtk
 
         r = self._result
+        ctl = self.controls()
         folder = Path(self.outputPath) / "vtk"
         stamp = str(int(time.time()))
         for name, values in (("dndt", r.dndtAse), ("phiase", r.phiAse), ("standard_error", r.standardError),
                              ("total_rays", r.totalRays), ("relative_standard_error", r.relativeStandardError)):
-            writePointsToVtk(medium, values, folder / f"{name}_{stamp}.vtk", rays, int(self.maxRaysPerSample),
-                             float(self.mseThreshold), bool(self.useReflections), runtime)
+            writePointsToVtk(medium, values, folder / f"{name}_{stamp}.vtk", rays, ctl.max_rays,
+                             ctl.mse_threshold, bool(self.useReflections), runtime)
 
     def getResults(self):
         if self._result is None:

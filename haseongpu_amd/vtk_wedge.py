@@ -33,111 +33,145 @@ import numpy as np
 __all__ = ["vtkWedge", "writePointsToVtk", "writePrismToVtk", "compareVtk", "readVtkScalars"]
 
 VTK_WEDGE = 13
+_ABSENT = object()
 
 
-def _topology_of(geometry):
-    """MeshTopology of a MeshTopology or of a GainMedium that owns one."""
+def _mesh(obj):
+    """MeshTopology behind ``obj`` (itself, or a GainMedium's), None otherwise."""
     from .geometry import GainMedium, MeshTopology
 
-    if isinstance(geometry, GainMedium):
-        return geometry.topology
-    if isinstance(geometry, MeshTopology):
-        return geometry
-    raise TypeError("geometry must be a GainMedium or MeshTopology")
+    if isinstance(obj, MeshTopology):
+        return obj
+    if isinstance(obj, GainMedium):
+        return obj.topology
+    return None
 
 
-def _is_geometry(obj):
-    from .geometry import GainMedium, MeshTopology
-
-    return isinstance(obj, (GainMedium, MeshTopology))
-
-
-def _is_list(obj):
+def _several(obj):
     return isinstance(obj, Sequence) and not isinstance(obj, (str, bytes))
 
 
-def _label(field):
-    if isinstance(field, Mapping):
-        return "_".join(str(k) for k in field)
-    if _is_list(field):
-        return "_".join(str(k) for k in field)
-    return str(field)
+def _stored(data, key):
+    """``data``'s entry (mappings) or attribute named ``key``; _ABSENT if none."""
+    if isinstance(data, Mapping) and key in data:
+        return data[key]
+    return getattr(data, key, _ABSENT)
 
 
-def _file_path(file_name, state, field):
-    text = str(file_name)
-    if state is not None:
-        text = text.format(step=getattr(state, "step", 0), time=getattr(state, "time", 0.0), field=_label(field))
-    if not text.endswith(".vtk"):
-        text += ".vtk"
-    return Path(text)
+class _Request:
+    """One array to write: its VTK name, and either the key it is stored
+    under in the data or the array itself.  ``else_data`` lets a missing key
+    stand for the data object itself (a bare array passed as ``data``)."""
+
+    __slots__ = ("name", "key", "array", "else_data")
+
+    def __init__(self, name, key=None, array=None, else_data=False):
+        self.name, self.key, self.array, self.else_data = name, key, array, else_data
+
+    def resolve(self, data):
+        if self.key is None:
+            values = self.array
+        else:
+            values = _stored(data, self.key)
+            if values is _ABSENT:
+                if not self.else_data:
+                    raise ValueError(f"VTK array '{self.name}': the data has no entry or attribute '{self.key}'")
+                values = data
+        if values is None:
+            raise ValueError(f"VTK array '{self.name}' is None; there is nothing to write")
+        return str(self.name), np.asarray(values, dtype=np.float64)
 
 
-def _lookup(data, key):
-    """data[key] for a mapping, data.key for an object; None when absent."""
-    if isinstance(data, Mapping):
-        return data.get(key)
-    return getattr(data, key, None)
+def _requests(field, scalar_name, fields):
+    """Normalise the three ways of choosing arrays into a list of _Request.
 
-
-def _has(data, key):
-    return (key in data) if isinstance(data, Mapping) else hasattr(data, key)
-
-
-def _named_arrays(data, field, scalar_name, fields):
-    """[(name, float64 array)] selected from ``data`` by ``field`` / ``fields``."""
-    out = []
+    ``fields`` (a mapping {VTK name: array or key}) wins over ``field``.
+    ``field`` is a key, a list of keys or a mapping {VTK name: key or array};
+    ``scalar_name`` renames a key's array: one string, a list by position or
+    a mapping by key.  A key the data lacks makes the data itself the array,
+    written as ``scalars`` unless renamed."""
     if fields is not None:
         if not isinstance(fields, Mapping):
-            raise TypeError("fields must be a mapping of scalar names to arrays or attribute names")
+            raise TypeError("fields= expects a mapping {VTK scalar name: array or key in the data}")
+        out = []
         for name, source in fields.items():
-            values = _lookup(data, source) if isinstance(source, str) and _has(data, source) else source
-            if values is None:
-                raise ValueError(f"no data available for VTK field '{name}'")
-            out.append((str(name), np.asarray(values, dtype=np.float64)))
+            if isinstance(source, str):
+                out.append(_Request(name, key=source))
+            else:
+                out.append(_Request(name, array=source))
         return out
 
-    def alias(key, index):
-        if scalar_name is None:
-            return None
+    def renamed(key, position):
         if isinstance(scalar_name, Mapping):
             return scalar_name.get(key)
-        if _is_list(scalar_name):
-            return scalar_name[index]
+        if _several(scalar_name):
+            return scalar_name[position]
         return scalar_name
 
-    def one(key, name):
-        if _has(data, key):
-            values, label = _lookup(data, key), name or key
-        else:
-            values, label = data, name or "scalars"
-        if values is None:
-            raise ValueError(f"no data available for VTK field '{key}'")
-        return str(label), np.asarray(values, dtype=np.float64)
-
     if isinstance(field, Mapping):
-        for i, (name, source) in enumerate(field.items()):
-            label = alias(name, i) or name
+        out = []
+        for position, (label, source) in enumerate(field.items()):
+            name = renamed(label, position) or label
             if isinstance(source, str):
-                out.append((str(label), one(source, label)[1]))
+                out.append(_Request(name, key=source, else_data=True))
             else:
-                out.append((str(label), np.asarray(source, dtype=np.float64)))
+                out.append(_Request(name, array=source))
         return out
-    if _is_list(field):
-        return [one(key, alias(key, i)) for i, key in enumerate(field)]
-    return [one(field, alias(field, 0))]
+    keys = list(field) if _several(field) else [field]
+    return [_KeyRequest(key, renamed(key, position)) for position, key in enumerate(keys)]
 
 
-def _classify(values, topology):
-    """("POINT_DATA" | "CELL_DATA", values flattened in VTK order)."""
-    Np, T, L = topology.numberOfPoints, topology.numberOfTriangles, int(topology.levels)
-    point_shape, cell_shape = (Np, L), (T, L - 1)
-    if values.shape == point_shape or (values.ndim == 1 and values.size == Np * L):
-        return "POINT_DATA", values.reshape(-1, order="F")
-    if values.shape == cell_shape or (values.ndim == 1 and values.size == T * (L - 1)):
-        return "CELL_DATA", values.reshape(-1, order="F")
-    raise ValueError(f"VTK wedge data must match point data shape {point_shape} or cell data shape "
-                     f"{cell_shape}, got {values.shape}")
+class _KeyRequest(_Request):
+    """A key whose array is written under its own name (or ``rename``); when
+    the data lacks the key the data is the array, named ``rename`` or
+    ``scalars``."""
+
+    __slots__ = ("rename",)
+
+    def __init__(self, key, rename):
+        super().__init__(rename or key, key=key, else_data=True)
+        self.rename = rename
+
+    def resolve(self, data):
+        if _stored(data, self.key) is _ABSENT:
+            return _Request(self.rename or "scalars", array=data).resolve(data)
+        return super().resolve(data)
+
+
+def _label(field):
+    """File-name label of a field choice: its keys joined by underscores."""
+    names = list(field) if isinstance(field, Mapping) or _several(field) else [field]
+    return "_".join(str(n) for n in names)
+
+
+def _file_path(template, state, field):
+    text = str(template)
+    if state is not None:
+        text = text.format(step=getattr(state, "step", 0), time=getattr(state, "time", 0.0), field=_label(field))
+    return Path(text if text.endswith(".vtk") else text + ".vtk")
+
+
+class _WedgeSizes:
+    """Point and prism counts of the extruded mesh, and where an array goes."""
+
+    def __init__(self, topology):
+        L = int(topology.levels)
+        self.points = (topology.numberOfPoints, L)
+        self.prisms = (topology.numberOfTriangles, L - 1)
+
+    def place(self, values):
+        """("POINT_DATA" | "CELL_DATA", values in VTK order).  Matrix input is
+        (Np, L) or (T, L-1) in the mesh's column-major order; flat input of
+        either size is taken as already ordered (sample points first)."""
+        for kind, shape in (("POINT_DATA", self.points), ("CELL_DATA", self.prisms)):
+            if values.shape == shape:
+                return kind, values.reshape(-1, order="F")
+        if values.ndim == 1:
+            for kind, shape in (("POINT_DATA", self.points), ("CELL_DATA", self.prisms)):
+                if values.size == shape[0] * shape[1]:
+                    return kind, values
+        raise ValueError(f"an array of shape {values.shape} is neither per sample point {self.points} "
+                         f"nor per prism {self.prisms} of this wedge mesh")
 
 
 def _grid_lines(topology, coord_fmt):
@@ -167,64 +201,74 @@ def _g17(v):
 
 
 def _write_wedge(path, arrays, topology):
-    sections = {"POINT_DATA": [], "CELL_DATA": []}
+    sizes = _WedgeSizes(topology)
+    by_kind = {"POINT_DATA": [], "CELL_DATA": []}
     for name, values in arrays:
-        kind, flat = _classify(values, topology)
-        sections[kind].append((name, flat))
-    parts = ["# vtk DataFile Version 2.0\n", "HASEonGPU wedge output\n", "ASCII\n", "DATASET UNSTRUCTURED_GRID\n"]
-    parts += _grid_lines(topology, _g17)
-    for kind in ("POINT_DATA", "CELL_DATA"):
-        if not sections[kind]:
-            continue
-        parts.append(f"{kind} {sections[kind][0][1].size}\n")
-        for name, flat in sections[kind]:
-            parts.append(f"SCALARS {name} float 1\nLOOKUP_TABLE default\n")
-            parts.append("".join(f"{v:.17g}\n" for v in flat.tolist()))
+        kind, flat = sizes.place(values)
+        by_kind[kind].append((name, flat))
+    text = ["# vtk DataFile Version 2.0\n", "HASEonGPU wedge output\n", "ASCII\n", "DATASET UNSTRUCTURED_GRID\n"]
+    text += _grid_lines(topology, _g17)
+    for kind, entries in by_kind.items():
+        if entries:
+            text.append(f"{kind} {entries[0][1].size}\n")
+        for name, flat in entries:
+            text.append(f"SCALARS {name} float 1\nLOOKUP_TABLE default\n")
+            text.append("".join(f"{_g17(v)}\n" for v in flat.tolist()))
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    path.write_text("".join(parts), encoding="utf-8")
+    path.write_text("".join(text), encoding="utf-8")
     return path
+
+
+class _StepWriter:
+    """``Simulation.on_step`` callback: writes the chosen arrays of every
+    ``every``-th step's state on a fixed mesh; other steps return None."""
+
+    def __init__(self, template, topology, field, scalar_name, fields, every):
+        self.template, self.topology, self.every = template, topology, every
+        self.field = field
+        self.requests = _requests(field, scalar_name, fields)
+
+    def __call__(self, state):
+        period = int(self.every)
+        if period < 1:
+            raise ValueError(f"every must be a positive step count, got {self.every}")
+        if getattr(state, "step", 0) % period:
+            return None
+        return _write_wedge(_file_path(self.template, state, self.field),
+                            [r.resolve(state) for r in self.requests], self.topology)
 
 
 def vtkWedge(file_name, data=None, geometry=None, field="phiAse", scalar_name=None, every=1, fields=None):
     """Write scalar arrays on the wedge mesh to a legacy ASCII VTK file
-    (pyInclude/vtkWedge.py:203-265).
+    (the front end's writer, pyInclude/vtkWedge.py:203-265).
 
-    ``vtkWedge(name, state, fields=...)`` writes a ``TimeStepState`` (its
-    ``topology`` gives the grid); ``vtkWedge(name, values, geometry=...)``
-    writes plain arrays on a ``MeshTopology`` / ``GainMedium``; and
-    ``vtkWedge(name, geometry, ...)`` returns a callback for
-    ``Simulation.on_step`` that writes every ``every``-th step.
+    Three call forms: ``vtkWedge(name, state, fields=...)`` writes a
+    ``TimeStepState`` on the mesh the state carries; ``vtkWedge(name, values,
+    geometry=mesh)`` writes arrays on a ``MeshTopology`` / ``GainMedium``; and
+    ``vtkWedge(name, mesh, ...)`` returns a callback for ``Simulation.on_step``
+    that writes every ``every``-th step.
 
-    ``field`` names attributes or keys of the data: a string writes one array,
-    a sequence several under their own names, a mapping {alias: source}
-    several under aliases.  ``fields`` maps scalar names to arrays (or to
-    attribute names) and takes precedence.  Arrays of shape (Np, L) are
-    POINT_DATA, (T, L-1) CELL_DATA (flat arrays of those sizes too, in the
-    column-major order of the mesh); both kinds may share a file.  The file
-    name takes ``{step}``, ``{time}`` and ``{field}`` placeholders and gets a
-    ``.vtk`` suffix when it has none.  Returns the written path (the callback
-    returns None on skipped steps)."""
-    if _is_geometry(data) and geometry is None:
-        topology = _topology_of(data)
-
-        def on_step(state):
-            if int(every) <= 0:
-                raise ValueError("every must be a positive integer")
-            if getattr(state, "step", 0) % int(every) != 0:
-                return None
-            return _write_wedge(_file_path(file_name, state, field),
-                                _named_arrays(state, field, scalar_name, fields), topology)
-
-        return on_step
-
-    if geometry is None:
+    Arrays are chosen by ``field`` (a key, a list of keys, or {VTK name:
+    key or array}), renamed by ``scalar_name``, or given outright by
+    ``fields`` ({VTK name: array or key}), which wins.  (Np, L) arrays are
+    written as POINT_DATA and (T, L-1) arrays as CELL_DATA, flat arrays of
+    those sizes too; one file may hold both.  ``{step}``, ``{time}`` and
+    ``{field}`` in the name are filled from the state, and ``.vtk`` is
+    appended when missing.  Returns the path written (None from the callback
+    on skipped steps)."""
+    if geometry is None and _mesh(data) is not None:
+        return _StepWriter(file_name, _mesh(data), field, scalar_name, fields, every)
+    if geometry is not None:
+        topology = _mesh(geometry)
+        if topology is None:
+            raise TypeError(f"geometry must be a MeshTopology or GainMedium, not {type(geometry).__name__}")
+    else:
         topology = getattr(data, "topology", None)
         if topology is None:
-            raise TypeError("vtkWedge needs a TimeStepState with a topology or an explicit GainMedium/MeshTopology")
-    else:
-        topology = _topology_of(geometry)
-    return _write_wedge(_file_path(file_name, data, field), _named_arrays(data, field, scalar_name, fields), topology)
+            raise TypeError("vtkWedge cannot tell the mesh: pass geometry=, or a state that carries its topology")
+    arrays = [r.resolve(data) for r in _requests(field, scalar_name, fields)]
+    return _write_wedge(_file_path(file_name, data, field), arrays, topology)
 
 
 def _run_header(rays, max_rays, threshold, use_reflections, runtime):
@@ -234,9 +278,11 @@ def _run_header(rays, max_rays, threshold, use_reflections, runtime):
 
 
 def _write_run(kind, geometry, values, filename, rays, max_rays, threshold, use_reflections, runtime):
-    topology = _topology_of(geometry)
+    topology = _mesh(geometry)
+    if topology is None:
+        raise TypeError(f"geometry must be a MeshTopology or GainMedium, not {type(geometry).__name__}")
     values = np.asarray(values, dtype=np.float64)
-    got_kind, flat = _classify(values, topology)
+    got_kind, flat = _WedgeSizes(topology).place(values)
     if got_kind != kind:
         raise ValueError(f"{kind} needs {'(Np, L)' if kind == 'POINT_DATA' else '(T, L-1)'} values, "
                          f"got {values.shape}")

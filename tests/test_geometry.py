@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from haseongpu_amd import CrossSectionData, GainMedium, Grid, MeshTopology, PhiASE
+from haseongpu_amd.geometry import bowyerWatson
 from haseongpu_amd.laser import interpolateLinear
 from haseongpu_amd.workloads import combined_disk, depletion_field, disk_points, plain_small, production_disk
 
@@ -67,6 +68,46 @@ def test_delaunay_disk_is_delaunay_and_consistent():
     dist2 = (P[None, :, 0] - ux[:, None]) ** 2 + (P[None, :, 1] - uy[:, None]) ** 2
     assert np.all(dist2 >= r2[:, None] * (1 - 1e-9))
     assert hull_area > 0
+
+
+def test_insertion_order_of_a_fan_derived_by_hand():
+    """Points 0..2 form one triangle and point 3 lies inside it.  Before 3 is
+    inserted the only triangle without an enclosing vertex is the one made
+    when 2 arrived, on the sorted edge (0, 1), already counter-clockwise:
+    (0, 1, 2).  Point 3 lies in its circumcircle (centre (2, 2), r^2 = 8) and
+    in no circle of a triangle through a far enclosing vertex, so the cavity
+    is (0, 1, 2) alone; its edges in rotation order (0, 1), (1, 2), (2, 0)
+    become (0, 1, 3), (1, 2, 3) and (0, 2, 3) -> swapped to (2, 0, 3), the
+    last being clockwise as written (core.py:583-627)."""
+    pts, tri = bowyerWatson([[0.0, 0.0], [4.0, 0.0], [0.0, 4.0], [1.0, 1.0]])
+    assert tri.tolist() == [[0, 1, 3], [1, 2, 3], [2, 0, 3]]
+    # a repeated point is dropped where it repeats, not where it first appears
+    pts2, tri2 = bowyerWatson([[0.0, 0.0], [4.0, 0.0], [0.0, 4.0], [4.0, 0.0], [1.0, 1.0]])
+    np.testing.assert_array_equal(pts2, pts)
+    np.testing.assert_array_equal(tri2, tri)
+    with pytest.raises(ValueError):
+        bowyerWatson([[0.0, 0.0], [1.0, 1.0], [2.0, 2.0]])
+
+
+def test_insertion_makes_the_last_point_the_third_vertex():
+    """Unit square with an interior point inserted last: all four triangles
+    are made by that insertion, so each ends with point 4 and starts with
+    its rim edge in ascending order unless that order is clockwise:
+    (0,1,4) and (1,3,4) stay, (2,3,4) -> (3,2,4), (0,2,4) -> (2,0,4)."""
+    pts, tri = bowyerWatson([[0, 0], [1, 0], [0, 1], [1, 1], [0.4, 0.6]])
+    assert sorted(tri.tolist()) == [[0, 1, 4], [1, 3, 4], [2, 0, 4], [3, 2, 4]]
+    # triangles are appended when made: the list runs in order of their last vertex
+    topo = MeshTopology.fromPoints(disk_points(3.0, 400, seed=7), numberOfLevels=2, thickness=0.1)
+    t = topo.trianglePointIndices.astype(np.int64)
+    assert np.all(t[:, 2] == t.max(axis=1))
+    assert np.all(np.diff(t[:, 2]) >= 0)
+    # the rim edge keeps ascending order exactly when that is counter-clockwise
+    P = topo.points
+    a, b, c = P[t[:, 0]], P[t[:, 1]], P[t[:, 2]]
+    lo, hi = np.minimum(t[:, 0], t[:, 1]), np.maximum(t[:, 0], t[:, 1])
+    A, B = P[lo], P[hi]
+    ascending_ccw = (B[:, 0] - A[:, 0]) * (c[:, 1] - A[:, 1]) - (B[:, 1] - A[:, 1]) * (c[:, 0] - A[:, 0]) >= 0
+    assert np.array_equal(t[:, 0] < t[:, 1], ascending_ccw)
 
 
 def test_production_disk_workload():
@@ -160,11 +201,52 @@ def test_phiase_config_accepts_legacy_keys(tmp_path):
     p = PhiASE(str(cfg))
     assert (p.minRaysPerSample, p.maxRaysPerSample, p.mseThreshold, p.adaptiveSteps, p.useReflections,
             p.repetitions, p.rngSeed) == (2000, 1000000, 0.01087, 8, True, 3, 5489)
-    p2 = PhiASE({"minRays": 10, "maxRays": 5})
+    with pytest.raises(ValueError, match="maxRays"):
+        PhiASE({"minRays": 10, "maxRays": 5})
     w = plain_small()
-    with pytest.raises(ValueError, match="maxRaysPerSample"):
-        p2.run(w.medium, w.spectra)
     with pytest.raises(RuntimeError):
         PhiASE().getResults()
     with pytest.raises(ValueError, match="crossSections"):
         PhiASE().run(w.medium)
+
+
+def test_phiase_takes_the_present_keyword_surface():
+    """The 2.2.0 constructor keywords (pyInclude/simulation.py:130-175):
+    minRays / maxRays are each sample point's ray counts (the reference maps
+    minRaysPerSample onto minRays, simulation.py:318-323), a relative
+    standard error threshold r stops a sample at mse = RSE^2 <= r^2, and the
+    settings this one-GPU path cannot honour are refused by name."""
+    w = plain_small()
+    p = PhiASE(crossSections=w.spectra, minRays=1000, maxRays=2000, relativeStandardErrorThreshold=0.1)
+    c = p.controls()
+    assert (c.min_rays, c.max_rays, c.mse_threshold) == (1000, 2000, pytest.approx(0.01))
+    mesh, exp, comp = p.nativeArguments(w.medium)
+    e = exp.struct
+    assert (e.min_rays_per_sample, e.max_rays_per_sample) == (1000, 2000)
+    assert e.mse_threshold == pytest.approx(0.01, rel=1e-15)
+    # the legacy names mean the same; both at once must agree
+    legacy = PhiASE(crossSections=w.spectra, minRaysPerSample=1000, maxRaysPerSample=2000, mseThreshold=0.01)
+    lc = legacy.controls()
+    assert (lc.min_rays, lc.max_rays, lc.adaptive_steps) == (c.min_rays, c.max_rays, c.adaptive_steps)
+    assert lc.mse_threshold == pytest.approx(c.mse_threshold, rel=1e-15)
+    assert PhiASE(minRays=1000, minRaysPerSample=1000).controls().min_rays == 1000
+    with pytest.raises(ValueError, match="contradicts"):
+        PhiASE(minRays=1000, minRaysPerSample=2000)
+    with pytest.raises(ValueError, match="contradicts"):
+        PhiASE(mseThreshold=0.005, relativeStandardErrorThreshold=0.1)
+    # a fixed ray count disables escalation
+    c = PhiASE(forwardRayCount=4096, minRays=100, maxRays=10 ** 6).controls()
+    assert (c.min_rays, c.max_rays) == (4096, 4096)
+    # YAML and mapping configs under the present snake-case keys
+    c = PhiASE({"phi_ase": {"min_rays": 500, "max_rays": 5000, "relative_standard_error_threshold": 0.05,
+                            "adaptive_steps": 2, "devices": [0]}}).controls()
+    assert (c.min_rays, c.max_rays, c.adaptive_steps, c.device) == (500, 5000, 2, 0)
+    assert c.mse_threshold == pytest.approx(0.0025)
+    for bad in (dict(propagationMode="forward"), dict(parallelMode="mpi"), dict(numDevices=2),
+                dict(devices=[0, 1]), dict(backend="Host_Cpu_CpuOmpBlocks")):
+        with pytest.raises(ValueError):
+            PhiASE(**bad)
+    assert PhiASE(propagationMode="backward", backend="Hip_AmdGpu_GpuHip").controls().min_rays == 100000
+    # defaults when nothing is given: 1e5 rays per sample, mse 0.005
+    c = PhiASE().controls()
+    assert (c.min_rays, c.max_rays, c.mse_threshold) == (100000, 100000, 0.005)
