@@ -62,6 +62,7 @@ EXPORTED_SYMBOLS = (
     "hase_forward_set_reflections",
     "hase_forward_destroy",
     "hase_calc_forward_phi_ase",
+    "hase_fold_tet4_mesh",
     "hase_last_error",
     "hase_abi_version",
     "hase_build_info",
@@ -298,6 +299,42 @@ class ForwardReflections(C.Structure):
 SRM_STATUS = {0: "DISABLED", 1: "CONVERGED", 2: "STABLE", 3: "MAX_PASSES", 4: "DIVERGED"}
 
 
+class Tet4HostMesh(C.Structure):
+    _fields_ = [
+        ("number_of_mesh_points", C.c_uint32),
+        ("number_of_cells", C.c_uint32),
+        ("number_of_points", C.c_uint32),
+        ("number_of_levels", C.c_uint32),
+        ("points", _f64p),
+        ("cell_point_indices", _u32p),
+        ("beta_volume", _f64p),
+        ("cladding_cell_types", _u32p),
+        ("cell_face_boundaries", _i32p),
+        ("number_of_surfaces", C.c_uint32),
+        ("surface_reflectivities", _f32p),
+        ("surface_refractive_index_inside", _f32p),
+        ("surface_refractive_index_outside", _f32p),
+        ("number_of_reflectivities", C.c_uint32),
+        ("reflectivities", _f32p),
+        ("refractive_indices", _f32p),
+    ]
+
+
+class PrismArrays(C.Structure):
+    _fields_ = [
+        ("points", _f64p),
+        ("triangle_point_indices", _u32p),
+        ("triangle_neighbors", _i32p),
+        ("forbidden_edge", _i32p),
+        ("beta_volume", _f64p),
+        ("cladding_cell_types", _u32p),
+        ("reflectivities", _f32p),
+        ("refractive_indices", C.c_float * 4),
+        ("thickness", C.c_double),
+        ("number_of_triangles", C.c_uint32),
+    ]
+
+
 def load():
     """Load the HIP library; raise if it has not been built."""
     global _lib
@@ -356,6 +393,7 @@ def load():
     L.hase_forward_destroy.argtypes = [ctx_p]
     L.hase_calc_forward_phi_ase.argtypes = [C.POINTER(TetMeshC), C.POINTER(Experiment), C.POINTER(Compute),
                                             C.POINTER(ForwardResultC)]
+    L.hase_fold_tet4_mesh.argtypes = [C.POINTER(Tet4HostMesh), C.POINTER(PrismArrays)]
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
@@ -370,7 +408,7 @@ def load():
                  "hase_simulation_times", "hase_forward_create", "hase_forward_update_beta",
                  "hase_forward_evaluate", "hase_forward_trace", "hase_forward_stats", "hase_forward_destroy",
                  "hase_forward_set_reflections",
-                 "hase_calc_forward_phi_ase"):
+                 "hase_calc_forward_phi_ase", "hase_fold_tet4_mesh"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -407,6 +445,31 @@ class NativeMesh:
             _ptr(self.clad, C.c_uint32), int(medium.value("claddingNumber")),
             float(medium.value("claddingAbsorption")), float(medium.value("nTot")),
             float(medium.value("crystalTFluo")), _ptr(self.refl, C.c_float), _ptr(self.nidx, C.c_float))
+
+
+    @classmethod
+    def fromFolded(cls, folded, *, beta_cells, number_of_levels, cladding_number, cladding_absorption, n_tot,
+                   crystal_t_fluo):
+        """hase_mesh_t straight from the arrays of ``fold_tet4_mesh`` (no
+        topology recomputed on the host)."""
+        self = cls.__new__(cls)
+        self.points = np.ascontiguousarray(folded["points"], dtype=np.float64)
+        self.tri = np.ascontiguousarray(folded["triangles"], dtype=np.uint32)
+        self.nbr = np.ascontiguousarray(folded["neighbors"], dtype=np.int32)
+        self.fe = np.ascontiguousarray(folded["forbidden"], dtype=np.int32)
+        self.beta = np.ascontiguousarray(folded["beta"], dtype=np.float64)
+        self.beta_cells = np.ascontiguousarray(beta_cells, dtype=np.float64)
+        self.clad = np.ascontiguousarray(folded["cladding"], dtype=np.uint32)
+        self.refl = np.ascontiguousarray(folded["reflectivities"], dtype=np.float32)
+        self.nidx = np.ascontiguousarray(folded["refractive_indices"], dtype=np.float32)
+        self.struct = Mesh(
+            self.points.size // 2, int(folded["number_of_triangles"]), int(number_of_levels),
+            float(folded["thickness"]), _ptr(self.points, C.c_double), _ptr(self.tri, C.c_uint32),
+            _ptr(self.nbr, C.c_int32), _ptr(self.fe, C.c_int32), _ptr(self.beta, C.c_double),
+            _ptr(self.beta_cells, C.c_double), _ptr(self.clad, C.c_uint32), int(cladding_number),
+            float(cladding_absorption), float(n_tot), float(crystal_t_fluo), _ptr(self.refl, C.c_float),
+            _ptr(self.nidx, C.c_float))
+        return self
 
 
 class NativeExperiment:
@@ -773,6 +836,50 @@ def calc_phi_ase(mesh: NativeMesh, experiment: NativeExperiment, compute: Comput
     check(load().hase_calc_phi_ase(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
                                    C.byref(res.struct)))
     return res
+
+
+def fold_tet4_mesh(points, cells, beta_volume, number_of_points, number_of_levels, *, cladding_cell_types=None,
+                   cell_face_boundaries=None, surface_reflectivities=None, surface_refractive_index_inside=None,
+                   surface_refractive_index_outside=None, reflectivities=None, refractive_indices=None):
+    """``hase_fold_tet4_mesh``: prism arrays from the Tet4 arrays of a 2.2.0
+    HostMesh (points [3 Nm] SoA, cells [4 C], per-cell beta / types / face
+    surface ids).  Returns a dict of the hase_mesh_t arrays: points [2 Np],
+    triangles [3T] / neighbours / forbidden edges column-major, beta [P],
+    cladding types [T], reflectivities [2T], refractive indices [4],
+    thickness and T."""
+    Np, L = int(number_of_points), int(number_of_levels)
+    keep = []
+
+    def arr(x, dtype):
+        if x is None:
+            return None
+        a = np.ascontiguousarray(np.asarray(x).reshape(-1), dtype=dtype)
+        keep.append(a)
+        return a
+
+    P3, cl, b = arr(points, np.float64), arr(cells, np.uint32), arr(beta_volume, np.float64)
+    ct, fb = arr(cladding_cell_types, np.uint32), arr(cell_face_boundaries, np.int32)
+    sr, si, so = (arr(v, np.float32) for v in (surface_reflectivities, surface_refractive_index_inside,
+                                                surface_refractive_index_outside))
+    lr, ri = arr(reflectivities, np.float32), arr(refractive_indices, np.float32)
+    n_surf = max((v.size for v in (sr, si, so) if v is not None), default=0)
+    m = Tet4HostMesh(P3.size // 3, cl.size // 4, Np, L, _ptr(P3, C.c_double), _ptr(cl, C.c_uint32),
+                     _ptr(b, C.c_double), _ptr(ct, C.c_uint32), _ptr(fb, C.c_int32), n_surf, _ptr(sr, C.c_float),
+                     _ptr(si, C.c_float), _ptr(so, C.c_float), 0 if lr is None else lr.size, _ptr(lr, C.c_float),
+                     _ptr(ri, C.c_float))
+    T = (cl.size // 4) // (3 * max(L - 1, 1))
+    out = {"points": np.zeros(2 * Np), "triangles": np.zeros(3 * T, np.uint32),
+           "neighbors": np.zeros(3 * T, np.int32), "forbidden": np.zeros(3 * T, np.int32),
+           "beta": np.zeros(T * (L - 1)), "cladding": np.zeros(T, np.uint32), "reflectivities": np.zeros(2 * T, np.float32)}
+    o = PrismArrays(_ptr(out["points"], C.c_double), _ptr(out["triangles"], C.c_uint32),
+                    _ptr(out["neighbors"], C.c_int32), _ptr(out["forbidden"], C.c_int32),
+                    _ptr(out["beta"], C.c_double), _ptr(out["cladding"], C.c_uint32),
+                    _ptr(out["reflectivities"], C.c_float))
+    check(load().hase_fold_tet4_mesh(C.byref(m), C.byref(o)))
+    out["refractive_indices"] = np.array(o.refractive_indices[:], dtype=np.float32)
+    out["thickness"] = float(o.thickness)
+    out["number_of_triangles"] = int(o.number_of_triangles)
+    return out
 
 
 class NativeTetMesh:

@@ -551,6 +551,65 @@ int hase_forward_destroy(hase_forward_t* forward);
 int hase_calc_forward_phi_ase(const hase_tet_mesh_t* mesh, const hase_experiment_t* experiment,
                               const hase_compute_t* compute, hase_forward_result_t* result);
 
+/* ------------------------------------------------------------------------
+ * Prism arrays from a 2.2.0 HostMesh.  In 2.2.0 the HostMesh of an extruded
+ * slab holds Tet4 cells, three per wedge (utils/convert_vtk_topology.py:38-53
+ * splits wedge (a,b,c | d,e,f), base vertices in ascending global order, into
+ * [a,b,c,d], [b,e,c,d], [c,e,f,d]; wedge w = t + l*T becomes cells 3w..3w+2),
+ * and its legacy alias fields trianglePointIndices / triangleNeighbors /
+ * forbiddenEdge are copies of the Tet4 arrays cellPointIndices /
+ * cellNeighborCells / cellFaceBoundaries (include/core/mesh.hpp:605-625), not
+ * prism arrays.  hase_fold_tet4_mesh folds the cells back into the prism
+ * arrays hase_mesh_t takes.  Host-only: it needs no device.
+ *
+ * Each triangle comes out counter-clockwise and starting at its lowest point
+ * index (the ascending base order of the split, swapped when clockwise);
+ * neighbours and forbidden edges are derived from those triangles.  Prism
+ * beta is the tetrahedra's common value, or their volume-weighted mean when
+ * they differ; a triangle's cladding type must be the same in all its cells.
+ * Bottom / top reflectivity of triangle t is that of the base face of its
+ * lowest cells / the top face of its highest cells, looked up as
+ * getSurfaceReflectivity does (include/core/mesh.hpp:300-310: the face's
+ * surface id in surface_reflectivities when valid, else the legacy entry
+ * cell*2 + face%2); refractive indices likewise per plane (mesh.hpp:312-332),
+ * which must agree over all triangles. */
+typedef struct hase_tet4_host_mesh {
+    uint32_t number_of_mesh_points;      /* numberOfMeshPoints = Np * L */
+    uint32_t number_of_cells;            /* numberOfCells = 3 * T * (L - 1) */
+    uint32_t number_of_points;           /* numberOfPoints: points per level, Np */
+    uint32_t number_of_levels;           /* numberOfLevels, L >= 2 */
+    const double* points;                /* [3 * Nm] SoA x, y, z; point p + l*Np is p on level l */
+    const uint32_t* cell_point_indices;  /* [4 * C] cell-major */
+    const double* beta_volume;           /* [C] */
+    const uint32_t* cladding_cell_types; /* [C] or NULL */
+    const int32_t* cell_face_boundaries; /* [4 * C] or NULL: surface id of face k (opposite vertex k) */
+    uint32_t number_of_surfaces;         /* entries of the three surface tables below */
+    const float* surface_reflectivities; /* [number_of_surfaces] or NULL */
+    const float* surface_refractive_index_inside;  /* [number_of_surfaces] or NULL */
+    const float* surface_refractive_index_outside; /* [number_of_surfaces] or NULL */
+    uint32_t number_of_reflectivities;   /* entries of the legacy reflectivities */
+    const float* reflectivities;         /* legacy per-cell table or NULL */
+    const float* refractive_indices;     /* legacy [4] or NULL */
+} hase_tet4_host_mesh_t;
+
+/* Caller-owned outputs, T = C / (3 (L - 1)) triangles, P = T (L - 1). */
+typedef struct hase_prism_arrays {
+    double* points;                      /* [2 Np]: x then y */
+    uint32_t* triangle_point_indices;    /* [3T] column-major */
+    int32_t* triangle_neighbors;         /* [3T] column-major, -1 at the boundary */
+    int32_t* forbidden_edge;             /* [3T] column-major */
+    double* beta_volume;                 /* [P] level-major */
+    uint32_t* cladding_cell_types;       /* [T] or NULL */
+    float* reflectivities;               /* [2T] or NULL: t*2 + {0 bottom, 1 top} */
+    float refractive_indices[4];         /* {bottom in, bottom out, top in, top out} */
+    double thickness;                    /* distance of levels 0 and 1 */
+    uint32_t number_of_triangles;        /* T */
+} hase_prism_arrays_t;
+
+/* Replaces the legacy alias fields of HostMesh (include/core/mesh.hpp:532-538,
+ * 605-625) with real prism arrays. */
+int hase_fold_tet4_mesh(const hase_tet4_host_mesh_t* mesh, hase_prism_arrays_t* out);
+
 /* Most recent error message of this thread ("" if none). */
 const char* hase_last_error(void);
 

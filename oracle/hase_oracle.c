@@ -526,11 +526,14 @@ int orc_calc_phi_ase(const orc_mesh* m, const orc_params* prm, double* phi, doub
         double* prefix = (double*)malloc(n * sizeof(double));
         uint32_t* order = (uint32_t*)malloc(T * sizeof(uint32_t));
         orc_triangle_order(m, order);
+        const int64_t count = prm->sample_list ? (int64_t)prm->n_list
+                                               : (int64_t)prm->sample_max - (int64_t)prm->sample_min + 1;
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
-        for (int64_t si = (int64_t)prm->sample_min; si <= (int64_t)prm->sample_max; ++si) {
-            uint32_t s = (uint32_t)si;
+        for (int64_t si = 0; si < count; ++si) {
+            uint32_t s = prm->sample_list ? prm->sample_list[si] : prm->sample_min + (uint32_t)si;
+            if (s < prm->sample_min || s > prm->sample_max) continue;
             if (prm->importance) {
                 const double* src = prm->importance + (size_t)(s - prm->sample_min) * n;
                 for (size_t e = 0; e < n; ++e) imp[e] = src[e];

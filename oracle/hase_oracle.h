@@ -75,6 +75,9 @@ typedef struct {
                                    [(s - sample_min) * R*P + k*P + j], caller prism order */
     int32_t fallback;           /* with importance: samples that miss the threshold after the
                                    first launch restart on the walked map */
+    const uint32_t* sample_list;/* NULL: the range [sample_min, sample_max]; else these samples
+                                   (each within the range), spread over the threads */
+    uint32_t n_list;
 } orc_params;
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -49,6 +49,7 @@ class _Params(C.Structure):
         ("sample_max", C.c_uint32), ("seed", C.c_uint32), ("n_tot", C.c_double),
         ("tau", C.c_double), ("beta_cells", _f64p), ("threads", C.c_int32),
         ("importance", _f64p), ("fallback", C.c_int32),
+        ("sample_list", _u32p), ("n_list", C.c_uint32),
     ]
 
 
@@ -128,7 +129,7 @@ class OracleParams:
                  max_sigma_a=None, max_sigma_e=None, mse_threshold=0.005, use_reflections=False,
                  max_reflections=1, defensive=0.05, max_repetitions=1, adaptive_steps=1,
                  sample_min=0, sample_max=None, seed=5489, threads=0, n_samples=None, importance=None,
-                 fallback=False):
+                 fallback=False, samples=None):
         self.sa = np.ascontiguousarray(np.atleast_1d(sigma_a), dtype=np.float64)
         self.se = np.ascontiguousarray(np.atleast_1d(sigma_e), dtype=np.float64)
         self.bc = np.ascontiguousarray(beta_cells, dtype=np.float64)
@@ -142,7 +143,13 @@ class OracleParams:
                          msa, mse_, mse_threshold, int(bool(use_reflections)), max_reflections, defensive,
                          max_repetitions, adaptive_steps, sample_min, smax, seed, n_tot, tau,
                          _p(self.bc, C.c_double), threads,
-                         None if self.imp is None else _p(self.imp, C.c_double), int(bool(fallback)))
+                         None if self.imp is None else _p(self.imp, C.c_double), int(bool(fallback)),
+                         None, 0)
+        # an explicit sample list (any order) instead of the contiguous range
+        self.list = None if samples is None else np.ascontiguousarray(samples, dtype=np.uint32)
+        if self.list is not None:
+            self.s.sample_list = _p(self.list, C.c_uint32)
+            self.s.n_list = self.list.size
 
 
 def philox4x32_10(ctr, key):

@@ -41,7 +41,8 @@ def test_struct_layouts_match_header(tmp_path):
                "hase_run_control_t": _native.RunControl, "hase_time_state_t": _native.TimeState,
                "hase_tet_mesh_t": _native.TetMeshC, "hase_forward_result_t": _native.ForwardResultC,
                "hase_forward_stats_t": _native.ForwardStats,
-               "hase_forward_reflections_t": _native.ForwardReflections}
+               "hase_forward_reflections_t": _native.ForwardReflections,
+               "hase_tet4_host_mesh_t": _native.Tet4HostMesh, "hase_prism_arrays_t": _native.PrismArrays}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){", 'printf("{");']
     first = True
     for cname, cls in structs.items():

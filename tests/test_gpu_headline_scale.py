@@ -1,0 +1,144 @@
+"""Parity at the benchmark's own sizes.
+
+* Production disk, whole (4072 triangles, 20 levels, 77,368 prisms, 42,240
+  sample points, 64 wavelengths, first-order reflections, cladding ring,
+  adaptive 1e5 -> 1e7 rays per sample, mse <= 0.005): about 200 samples
+  stratified over levels 0, 9 and 19 - the rim points of the cladding ring,
+  the points near the disk centre and random others - are compared with the
+  FP64 oracle run on the same settings with an independent seed.  Every form
+  the benchmark times is checked: the walked importance map, the learned
+  proposal (a network trained on this mesh's random pump fields, with the
+  walked-map fallback) and beta*V sampling without a map.
+* Batching, whole ranges (64 seeded Delaunay disks of 200-8000 triangles and
+  5-40 levels, 1-4 pump spots, 1-16 wavelengths, adaptive 1e4 -> 1e7 rays,
+  mse <= 0.005) in one work queue: the same ray counts as each problem alone,
+  every sample converged or at the cap, and ~100 samples stratified over 8
+  problems against the FP64 oracle.
+
+Acceptance, statistical form of "every phi within 3 sigma":
+z = (phi_gpu - phi_oracle) / sqrt(SE_gpu^2 + SE_oracle^2); at most 2% of the
+checked samples beyond 3 (0.27% expected by chance), none beyond 5, and
+|mean z| < 0.3 (a bias of a third of a standard error would fail)."""
+import numpy as np
+import pytest
+
+from haseongpu_amd import _native
+from haseongpu_amd import proposal as prop
+from haseongpu_amd.simulation import native_problem
+from haseongpu_amd.workloads import batch_problems, production_disk
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _z(phi_a, se_a, phi_b, se_b):
+    return (phi_a - phi_b) / np.sqrt(se_a ** 2 + se_b ** 2)
+
+
+def _accept(z, label):
+    print(f"{label}: {z.size} samples, mean z {np.mean(z):+.3f}, max |z| {np.max(np.abs(z)):.2f}, "
+          f"beyond 3: {int(np.sum(np.abs(z) > 3))}", flush=True)
+    assert np.mean(np.abs(z) > 3) <= 0.02 and np.max(np.abs(z)) < 5, (label, np.sort(np.abs(z))[-5:])
+    assert abs(np.mean(z)) < 0.3, (label, np.mean(z))
+
+
+def _oracle_at(w, samples, seed):
+    s = w.settings
+    sa, se, msa, mse_ = w.spectra.spectralPairs(bool(s["monochromatic"]))
+    om = O.OracleMesh(w.topology, w.medium)
+    op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_,
+                        n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")),
+                        beta_cells=w.medium.value("betaCells"), min_rays=s["minRaysPerSample"],
+                        max_rays=s["maxRaysPerSample"], mse_threshold=s["mseThreshold"],
+                        use_reflections=s["useReflections"], max_reflections=s.get("maxReflections", 1),
+                        adaptive_steps=s["adaptiveSteps"], seed=seed, n_samples=w.samples, samples=samples)
+    ref = O.calc_phi_ase(om, op)
+    return ref["phiAse"][samples], np.sqrt(ref["mse"][samples]) * ref["phiAse"][samples], ref
+
+
+def _production_picks(w, rng):
+    """~200 samples: on each of levels 0, 9 and 19, the 22 rim points (the
+    outer ring, next to the cladding), the 22 points nearest the centre and
+    22 random others."""
+    topo = w.topology
+    Np = topo.numberOfPoints
+    r = np.hypot(topo.points[:, 0], topo.points[:, 1])
+    order = np.argsort(r)
+    centre, rim = order[:22], order[-22:]
+    middle = rng.choice(order[22:-22], 22, replace=False)
+    pts = np.concatenate([rim, centre, middle])
+    return np.concatenate([pts + lv * Np for lv in (0, 9, 19)]).astype(np.int64)
+
+
+@pytest.mark.timeout(900)
+def test_production_disk_whole_against_the_oracle():
+    w = production_disk()
+    assert (w.topology.numberOfTriangles, int(w.topology.levels), w.samples) == (4072, 20, 42240)
+    picks = _production_picks(w, np.random.default_rng(29))
+    mesh, exp, comp, _ = native_problem(w)
+    forms = {}
+    with _native.Context(mesh, exp, comp) as ctx:
+        print("production disk: walked importance map", flush=True)
+        forms["walked map"] = ctx.evaluate()
+        print("production disk: learned proposal (training)", flush=True)
+        net, _ = prop.train_on_workload(w, fields=4, samples_per_field=32, rows_per_sample=1024, seed=3, epochs=6,
+                                        cluster_triangles=16, cluster_levels=4)
+        ctx.set_proposal(net)
+        forms["learned"] = ctx.evaluate()
+        if hasattr(ctx, "set_beta_volume_proposal"):
+            print("production disk: beta*V sampling", flush=True)
+            ctx.set_beta_volume_proposal()
+            forms["beta*V"] = ctx.evaluate()
+    print(f"production disk: FP64 oracle on {picks.size} samples", flush=True)
+    phi_o, se_o, ref = _oracle_at(w, picks, seed=424242)
+    assert np.all(ref["droppedRays"][picks] == 0)
+    assert np.all(ref["mse"][picks] <= 0.005)
+    for name, res in forms.items():
+        assert np.all(res.mse <= 0.005), name                  # every sample converged
+        assert np.all(res.droppedRays == 0), name
+        z = _z(res.phiAse[picks].astype(np.float64), res.standardError[picks], phi_o, se_o)
+        _accept(z, f"production disk, {name}")
+        # each stratum on its own: rim, centre, others, on every level
+        for k, stratum in enumerate(("rim", "centre", "other")):
+            part = np.concatenate([np.arange(66 * lv + 22 * k, 66 * lv + 22 * (k + 1)) for lv in range(3)])
+            assert abs(np.mean(z[part])) < 0.6, (name, stratum, np.mean(z[part]))
+
+
+@pytest.mark.timeout(1200)
+def test_batching_whole_ranges():
+    ws = batch_problems(64, seed=0, triangles=(200, 8000), levels=(5, 40), wavelengths=(1, 16), min_rays=10_000,
+                        max_rays=10_000_000)
+    tri = np.array([w.topology.numberOfTriangles for w in ws])
+    lev = np.array([int(w.topology.levels) for w in ws])
+    assert tri.min() < 400 and tri.max() > 5000 and lev.min() <= 8 and lev.max() >= 35
+    ctxs = []
+    for w in ws:
+        mesh, exp, comp, _ = native_problem(w)
+        ctxs.append(_native.Context(mesh, exp, comp))
+    print(f"batching: {len(ws)} problems, {sum(w.samples for w in ws)} samples, alone", flush=True)
+    alone = [c.evaluate() for c in ctxs]
+    print("batching: together", flush=True)
+    with _native.Batch(ctxs) as batch:
+        together = batch.evaluate()
+    for i, (a, b) in enumerate(zip(alone, together)):
+        np.testing.assert_array_equal(a.totalRays, b.totalRays, err_msg=f"problem {i}")
+        cap = ws[i].settings["maxRaysPerSample"]
+        assert np.all((b.mse <= 0.005) | (b.totalRays >= cap)), f"problem {i}"
+        assert np.all(b.droppedRays == 0), f"problem {i}"
+        rel = np.abs(a.phiAse.astype(np.float64) - b.phiAse) / np.maximum(np.abs(a.phiAse), 1e-300)
+        assert np.max(rel) < 1e-5, f"problem {i}"      # same rays: FP64 summation order only
+    for c in ctxs:
+        c.close()
+    # ~100 samples over 8 problems from the smallest to the largest, vs the oracle
+    rng = np.random.default_rng(5)
+    chosen = np.argsort(tri)[np.linspace(0, len(ws) - 1, 8).astype(int)]
+    zs = []
+    for i in chosen:
+        w = ws[i]
+        picks = np.sort(rng.choice(w.samples, min(13, w.samples), replace=False)).astype(np.int64)
+        print(f"batching: oracle on problem {i} ({tri[i]} triangles, {lev[i]} levels), {picks.size} samples",
+              flush=True)
+        phi_o, se_o, ref = _oracle_at(w, picks, seed=777 + int(i))
+        b = together[i]
+        zs.append(_z(b.phiAse[picks].astype(np.float64), b.standardError[picks], phi_o, se_o))
+    _accept(np.concatenate(zs), "batching, 8 problems")
