@@ -260,6 +260,7 @@ MLP_INPUTS = 10
 MLP_HIDDEN = 128
 PROPOSAL_PLAIN = 0
 PROPOSAL_LEARNED = 1
+PROPOSAL_BETA_VOLUME = 2
 
 _lib = None
 
@@ -399,7 +400,7 @@ def load():
     L.hase_build_info.restype = C.c_char_p
     for name in ("hase_calc_phi_ase", "hase_context_create", "hase_context_update_beta",
                  "hase_context_evaluate", "hase_context_stats", "hase_context_trace_rays",
-                 "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
+                              "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
                  "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
                  "hase_context_proposal_logits", "hase_context_proposal_stats", "hase_context_set_tuning",
                  "hase_context_get_tuning", "hase_context_autotune", "hase_batch_create",
@@ -595,6 +596,12 @@ class Context:
                       int(proposal.cluster_triangles), int(proposal.cluster_levels), int(bool(proposal.fallback)))
         check(load().hase_context_set_proposal(self._h, C.byref(st)))
         self._proposal = (params, shift, scale)
+
+    def set_beta_volume_proposal(self):
+        """Rays in proportion to beta * V for every sample (no importance map)."""
+        st = Proposal(PROPOSAL_BETA_VOLUME, None, None, None, 1, 1, 0)
+        check(load().hase_context_set_proposal(self._h, C.byref(st)))
+        self._proposal = None
 
     def transfer_map(self, first, count):
         """(count, R, P) transfer factors of the centroid walks, caller prism order."""

@@ -84,6 +84,7 @@ struct hase_context {
     hase::DevBuf<float> phi_all;                                                // [S]
     hase::DevBuf<double> tri_area_dev;                                          // [T] device order
     hase::DevBuf<uint32_t> clad_dev;                                            // [T] device order
+    hase::DevBuf<uint32_t> user2dev_dev;                                        // [T] caller -> device triangle
 
     ~hase_context()
     {
@@ -124,6 +125,13 @@ void evaluate_context(hase_context* ctx, hase_result_t* res);
 // the sum of beta * V) by those of `beta` [P], FP64 in device prism order,
 // already on the device; block_sums holds ceil(P / 256) doubles.
 void refresh_beta_device(hase_context* ctx, const double* beta, double* block_sums);
+
+// single-ray check in the caller's units (hase_ase.hip): heights in cm, caller
+// prism order; one thread per ray
+__global__ void hase_single_rays_kernel(MeshDev m, double h, const uint32_t* user2dev, uint32_t n,
+                                        const float* start, const uint32_t* prism, const float* end,
+                                        const uint32_t* nrefl, const uint32_t* first, float* bl, float* gl, float* cl,
+                                        float* refl, uint32_t* steps, int32_t* status);
 
 // persistent work queue (queue.hip)
 void check_tuning(const hase_tuning_t& t);

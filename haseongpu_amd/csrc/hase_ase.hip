@@ -331,7 +331,7 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
         s = snext;
         const uint32_t nw = x.nw;
         bool done = at_end;
-        if (!done && nw == 0xFFFFFFFFu) {
+        if (!done && nw_boundary(nw)) {
             if (Dxy - s <= tol) done = true;
             else { ok = false; return; }
         }
@@ -349,8 +349,8 @@ __device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0,
             }
             return;
         }
-        t = nw >> 2;
-        entry = (int)(nw & 3u);
+        t = nw_next(nw);
+        entry = (int)nw_edge(nw);
     }
     ok = false;
 }
@@ -422,6 +422,19 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
                    [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf,
                    p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
+}
+
+// beta * V proposal shared by every sample (one work-group): the mixture with
+// no importance term, q_e = bv_j / (R * Bsum), its FP64 prefix, total and
+// buckets in slot 0 of the prefix tables.
+__global__ void __launch_bounds__(kBlock) hase_beta_volume_prefix_kernel(ImportanceParams p)
+{
+    const MeshDev& m = p.m;
+    const uint32_t T = m.T, P = T * m.layers;
+    __shared__ double red[kBlock / 64 + 2];
+    __shared__ double buf[kScanLds];
+    mixture_prefix(0.0, p.R * P, P, T, m.bv, p.eps, p.Bsum, p.R, [&](const EntryIndex&, uint32_t) { return 0.0f; },
+                   p.prefix, p.Q, red, buf, p.buckets);
 }
 
 // Prefix of importance maps written by split work-groups (one per sample here).
@@ -521,6 +534,7 @@ struct TraceParams {
     uint32_t* partial_drop;   // [slots * max_chunks]
     uint32_t max_chunks;
     unsigned long long* counters;
+    uint32_t shared;          // 1: every slot follows the proposal of slot 0 (beta * V sampling)
 };
 
 // blockIdx -> logical block so that consecutive logical blocks (the chunks of
@@ -543,8 +557,9 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
     const size_t n = (size_t)p.R * P;
     const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
     const SampleGeom sp = sample_point(p.pts, m.np, sample);
-    const double* pre = p.prefix + (size_t)slot * n;
-    const double Q = p.Q[slot];
+    const uint32_t own = p.shared ? 0u : slot;
+    const double* pre = p.prefix + (size_t)own * n;
+    const double Q = p.Q[own];
     const double U = systematic_offset(p.seed, sample, p.launch);
     const double scale = Q / (double)p.nrays;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -558,7 +573,7 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
         if (ray >= p.nrays) break;
         double target = ((double)ray + U) * scale;
         // first entry whose prefix exceeds the target
-        const uint32_t* bk = p.buckets + (size_t)slot * kPrefixBuckets;
+        const uint32_t* bk = p.buckets + (size_t)own * kPrefixBuckets;
         const double kt = target * (double)kPrefixBuckets;
         const uint32_t kb = kt >= (double)(kPrefixBuckets - 1u) ? kPrefixBuckets - 1u : (uint32_t)kt;
         size_t lo = bk[kb], hi = (kb + 1u < kPrefixBuckets ? (size_t)bk[kb + 1u] : n - 1) + 1;
@@ -574,17 +589,14 @@ __global__ void __launch_bounds__(kBlock) hase_trace_kernel(TraceParams p)
         philox4x32_10(ray, sample, p.launch, kTagRay, p.seed, kKeyTag, rnd);
         float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
         if (u1 + u2 > 1.0f) { u1 = 1.0f - u1; u2 = 1.0f - u2; }
-        uint4 q0, q1;
-        load_corners(m, t, q0, q1);
-        float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
-        float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
-        float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
-        float ax = x0 + u1 * (x1 - x0) + u2 * (x2 - x0);
-        float ay = y0 + u1 * (y1 - y0) + u2 * (y2 - y0);
+        const GlobalTriangles fetch;
+        const float2 p0 = fetch.vertex(m, t, 0u), p1 = fetch.vertex(m, t, 1u), p2 = fetch.vertex(m, t, 2u);
+        float ax = p0.x + u1 * (p1.x - p0.x) + u2 * (p2.x - p0.x);
+        float ay = p0.y + u1 * (p1.y - p0.y) + u2 * (p2.y - p0.y);
         float az = (float)l + u3;   // layer units
         uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
         float ez = (float)image_plane(sp.plane, (int)m.layers, nr, fp);
-        WalkOut w = walk_ray(m, ax, ay, az, t, l, sp.px, sp.py, ez, nr, fp, sp.p);
+        WalkOut w = walk_ray(m, ax, ay, az, t, l, p0, p1, p2, sp.px, sp.py, ez, nr, fp, fetch);
         steps += w.steps;
         rays++;
         if (w.status != 0 || !(w.length > 0.0f)) {
@@ -683,17 +695,28 @@ __global__ void hase_finalize_kernel(FinalizeParams p)
     }
 }
 
-__global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* start, const uint32_t* prism,
-                                        const float* end, const uint32_t* end_point, const uint32_t* nrefl,
-                                        const uint32_t* first, float* bl, float* gl, float* cl, float* refl,
-                                        uint32_t* steps, int32_t* status)
+}  // namespace
+
+namespace hase {
+
+// Single-ray check in the caller's units: start / end (x, y, z in cm) and the
+// caller's prism index.  Heights go to layer units in FP64 (an end within
+// 1e-5 layers of a plane is put on it), the prism to device triangle order,
+// then the device walk runs and its sums are written per ray.
+__global__ void hase_single_rays_kernel(MeshDev m, double h, const uint32_t* user2dev, uint32_t n,
+                                        const float* start, const uint32_t* prism, const float* end,
+                                        const uint32_t* nrefl, const uint32_t* first, float* bl, float* gl, float* cl,
+                                        float* refl, uint32_t* steps, int32_t* status)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t j = prism[i];
-    uint32_t t = j % m.T, l = j / m.T;
-    WalkOut w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
-                         end[3 * i + 2], nrefl[i], first[i], end_point[i]);
+    const uint32_t j = prism[i];
+    const uint32_t t = user2dev[j % m.T], l = j / m.T;
+    const float za = (float)((double)start[3 * i + 2] / h);
+    const double ze = (double)end[3 * i + 2] / h, zr = rint(ze);
+    const float zef = (float)(fabs(ze - zr) < 1e-5 ? zr : ze);
+    const WalkOut w = walk_ray(m, start[3 * i], start[3 * i + 1], za, t, l, end[3 * i], end[3 * i + 1], zef,
+                               nrefl[i], first[i]);
     bl[i] = w.beta_len;
     gl[i] = w.gain_len;
     cl[i] = w.clad_len;
@@ -701,6 +724,12 @@ __global__ void hase_single_rays_kernel(MeshDev m, uint32_t n, const float* star
     steps[i] = w.steps;
     status[i] = w.status;
 }
+
+}  // namespace hase
+
+namespace {
+
+using namespace hase;
 
 // Flux of the resident slots into the flux of every sample.
 __global__ void hase_phi_scatter_kernel(const float* __restrict__ phi, const uint32_t* __restrict__ sample_of,
@@ -925,6 +954,7 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         const uint32_t t = ctx->dev2user[td];
         uint32_t v[3];
         uint32_t nw[3];
+        const uint32_t clad = mesh->cladding_cell_types && mesh->cladding_cell_types[t] == mesh->cladding_number ? 1u : 0u;
         for (int k = 0; k < 3; ++k) {
             v[k] = mesh->triangle_point_indices[t + (size_t)k * T];
             ctx->tri_vertex[3u * td + k] = v[k];
@@ -935,14 +965,14 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
                     throw HaseError(HASE_E_INVALID, "triangle_neighbors / forbidden_edge out of range");
                 nw[k] = (ctx->user2dev[(uint32_t)nb] << 2) | (uint32_t)fe;
             } else {
-                nw[k] = 0xFFFFFFFFu;
+                nw[k] = 0x7FFFFFFFu;
             }
+            nw[k] |= clad << 31;   // the owning triangle's cladding flag rides in every neighbour word
         }
         double cr = (px[v[1]] - px[v[0]]) * (py[v[2]] - py[v[0]]) - (py[v[1]] - py[v[0]]) * (px[v[2]] - px[v[0]]);
         if (!(cr > 0.0))
             throw HaseError(HASE_E_INVALID, "triangles must be non-degenerate and counter-clockwise");
         ctx->tri_area[td] = 0.5 * cr;
-        uint32_t clad = mesh->cladding_cell_types && mesh->cladding_cell_types[t] == mesh->cladding_number ? 1u : 0u;
         ctx->clad_flag[td] = clad;
         auto fb = [](double x) { float f = (float)x; uint32_t u; std::memcpy(&u, &f, 4); return u; };
         rec[3u * td] = make_uint4(fb(px[v[0]]), fb(py[v[0]]), fb(px[v[1]]), fb(py[v[1]]));
@@ -953,6 +983,7 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         ctx->centroid_host[td] = make_float2((x0 + x1 + x2) * (1.0f / 3.0f), (y0 + y1 + y2) * (1.0f / 3.0f));
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
+    ctx->user2dev_dev.upload(ctx->user2dev.data(), T, ctx->stream);
     if (T <= hase::kLdsTriangles) {
         // compact records for LDS staging: corners + 21-bit neighbour fields + cladding bit
         std::vector<uint4> lrec(2u * (size_t)T);
@@ -961,7 +992,8 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
             const uint32_t nwk[3] = {b.z, b.w, c.x};
             uint64_t W = 0;
             for (int k = 0; k < 3; ++k) {
-                const uint64_t f = nwk[k] == 0xFFFFFFFFu ? 0x1FFFFFull : (uint64_t)nwk[k];
+                const uint64_t f = nw_boundary(nwk[k]) ? 0x1FFFFFull
+                                                       : (uint64_t)(nw_next(nwk[k]) << 2 | nw_edge(nwk[k]));
                 W |= f << (21 * k);
             }
             W |= (uint64_t)(c.y >> 31) << 63;
@@ -1054,6 +1086,7 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
     if (n_ent >= ((size_t)1 << 32))
         throw HaseError(HASE_E_INVALID, "reflection slices times prisms must stay below 2^32");
     const bool learned = ctx->proposal_kind == HASE_PROPOSAL_LEARNED;
+    const bool shared = ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME;
     const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
     const uint32_t rays_per_block = kBlock * rpt;
     const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
@@ -1062,7 +1095,7 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
         const uint32_t per_item = ctx->tuning.mode == HASE_TRACE_QUEUE
                                       ? (uint32_t)queue_items_per_slot(ctx, ctx->tuning)
                                       : max_chunks;
-        size_t per_sample = n_ent * 12u + (size_t)per_item * 12u + 96u + kPrefixBuckets * 4u +
+        size_t per_sample = (shared ? 0u : n_ent * 12u + kPrefixBuckets * 4u) + (size_t)per_item * 12u + 96u +
                             (learned ? (size_t)R * ctx->C * 4u : 0u);
         if (budget == 0) {
             if (ctx->batch_budget == 0) {
@@ -1079,9 +1112,11 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
         batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(budget / per_sample, 8192));
     }
     batch = std::min(batch, want);
-    ctx->imp.alloc((size_t)batch * n_ent);
-    ctx->prefix.alloc((size_t)batch * n_ent);
-    ctx->buckets.alloc((size_t)batch * kPrefixBuckets);
+    // the shared beta * V proposal keeps one slot of prefix tables
+    const size_t tables = shared ? 1u : batch;
+    if (!shared) ctx->imp.alloc(tables * n_ent);
+    ctx->prefix.alloc(tables * n_ent);
+    ctx->buckets.alloc(tables * kPrefixBuckets);
     if (learned) ctx->logits.alloc((size_t)batch * R * ctx->C);
     ctx->Q.alloc(batch);
     ctx->slot_sample.alloc(batch);
@@ -1120,11 +1155,33 @@ uint32_t importance_parts(const hase_context* ctx, uint32_t slots)
     return std::min<uint32_t>(64u, (want + slots - 1u) / slots);
 }
 
+// The beta * V proposal every sample shares, into slot 0 of the prefix
+// tables (once per evaluation: beta may have changed since the last).
+void build_shared_prefix(hase_context* ctx, BatchTimers& tm)
+{
+    const hase_experiment_t& e = ctx->exp;
+    ImportanceParams ip{};
+    ip.m = mesh_view(ctx);
+    ip.R = slices(ctx);
+    ip.eps = e.defensive_fraction;
+    ip.Bsum = ctx->Bsum;
+    ip.prefix = ctx->prefix.p;
+    ip.Q = ctx->Q.p;
+    ip.buckets = ctx->buckets.p;
+    hipEvent_t i0 = record(ctx);
+    hipLaunchKernelGGL(hase_beta_volume_prefix_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, ip);
+    HIP_CHECK(hipGetLastError());
+    tm.imp.emplace_back(i0, record(ctx));
+    ctx->stats.importance_launches++;
+}
+
 // Proposal prefix and Q of slots [0, slots): the walked importance map or the
-// learned proposal (which needs contiguous samples b0 + slot).
+// learned proposal (which needs contiguous samples b0 + slot); nothing for
+// the beta * V proposal, built once per evaluation by build_shared_prefix.
 void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, uint32_t slots, bool learned,
                     BatchTimers& tm)
 {
+    if (ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME && !learned) return;
     const hase_experiment_t& e = ctx->exp;
     const uint32_t R = slices(ctx);
     const MeshDev m = mesh_view(ctx);
@@ -1277,6 +1334,7 @@ void run_batch(hase_context* ctx, hase_result_t* res, uint32_t b0, const uint32_
         tp.partial_drop = ctx->partial_drop.p;
         tp.max_chunks = max_chunks;
         tp.counters = ctx->counters.p;
+        tp.shared = ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME ? 1u : 0u;
         hipEvent_t t0 = record(ctx);
         hipLaunchKernelGGL(hase_trace_kernel, dim3(active * chunks), dim3(kBlock), 0, ctx->stream, tp);
         HIP_CHECK(hipGetLastError());
@@ -1384,6 +1442,7 @@ void evaluate(hase_context* ctx, hase_result_t* res)
 
     BatchTimers tm;
     hipEvent_t ev_begin = record(ctx);
+    if (ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME) build_shared_prefix(ctx, tm);
     std::vector<uint32_t> deferred;
     std::vector<uint32_t>* defer = (learned && ctx->proposal_fallback) ? &deferred : nullptr;
     for (uint32_t b0 = first; b0 <= last; b0 += batch) {
@@ -1513,9 +1572,10 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         for (size_t p = 0; p < n; ++p) {
             const hase_context* ctx = b->ctx[p];
             const size_t count = std::min<size_t>(8192, ctx->comp.max_sample_range - ctx->comp.min_sample_range + 1u);
-            unit[p] = (size_t)slices(ctx) * ctx->P * 12u + (size_t)queue_items_per_slot(ctx, tu) * 12u + 96u +
-                      kPrefixBuckets * 4u + (ctx->proposal_kind == HASE_PROPOSAL_LEARNED
-                                                 ? (size_t)slices(ctx) * ctx->C * 4u : 0u);
+            const bool shared = ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME;
+            unit[p] = (shared ? 0u : (size_t)slices(ctx) * ctx->P * 12u + kPrefixBuckets * 4u) +
+                      (size_t)queue_items_per_slot(ctx, tu) * 12u + 96u +
+                      (ctx->proposal_kind == HASE_PROPOSAL_LEARNED ? (size_t)slices(ctx) * ctx->C * 4u : 0u);
             held[p] = ctx->imp.n * sizeof(float) + ctx->prefix.n * sizeof(double);
             budget[p] = std::min(unit[p] * count, (size_t)6 << 30);
             want_sum += budget[p] > held[p] ? budget[p] - held[p] : 0u;
@@ -1553,6 +1613,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         }
         batch[p] = reserve_batch(ctx, last[p] - next[p] + 1u, budget[p], true);
         HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        if (ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME) build_shared_prefix(ctx, tm[p]);
     }
     b->counters.alloc(4);
     HIP_CHECK(hipMemsetAsync(b->counters.p, 0, 4 * sizeof(unsigned long long), b->stream));
@@ -1736,6 +1797,7 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
     const uint32_t slots = reserve_batch(ctx, want);
     HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
     BatchTimers tm;
+    if (ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME) build_shared_prefix(ctx, tm);
     build_proposal(ctx, first, nullptr, slots, learned, tm);
     auto time_it = [&](const hase_tuning_t& t) {
         ctx->tuning = t;
@@ -1881,47 +1943,18 @@ int hase_context_trace_rays(hase_context_t* context, uint32_t n, const float* st
             if (start_prism[i] >= context->P) throw HaseError(HASE_E_INVALID, "start_prism out of range");
         HIP_CHECK(hipSetDevice(context->device));
         hipStream_t st = context->stream;
-        // heights to layer units; an end point within 1e-5 layers of a plane is on it
-        std::vector<float> hs(start, start + 3u * (size_t)n), he(end, end + 3u * (size_t)n);
-        for (uint32_t i = 0; i < n; ++i) {
-            hs[3 * i + 2] = (float)((double)start[3 * i + 2] / context->h);
-            double ze = (double)end[3 * i + 2] / context->h, zr = std::nearbyint(ze);
-            he[3 * i + 2] = (float)(std::fabs(ze - zr) < 1e-5 ? zr : ze);
-        }
-        std::vector<uint32_t> dev_prism(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            uint32_t j = start_prism[i];
-            dev_prism[i] = context->user2dev[j % context->T] + (j / context->T) * context->T;
-        }
-        // the mesh point under each end (x, y), compared as stored in FP32
-        std::unordered_map<uint64_t, uint32_t> at;
-        at.reserve(context->pts_host.size());
-        auto key = [](float x, float y) {
-            uint32_t a, b;
-            std::memcpy(&a, &x, 4);
-            std::memcpy(&b, &y, 4);
-            return ((uint64_t)a << 32) | b;
-        };
-        for (uint32_t i = 0; i < (uint32_t)context->pts_host.size(); ++i)
-            at.emplace(key(context->pts_host[i].x, context->pts_host[i].y), i);
-        std::vector<uint32_t> end_point(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            auto it = at.find(key(end[3 * i], end[3 * i + 1]));
-            end_point[i] = it == at.end() ? hase::kNoPoint : it->second;
-        }
         DevBuf<float> d_start, d_end, d_bl, d_gl, d_cl, d_rf;
-        DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps, d_pe;
+        DevBuf<uint32_t> d_prism, d_nr, d_fp, d_steps;
         DevBuf<int32_t> d_status;
-        d_start.upload(hs.data(), 3u * (size_t)n, st);
-        d_end.upload(he.data(), 3u * (size_t)n, st);
-        d_pe.upload(end_point.data(), n, st);
-        d_prism.upload(dev_prism.data(), n, st);
+        d_start.upload(start, 3u * (size_t)n, st);
+        d_end.upload(end, 3u * (size_t)n, st);
+        d_prism.upload(start_prism, n, st);
         d_nr.upload(reflections, n, st);
         d_fp.upload(first_plane, n, st);
         d_bl.alloc(n); d_gl.alloc(n); d_cl.alloc(n); d_rf.alloc(n); d_steps.alloc(n); d_status.alloc(n);
-        hipLaunchKernelGGL(hase_single_rays_kernel, dim3((n + 127) / 128), dim3(128), 0, st, mesh_view(context), n,
-                           d_start.p, d_prism.p, d_end.p, d_pe.p, d_nr.p, d_fp.p, d_bl.p, d_gl.p, d_cl.p, d_rf.p,
-                           d_steps.p, d_status.p);
+        hipLaunchKernelGGL(hase_single_rays_kernel, dim3((n + 127) / 128), dim3(128), 0, st, mesh_view(context),
+                           context->h, context->user2dev_dev.p, n, d_start.p, d_prism.p, d_end.p, d_nr.p, d_fp.p,
+                           d_bl.p, d_gl.p, d_cl.p, d_rf.p, d_steps.p, d_status.p);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(beta_length, d_bl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(gain_length, d_gl.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -1990,8 +2023,8 @@ int hase_context_set_proposal(hase_context_t* context, const hase_proposal_t* pr
 {
     return guarded([&] {
         if (!context || !proposal) throw HaseError(HASE_E_INVALID, "context and proposal must not be NULL");
-        if (proposal->kind == HASE_PROPOSAL_PLAIN) {
-            context->proposal_kind = HASE_PROPOSAL_PLAIN;
+        if (proposal->kind == HASE_PROPOSAL_PLAIN || proposal->kind == HASE_PROPOSAL_BETA_VOLUME) {
+            context->proposal_kind = proposal->kind;
             return;
         }
         if (proposal->kind != HASE_PROPOSAL_LEARNED) throw HaseError(HASE_E_INVALID, "unknown proposal kind");

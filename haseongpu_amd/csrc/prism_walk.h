@@ -25,13 +25,16 @@ HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float
 
 // Triangle records, 48 B per triangle t at byte 48 t:
 //   {x0, y0, x1, y1}  {x2, y2, n0, n1}  {n2, v0 | cladding << 31, v1, v2}
-// with vertex coordinates as FP32 bits, n_k = neighbour << 2 | shared edge
-// index inside the neighbour (0xFFFFFFFF at the boundary) and v_k the mesh
-// point index of vertex k.
+// with vertex coordinates as FP32 bits (vertex k at byte 8 k), v_k the mesh
+// point index of vertex k and n_k the neighbour word of edge k (vertex k to
+// k+1): the neighbour << 2 | the shared edge's index inside the neighbour in
+// bits 0-30 (all ones at the boundary) and this triangle's cladding flag in
+// bit 31, so every neighbour word read also tells the walk what its current
+// triangle is made of.
 // Compact records, 32 B per triangle for staging the mesh in LDS:
 //   {x0, y0, x1, y1}  {x2, y2, W & 0xFFFFFFFF, W >> 32}
-// with W = f0 | f1 << 21 | f2 << 42 | cladding << 63 and f_k = n_k (21 bits:
-// neighbour < 2^19, shared edge index) or 0x1FFFFF at the boundary.
+// with W = f0 | f1 << 21 | f2 << 42 | cladding << 63 and f_k = neighbour << 2 |
+// shared edge (21 bits: neighbour < 2^19) or 0x1FFFFF at the boundary.
 constexpr uint32_t kLdsTriangles = 4800;   // 150 KiB of LDS
 
 struct MeshDev {
@@ -136,6 +139,12 @@ HASE_HD inline float load_f32(const float* table, uint32_t index)
 
 HASE_HD inline uint32_t record_cladding(const uint4& q2) { return q2.y >> 31; }
 
+// Neighbour word of the 48-byte records: next triangle, its entry edge,
+// boundary test (bit 31 is the owning triangle's cladding flag).
+HASE_HD inline uint32_t nw_next(uint32_t w) { return (w >> 2) & 0x1FFFFFFFu; }
+HASE_HD inline uint32_t nw_edge(uint32_t w) { return w & 3u; }
+HASE_HD inline bool nw_boundary(uint32_t w) { return (w & 0x7FFFFFFFu) == 0x7FFFFFFFu; }
+
 // A fetch gives the walk one triangle in the layout of the 48-byte records,
 // as three vectors (kept as separate values so that the walk's selects stay
 // selects of values, not of addresses into a stack copy):
@@ -150,6 +159,24 @@ struct GlobalTriangles {
     {
         load_triangle(m, t, a, b, c);
     }
+    // side-test walk: vertex v of triangle t
+    HASE_HD float2 vertex(const MeshDev& m, uint32_t t, uint32_t v) const
+    {
+        const char* base = reinterpret_cast<const char*>(m.rec);
+        return *reinterpret_cast<const float2*>(base + mul24(t, 48u) + 8u * v);
+    }
+    // neighbour words of edges e1 and e2 of t, and t's cladding flag
+    HASE_HD void words(const MeshDev& m, uint32_t t, uint32_t e1, uint32_t e2, uint32_t& w1, uint32_t& w2,
+                       uint32_t& clad) const
+    {
+        const char* base = reinterpret_cast<const char*>(m.rec) + mul24(t, 48u) + 24u;
+        w1 = *reinterpret_cast<const uint32_t*>(base + 4u * e1);
+        w2 = *reinterpret_cast<const uint32_t*>(base + 4u * e2);
+        clad = w1 >> 31;
+    }
+    HASE_HD static uint32_t next(uint32_t w) { return nw_next(w); }
+    HASE_HD static uint32_t edge(uint32_t w) { return nw_edge(w); }
+    HASE_HD static bool boundary(uint32_t w) { return nw_boundary(w); }
     // the end is a corner other than the one facing the exit edge
     HASE_HD static int touches(const uint4&, const uint4&, const uint4& c, bool c1, bool c2, uint32_t pe, float,
                                float)
@@ -175,6 +202,21 @@ struct LdsTriangles {
         b = make_uint4(q1.x, q1.y, q1.z & 0x1FFFFFu, ((q1.z >> 21) | (q1.w << 11)) & 0x1FFFFFu);
         c = make_uint4((q1.w >> 10) & 0x1FFFFFu, q1.w & 0x80000000u, 0u, 0u);
     }
+    HASE_HD float2 vertex(const MeshDev&, uint32_t t, uint32_t v) const
+    {
+        return reinterpret_cast<const float2*>(rec)[4u * t + v];
+    }
+    HASE_HD void words(const MeshDev&, uint32_t t, uint32_t e1, uint32_t e2, uint32_t& w1, uint32_t& w2,
+                       uint32_t& clad) const
+    {
+        const uint64_t W = reinterpret_cast<const uint64_t*>(rec)[4u * t + 3u];
+        w1 = (uint32_t)(W >> (21u * e1)) & 0x1FFFFFu;
+        w2 = (uint32_t)(W >> (21u * e2)) & 0x1FFFFFu;
+        clad = (uint32_t)(W >> 63);
+    }
+    HASE_HD static uint32_t next(uint32_t w) { return w >> 2; }
+    HASE_HD static uint32_t edge(uint32_t w) { return w & 3u; }
+    HASE_HD static bool boundary(uint32_t w) { return w == 0x1FFFFFu; }
     HASE_HD static int touches(const uint4& a, const uint4& b, const uint4&, bool c1, bool c2, uint32_t, float ex,
                                float ey)
     {
@@ -225,67 +267,130 @@ HASE_HD inline float fast_rcp(float x)
 }
 
 // Prism walk from a (inside triangle t, layer `layer` of copy 0) to the
-// unfolded end point e.  Heights are given in layer units (zeta = z / h), so
-// an end point on a level plane (every sample point and its mirror images)
-// sits on an exact integer plane, and the crossing of that plane is the end
-// of the walk.  Crossing parameters are measured from a; the exit edge is
-// the nearest one whose outward normal faces the ray (entry edge excluded);
-// an exit edge with an end point at the end's (x, y) - the mesh point pe of
-// the sample, or of its mirror images - meets the ray exactly at the end;
-// reflections are the copy boundaries of the unfolded z axis.
+// unfolded end point e, parametrised by sigma in [0, 1] along the straight
+// line (xy and height are both linear in sigma; lengths are sigma * D).
+// Heights are in layer units (zeta = z / h), so an end point on a level plane
+// (every sample point and its mirror images) sits on an exact integer plane
+// whose crossing is the end of the walk.
+//
+// Triangle steps use side tests against the line through a and e: with
+// d = e - a in xy, a vertex p has side c(p) = d x (p - a) (left of the line
+// when >= 0) and along-line coordinate w(p) = d . (p - a).  The walk carries
+// the two ends of the edge it entered through, L (left) and R (right), with
+// their c and w; in the next triangle only the third vertex N is new: left
+// of the line, the line leaves through edge R-N, else through N-L, and it
+// crosses that edge where c vanishes, at
+//   sigma = (c_L w_R - c_R w_L) / ((c_L - c_R) |d|^2).
+// One vertex and one side test per triangle instead of three edge
+// intersections.  A triangle whose new vertex is the end point's (x, y) is
+// the last one (the line reaches it at sigma = 1, exactly).  Level-plane
+// crossings are steps of their own; reflections are the copy boundaries of
+// the unfolded height axis.
 //
 // The walk is resumable: walk_begin sets up a WalkState and walk_advance
 // takes up to `budget` prism steps, so a lane can hand a finished ray back
 // and start another while its neighbours are still walking.
 struct WalkState {
-    float ax, ay, za;         // start (x, y in cm; z in layer units)
+    float ax, ay, za;         // start (x, y in cm; height in layer units)
     float ex, ey;             // end (x, y)
-    float vx, vy, D, kz, tol, sin2;
+    float dx, dy, inv_d2;     // e - a in xy and 1 / |e - a|^2
+    float D, inv_dz, tol, sin2;
     int zdir, end_plane, end_on_plane;
     uint32_t nrefl, first;
-    uint32_t pe;              // mesh point under the end (kNoPoint: none)
     uint32_t t;               // current triangle
-    int u, copy, r, entry;    // unfolded layer, mirror copy, layer in copy, entry edge
-    float s;                  // distance walked
+    uint32_t clad;            // its cladding flag
+    float cl, wl, cr, wr;     // side and along-line values of the entry edge's ends L and R
+    float sxy;                // sigma where the line leaves t (1: the end lies in t)
+    uint32_t wexit;           // neighbour word across that edge
+    int u, copy, r;           // unfolded layer, mirror copy, layer in copy
+    float sig;                // sigma reached
+    float beta;               // beta of the current prism
     uint32_t applied;         // surface factors applied
-    float beta_len, clad_len, refl;   // gain-prism length = walked length - clad_len
+    float beta_sig, clad_sig, refl;   // sum beta * dsigma, sum dsigma over cladding, surface factors
     uint32_t steps;
     int status;               // 0 ok, 1 left the mesh, 2 step limit
     int done;
 };
 
-HASE_HD inline void walk_begin(WalkState& w, float h, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                               float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe)
+HASE_HD inline float side_of(float dx, float dy, float x, float y) { return dx * y - dy * x; }
+HASE_HD inline float along(float dx, float dy, float x, float y) { return dx * x + dy * y; }
+
+// sigma at which the line crosses the edge from L (c >= 0) to R (c < 0)
+HASE_HD inline float edge_sigma(float cl, float wl, float cr, float wr, float inv_d2)
 {
-    w.beta_len = 0.0f; w.clad_len = 0.0f; w.refl = 1.0f;
+    return (cl * wr - cr * wl) * inv_d2 * fast_rcp(cl - cr);
+}
+
+template <bool kReflect>
+HASE_HD inline int folded_level(int u, int copy, int r, int layers)
+{
+    return kReflect ? ((copy & 1) ? (layers - 1 - r) : r) : u;
+}
+
+// Starts a walk in triangle t (corners p0, p1, p2 as the fetch returns them).
+template <class Fetch>
+HASE_HD inline void walk_begin(const MeshDev& m, WalkState& w, float ax, float ay, float za, uint32_t t,
+                               uint32_t layer, float2 p0, float2 p1, float2 p2, float ex, float ey, float ze,
+                               uint32_t nrefl, uint32_t first, const Fetch& fetch)
+{
+    const float h = m.h;
+    w.beta_sig = 0.0f; w.clad_sig = 0.0f; w.refl = 1.0f;
     w.steps = 0; w.status = 0; w.done = 0;
     w.ax = ax; w.ay = ay; w.za = za; w.ex = ex; w.ey = ey;
-    w.nrefl = nrefl; w.first = first; w.pe = pe;
-    const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
-    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
-    w.D = D;
-    if (!(D > 0.0f)) {
-        w.done = 1;
-        return;
-    }
-    const float inv = 1.0f / D;
-    w.vx = dx * inv;
-    w.vy = dy * inv;
-    const float vz = dz * inv;
-    w.sin2 = fmaxf(0.0f, 1.0f - vz * vz);
-    w.tol = 2e-5f * (1.0f + D);
-    // parameter per layer unit of height, and the end's plane index
-    w.kz = dzeta != 0.0f ? D / dzeta : 0.0f;
-    w.zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
-    w.end_plane = (int)floorf(ze + 0.5f);
-    w.end_on_plane = (float)w.end_plane == ze;
+    w.nrefl = nrefl; w.first = first;
     w.t = t;
     w.u = (int)layer;
     w.copy = 0;
     w.r = (int)layer;
-    w.entry = -1;
-    w.s = 0.0f;
+    w.sig = 0.0f;
     w.applied = 0;
+    const float dx = ex - ax, dy = ey - ay, dzeta = ze - za, dz = dzeta * h;
+    const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+    w.D = D;
+    w.dx = dx;
+    w.dy = dy;
+    if (!(D > 0.0f)) {
+        w.done = 1;
+        return;
+    }
+    const float d2 = dx * dx + dy * dy;
+    w.inv_d2 = d2 > 0.0f ? 1.0f / d2 : 0.0f;
+    w.sin2 = d2 / (D * D);
+    w.tol = 2e-5f * (1.0f + D);
+    w.inv_dz = dzeta != 0.0f ? 1.0f / dzeta : 0.0f;
+    w.zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
+    w.end_plane = (int)floorf(ze + 0.5f);
+    w.end_on_plane = (float)w.end_plane == ze;
+    // exit of the start triangle: the edge whose start vertex is right of the
+    // line and whose end vertex is left of it (counter-clockwise order)
+    const float c0 = side_of(dx, dy, p0.x - ax, p0.y - ay), c1 = side_of(dx, dy, p1.x - ax, p1.y - ay);
+    const float c2 = side_of(dx, dy, p2.x - ax, p2.y - ay);
+    const bool b0 = c0 >= 0.0f, b1 = c1 >= 0.0f, b2 = c2 >= 0.0f;
+    int k = (!b0 && b1) ? 0 : (!b1 && b2) ? 1 : (!b2 && b0) ? 2 : -1;
+    if (k < 0) {
+        // every vertex on one side (the line grazes a vertex, or rounding):
+        // leave past the vertex furthest to the right, or enter left of the
+        // one furthest to the left
+        if (b0) {
+            k = (c0 <= c1 && c0 <= c2) ? 0 : (c1 <= c2 ? 1 : 2);
+        } else {
+            const int km = (c0 >= c1 && c0 >= c2) ? 0 : (c1 >= c2 ? 1 : 2);
+            k = km == 0 ? 2 : km - 1;
+        }
+    }
+    const float2 pr = k == 0 ? p0 : (k == 1 ? p1 : p2);
+    const float2 pl = k == 0 ? p1 : (k == 1 ? p2 : p0);
+    w.cr = k == 0 ? c0 : (k == 1 ? c1 : c2);
+    w.cl = k == 0 ? c1 : (k == 1 ? c2 : c0);
+    w.wr = along(dx, dy, pr.x - ax, pr.y - ay);
+    w.wl = along(dx, dy, pl.x - ax, pl.y - ay);
+    uint32_t wk, unused;
+    fetch.words(m, t, (uint32_t)k, (uint32_t)k, wk, unused, w.clad);
+    w.wexit = wk;
+    const bool at_end = (p0.x == ex && p0.y == ey) || (p1.x == ex && p1.y == ey) || (p2.x == ex && p2.y == ey) ||
+                        d2 == 0.0f;
+    w.sxy = at_end ? 1.0f : edge_sigma(w.cl, w.wl, w.cr, w.wr, w.inv_d2);
+    w.beta = load_f32(m.beta, t + mul24(layer, m.T));
 }
 
 // kReflect = false is the direct walk (start and end inside the slab), which
@@ -301,75 +406,36 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
             return;
         }
         w.steps++;
-        const uint32_t t = w.t;
-        const int level = kReflect ? ((w.copy & 1) ? (layers - 1 - w.r) : w.r) : w.u;
-        // the triangle record and this prism's beta are independent loads
-        uint4 qa, qb, qc;
-        fetch(m, t, qa, qb, qc);
-        const float beta = load_f32(m.beta, t + mul24((uint32_t)level, m.T));
-        const float ax = w.ax, ay = w.ay, vx = w.vx, vy = w.vy, D = w.D;
-        const float X0 = bits_to_float(qa.x) - ax, Y0 = bits_to_float(qa.y) - ay;
-        const float X1 = bits_to_float(qa.z) - ax, Y1 = bits_to_float(qa.w) - ay;
-        const float X2 = bits_to_float(qb.x) - ax, Y2 = bits_to_float(qb.y) - ay;
-        // edge k joins vertex k and k+1; crossing parameter num / nv for nv > 0
-        const float ex0 = X1 - X0, ey0 = Y1 - Y0;
-        const float ex1 = X2 - X1, ey1 = Y2 - Y1;
-        const float ex2 = X0 - X2, ey2 = Y0 - Y2;
-        const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
-        float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
-        float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
-        float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
-        s0 = (w.entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
-        s1 = (w.entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
-        s2 = (w.entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-        // the winning edge through the end's (x, y) meets the ray exactly
-        // there; decided on the vertices' point indices, which no rounding of
-        // the crossing parameters can perturb
-        const ExitEdge x = exit_edge<Fetch>(qa, qb, qc, s0, s1, s2, w.pe, w.ex, w.ey);
-        const float best = (x.touches & (int)(x.s < INFINITY)) ? D : x.s;
-        // next level plane along the ray (exactly D for the end's plane)
+        // next level plane along the line (exactly sigma 1 for the end's plane)
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
-        float sz = w.zdir != 0 ? ((float)plane - w.za) * w.kz : INFINITY;
-        sz = (w.end_on_plane && plane == w.end_plane && w.zdir != 0) ? D : sz;
-        float snext = fminf(fminf(best, sz), D);
-        snext = fmaxf(snext, w.s);
-        const float seg = snext - w.s;
-        const uint32_t clad = record_cladding(qc);
+        float sz = w.zdir != 0 ? ((float)plane - w.za) * w.inv_dz : INFINITY;
+        sz = (w.end_on_plane && plane == w.end_plane && w.zdir != 0) ? 1.0f : sz;
+        const bool zstep = sz <= w.sxy;
+        float snext = fminf(fminf(w.sxy, sz), 1.0f);
+        snext = fmaxf(snext, w.sig);
+        const float seg = snext - w.sig;
+        w.clad_sig += w.clad ? seg : 0.0f;
         // the walk's beta copy is zero in cladding prisms
-        w.clad_len += clad ? seg : 0.0f;
-        w.beta_len = fmaf(beta, seg, w.beta_len);
-        w.s = snext;
-        const uint32_t nw = x.nw;
-        const bool zstep = sz <= best;
-        bool done = w.s >= D;
-        if (!done && !zstep && nw == Fetch::kBoundary) {
-            // leaving the mesh sideways: only rounding short of the end (a sample
-            // point on the rim) counts as arrival
-            if (D - w.s <= w.tol) done = true;
-            else {
-                w.status = 1;
-                w.done = 1;
-                return;
-            }
-        }
-        if (done) {
+        w.beta_sig = fmaf(w.beta, seg, w.beta_sig);
+        w.sig = snext;
+        if (w.sig >= 1.0f) {
             if (kReflect) {
                 // an end point on a reflecting plane takes its last factor here
                 while (w.applied < w.nrefl) {
-                    uint32_t pl = (w.applied & 1u) ? 1u - w.first : w.first;
-                    w.refl *= surface_factor(m, t, pl, w.sin2);
+                    const uint32_t pl = (w.applied & 1u) ? 1u - w.first : w.first;
+                    w.refl *= surface_factor(m, w.t, pl, w.sin2);
                     w.applied++;
                 }
             }
             w.done = 1;
             return;
         }
-        if (kReflect) {
-            if (zstep) {
+        if (zstep) {
+            if (kReflect) {
                 if (w.zdir > 0) {
                     if (++w.r == layers) {
                         if (w.applied < w.nrefl) {
-                            w.refl *= surface_factor(m, t, (w.copy & 1) ? 0u : 1u, w.sin2);
+                            w.refl *= surface_factor(m, w.t, (w.copy & 1) ? 0u : 1u, w.sin2);
                             w.applied++;
                         }
                         w.r = 0;
@@ -378,7 +444,7 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
                 } else {
                     if (--w.r < 0) {
                         if (w.applied < w.nrefl) {
-                            w.refl *= surface_factor(m, t, (w.copy & 1) ? 1u : 0u, w.sin2);
+                            w.refl *= surface_factor(m, w.t, (w.copy & 1) ? 1u : 0u, w.sin2);
                             w.applied++;
                         }
                         w.r = layers - 1;
@@ -386,19 +452,45 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
                     }
                 }
             }
+            w.u += w.zdir;
+        } else {
+            if (Fetch::boundary(w.wexit)) {
+                // leaving the mesh sideways: only rounding short of the end (a
+                // sample point on the rim) counts as arrival
+                w.done = 1;
+                if ((1.0f - w.sig) * w.D > w.tol) w.status = 1;
+                return;
+            }
+            // enter the neighbour through its edge f: L = vertex f, R = vertex f+1, N = vertex f+2
+            const uint32_t t = Fetch::next(w.wexit), f = Fetch::edge(w.wexit);
+            const uint32_t e1 = f == 2u ? 0u : f + 1u, v = f == 0u ? 2u : f - 1u;
+            const float2 pn = fetch.vertex(m, t, v);
+            uint32_t wr_n, wl_n;
+            fetch.words(m, t, e1, v, wr_n, wl_n, w.clad);
+            const float X = pn.x - w.ax, Y = pn.y - w.ay;
+            const float c = side_of(w.dx, w.dy, X, Y), a = along(w.dx, w.dy, X, Y);
+            const bool left = c >= 0.0f;
+            // left of the line: leave through R-N (edge f+1), N becomes L; else through N-L (edge f+2)
+            w.cl = left ? c : w.cl;
+            w.wl = left ? a : w.wl;
+            w.cr = left ? w.cr : c;
+            w.wr = left ? w.wr : a;
+            w.wexit = left ? wr_n : wl_n;
+            const bool at_end = (pn.x == w.ex) & (pn.y == w.ey);
+            w.sxy = at_end ? 1.0f : edge_sigma(w.cl, w.wl, w.cr, w.wr, w.inv_d2);
+            w.t = t;
         }
-        w.u = zstep ? w.u + w.zdir : w.u;
-        w.t = zstep ? t : (nw >> 2);
-        w.entry = zstep ? w.entry : (int)(nw & 3u);
+        const int level = folded_level<kReflect>(w.u, w.copy, w.r, layers);
+        w.beta = load_f32(m.beta, w.t + mul24((uint32_t)level, m.T));
     }
 }
 
 HASE_HD inline WalkOut walk_result(const WalkState& s)
 {
     WalkOut w;
-    w.beta_len = s.beta_len;
-    w.gain_len = s.s - s.clad_len;
-    w.clad_len = s.clad_len;
+    w.beta_len = s.beta_sig * s.D;
+    w.gain_len = (s.sig - s.clad_sig) * s.D;
+    w.clad_len = s.clad_sig * s.D;
     w.refl = s.refl;
     w.length = s.D;
     w.steps = s.steps;
@@ -406,24 +498,34 @@ HASE_HD inline WalkOut walk_result(const WalkState& s)
     return w;
 }
 
+// Whole walk of one ray from triangle t (corners as the fetch returns them).
 template <bool kReflect, class Fetch = GlobalTriangles>
 HASE_HD inline WalkOut walk_ray_t(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                  float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe,
-                                  const Fetch& fetch = Fetch())
+                                  float2 p0, float2 p1, float2 p2, float ex, float ey, float ze, uint32_t nrefl,
+                                  uint32_t first, const Fetch& fetch = Fetch())
 {
     WalkState s;
-    walk_begin(s, m.h, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe);
+    walk_begin(m, s, ax, ay, za, t, layer, p0, p1, p2, ex, ey, ze, nrefl, first, fetch);
     walk_advance<kReflect>(m, s, kMaxSteps + 1u, fetch);
     return walk_result(s);
 }
 
 template <class Fetch = GlobalTriangles>
 HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer,
-                                float ex, float ey, float ze, uint32_t nrefl, uint32_t first, uint32_t pe,
-                                const Fetch& fetch = Fetch())
+                                float2 p0, float2 p1, float2 p2, float ex, float ey, float ze, uint32_t nrefl,
+                                uint32_t first, const Fetch& fetch = Fetch())
 {
-    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, ex, ey, ze, 0u, first, pe, fetch)
-                      : walk_ray_t<true>(m, ax, ay, za, t, layer, ex, ey, ze, nrefl, first, pe, fetch);
+    return nrefl == 0 ? walk_ray_t<false>(m, ax, ay, za, t, layer, p0, p1, p2, ex, ey, ze, 0u, first, fetch)
+                      : walk_ray_t<true>(m, ax, ay, za, t, layer, p0, p1, p2, ex, ey, ze, nrefl, first, fetch);
+}
+
+// Whole walk with the start triangle's corners read through the fetch.
+template <class Fetch = GlobalTriangles>
+HASE_HD inline WalkOut walk_ray(const MeshDev& m, float ax, float ay, float za, uint32_t t, uint32_t layer, float ex,
+                                float ey, float ze, uint32_t nrefl, uint32_t first, const Fetch& fetch = Fetch())
+{
+    return walk_ray(m, ax, ay, za, t, layer, fetch.vertex(m, t, 0u), fetch.vertex(m, t, 1u), fetch.vertex(m, t, 2u),
+                    ex, ey, ze, nrefl, first, fetch);
 }
 
 // Height, in layer units, of the mirror image of a sample point on plane

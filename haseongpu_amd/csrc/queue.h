@@ -32,6 +32,7 @@ struct ProblemDev {
     size_t n;                  // R * P entries per slot
     double mse_threshold, phi_scale;
     uint32_t n_lambda, R, seed, slot0, sample_base, n_targets, max_rays, reps, launch0, defer;
+    uint32_t shared;           // 1: every slot reads the proposal of slot 0 (beta * V sampling)
     float inv_lambda;
 };
 

@@ -127,13 +127,17 @@ struct ItemCtx {
     uint32_t sample, launch, r0, seed;
 };
 
-// Start point, end image and proposal probability of ray `ray` of an item.
+// Start point, start triangle's corners, end image and proposal probability
+// of ray `ray` of an item.
 struct RayStart {
     float ax, ay, az, ez;
+    float2 p0, p1, p2;
     uint32_t t, l, nr, fp;
 };
 
-__device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, double& q, uint32_t& jout)
+template <class Fetch>
+__device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, double& q, uint32_t& jout,
+                                             const Fetch& fetch)
 {
     const uint32_t T = m.T, P = T * m.layers;
     const double target = ((double)ray + c.U) * c.scale;
@@ -154,14 +158,12 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
     philox4x32_10(ray, c.sample, c.launch, kTagRay, c.seed, kKeyTag, rnd);
     float u1 = u32_to_unit(rnd[0]), u2 = u32_to_unit(rnd[1]), u3 = u32_to_unit(rnd[2]);
     if (u1 + u2 > 1.0f) { u1 = 1.0f - u1; u2 = 1.0f - u2; }
-    uint4 q0, q1;
-    load_corners(m, t, q0, q1);
-    const float x0 = bits_to_float(q0.x), y0 = bits_to_float(q0.y);
-    const float x1 = bits_to_float(q0.z), y1 = bits_to_float(q0.w);
-    const float x2 = bits_to_float(q1.x), y2 = bits_to_float(q1.y);
     RayStart r;
-    r.ax = x0 + u1 * (x1 - x0) + u2 * (x2 - x0);
-    r.ay = y0 + u1 * (y1 - y0) + u2 * (y2 - y0);
+    r.p0 = fetch.vertex(m, t, 0u);
+    r.p1 = fetch.vertex(m, t, 1u);
+    r.p2 = fetch.vertex(m, t, 2u);
+    r.ax = r.p0.x + u1 * (r.p1.x - r.p0.x) + u2 * (r.p2.x - r.p0.x);
+    r.ay = r.p0.y + u1 * (r.p1.y - r.p0.y) + u2 * (r.p2.y - r.p0.y);
     r.az = (float)l + u3;   // layer units
     r.t = t;
     r.l = l;
@@ -173,8 +175,9 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
 
 __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, LaneRay& lr)
 {
-    const RayStart r = draw_ray(m, c, ray, lr.q, lr.j);
-    walk_begin(lr.w, m.h, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p);
+    const GlobalTriangles fetch;
+    const RayStart r = draw_ray(m, c, ray, lr.q, lr.j, fetch);
+    walk_begin(m, lr.w, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, fetch);
 }
 
 //
@@ -228,9 +231,10 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
             c.seed = pd.seed;
             c.sp = sample_point(pd.pts, m.np, c.sample);
             c.n = pd.n;
-            c.pre = pd.prefix + (size_t)local * pd.n;
-            c.bk = pd.buckets + (size_t)local * kPrefixBuckets;
-            c.Q = pd.Q[local];
+            const uint32_t own = pd.shared ? 0u : local;   // slot whose proposal the rays follow
+            c.pre = pd.prefix + (size_t)own * pd.n;
+            c.bk = pd.buckets + (size_t)own * kPrefixBuckets;
+            c.Q = pd.Q[own];
             c.U = systematic_offset(pd.seed, c.sample, c.launch);
             c.scale = c.Q / (double)nrays;
 
@@ -257,13 +261,18 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                 for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
                     double qprob;
                     uint32_t j;
-                    const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j);
                     WalkOut w;
-                    if constexpr (kLds)
-                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p,
-                                     LdsTriangles{lds_rec});
-                    else
-                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, c.sp.px, c.sp.py, r.ez, r.nr, r.fp, c.sp.p);
+                    if constexpr (kLds) {
+                        const LdsTriangles fetch{lds_rec};
+                        const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
+                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
+                                     r.fp, fetch);
+                    } else {
+                        const GlobalTriangles fetch;
+                        const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
+                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
+                                     r.fp, fetch);
+                    }
                     finish(w, qprob, j);
                 }
             } else {
@@ -480,6 +489,7 @@ ProblemDev queue_problem(const hase_context* ctx, uint32_t slot0, uint32_t base,
     pd.reps = ctx->comp.max_repetitions;
     pd.launch0 = launch0;
     pd.defer = defer ? 1u : 0u;
+    pd.shared = ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME ? 1u : 0u;
     pd.inv_lambda = 1.0f / (float)e.number_of_wavelengths;
     return pd;
 }

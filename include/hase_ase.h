@@ -222,6 +222,13 @@ int hase_context_importance(hase_context_t* context,
 
 #define HASE_PROPOSAL_PLAIN 0
 #define HASE_PROPOSAL_LEARNED 1
+/* Rays drawn in proportion to beta * V (each reflection slice 1/R of it),
+ * one table shared by every sample point and no per-sample map: the
+ * estimator without importance information, the way the present method
+ * chooses its ray sources (include/kernels/forwardPhiAseMapping.hpp:21-49
+ * with the scan of include/core/calcForwardPhiAse.hpp:347-378).  Only the
+ * kind field of hase_proposal_t is read. */
+#define HASE_PROPOSAL_BETA_VOLUME 2
 
 typedef struct hase_train {
     uint32_t epochs;
@@ -233,7 +240,7 @@ typedef struct hase_train {
 } hase_train_t;
 
 typedef struct hase_proposal {
-    int32_t kind;                   /* HASE_PROPOSAL_PLAIN or HASE_PROPOSAL_LEARNED */
+    int32_t kind;                   /* HASE_PROPOSAL_PLAIN, _LEARNED or _BETA_VOLUME */
     const float* params;            /* [hase_mlp_param_count()] FP32 weights (kept as BF16 on the device) */
     const float* input_shift;       /* [HASE_MLP_INPUTS] input normalisation: (x - shift) * scale */
     const float* input_scale;       /* [HASE_MLP_INPUTS] */

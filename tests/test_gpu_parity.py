@@ -219,3 +219,33 @@ def test_compare_serial_fixture(name):
     # dndtAse = (beta (sigma_e + sigma_a) - sigma_a) * phiAse (compareSerial README)
     gain = med.value("betaCells") * (sp["sigmaE"] + sp["sigmaA"]) - sp["sigmaA"]
     np.testing.assert_allclose(res.dndtAse, gain * phi, rtol=1e-12)
+
+
+def test_beta_volume_proposal_same_streams():
+    """HASE_PROPOSAL_BETA_VOLUME (no map: rays in proportion to beta*V, each
+    reflection slice 1/R) against the oracle fed an all-zero external
+    importance, whose defensive mixture is then beta*V / (R sum beta*V)
+    alone: identical rays, the same-stream tolerance, with reflections and
+    cladding on and an escalating schedule."""
+    w = production_disk(levels=6, n_points=500, wavelengths=8)
+    first, last = 500, 899
+    ctx, spectra = _context(w, min_rays=4096, max_rays=32768, mse_threshold=1e-3, adaptive_steps=3,
+                            use_reflections=True, max_reflections=1, rng_seed=23, min_sample_range=first,
+                            max_sample_range=last)
+    with ctx:
+        ctx.set_beta_volume_proposal()
+        res = ctx.evaluate()
+        st = ctx.stats()
+    R = 3
+    imp = np.zeros((last - first + 1, R * w.topology.numberOfPrisms))
+    om, op = _oracle(w, spectra, min_rays=4096, max_rays=32768, mse_threshold=1e-3, adaptive_steps=3,
+                     use_reflections=True, max_reflections=1, seed=23, sample_min=first, sample_max=last,
+                     importance=imp)
+    ref = O.calc_phi_ase(om, op)
+    sl = slice(first, last + 1)
+    assert st["importance_walks"] == 0                       # no map was walked
+    same = res.totalRays[sl] == ref["totalRays"][sl]
+    assert same.mean() > 0.97 and len(np.unique(res.totalRays[sl])) > 1
+    rel = np.abs(res.phiAse[sl][same].astype(np.float64) / ref["phiAse"][sl][same] - 1.0)
+    assert np.median(rel) <= 2e-5 and np.quantile(rel, 0.99) <= 2e-4, np.sort(rel)[-5:]
+    assert res.droppedRays[sl].sum() == 0

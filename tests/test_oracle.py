@@ -278,10 +278,8 @@ def test_oracle_reproduces_legacy_serial_solver(name):
 
 
 def test_oracle_matches_true_cladding_fixture_scale():
-    """Legacy importance-sampled phiASE/mse at 2000 rays/sample, cladding
-    absorption 5.5 cm^-1: the 2000-ray oracle sits on the same level, and the
-    legacy mse has the scale of the squared relative standard error (the exact
-    legacy mse formula is not in the reference and stays unpinned)."""
+    """Legacy importance-sampled phiASE at 2000 rays/sample, cladding
+    absorption 5.5 cm^-1: the 2000-ray oracle sits on the same level."""
     topo, med, sp, ref = load_true_cladding(1)
     om = O.OracleMesh(topo, med)
     op = O.OracleParams(sigma_a=sp["sigmaA"], sigma_e=sp["sigmaE"], n_tot=float(med.value("nTot")),
@@ -291,8 +289,45 @@ def test_oracle_matches_true_cladding_fixture_scale():
     assert np.all(r["totalRays"] == ref["totalRays"])
     assert abs(np.median(r["phiAse"] / ref["phiASE"]) - 1.0) < 0.01
     assert abs(r["phiAse"].mean() / ref["phiASE"].mean() - 1.0) < 0.02
-    ratio = np.median(r["mse"]) / np.median(ref["mse"])
-    assert 1 / 3 < ratio < 3
+
+
+@pytest.mark.parametrize("index", [0, 1, 2])
+def test_legacy_mse_is_the_squared_relative_standard_error(index):
+    """The legacy mse (F2: importance-sampled, 2000 rays per sample on 4210
+    sample points; its formula is not in the reference) against this
+    estimator's RSE^2 on the same mesh and ray count.
+
+    The estimator has an unbounded variance (a ray may start next to the
+    sample point, 1/r^2), so single-run RSEs scatter over a decade and the
+    tails of the two distributions follow the two samplers' different ray
+    allocations.  The interquartile range carries the scale: the geometric
+    mean over q in [0.25, 0.75] of Q_legacy(q) / Q(q) is 1.45-1.60 for RSE^2
+    across absorptions and seeds.  The window [1.2, 2.0] admits RSE^2 and
+    rejects the other candidate definitions, whose ratios are a fixed
+    multiple of it: mse = RSE (~0.05), 2 RSE^2 (~0.76), RSE^2 / 2 (~3.0).  The
+    factor ~1.5 is the legacy sampler's larger variance at equal rays, not a
+    property of the formula; it stays unpinned.  The reference's threshold
+    translations (mse 0.1 ~ RSE 0.14 / 0.10, tests/data/compareSerial/
+    README.md:83; 0.01087 ~ 0.05, fixed_legacy_reflection_pump3_wedge_reference/
+    README.md:76) relate the legacy mse to the forward Tet4 estimator's RSE,
+    a different estimator at a different effort, so they do not bear on it."""
+    topo, med, sp, ref = load_true_cladding(index)
+    om = O.OracleMesh(topo, med)
+    op = O.OracleParams(sigma_a=sp["sigmaA"], sigma_e=sp["sigmaE"], n_tot=float(med.value("nTot")),
+                        tau=float(med.value("crystalTFluo")), beta_cells=med.value("betaCells"),
+                        min_rays=ref["raysPerSample"], seed=ref["rngSeed"], mse_threshold=1.0)
+    rse2 = O.calc_phi_ase(om, op)["mse"]
+    qs = np.linspace(0.25, 0.75, 5)
+
+    def scale(model):
+        return float(np.exp(np.mean(np.log(np.quantile(ref["mse"], qs) / np.quantile(model, qs)))))
+
+    assert 1.2 <= scale(rse2) <= 2.0, scale(rse2)
+    # the alternatives the window excludes
+    for wrong in (np.sqrt(rse2), 2.0 * rse2, 0.5 * rse2):
+        assert not (1.2 <= scale(wrong) <= 2.0)
+    # the legacy values are finite, positive and capped as the fixture states
+    assert np.all(ref["mse"] > 0) and ref["mse"].max() < 0.13
 
 
 def test_cladding_absorption_lowers_flux_monotonically():

@@ -1,0 +1,174 @@
+"""The device prism walk (haseongpu_amd/csrc/prism_walk.h) compiled for the
+host and checked against the FP64 oracle walk on identical rays (CPU).
+
+The same source the GPU kernels use is built by hipcc for the host
+(tests/walk_host/walk_host.cpp) and walked over meshes packed here into the
+device's two record layouts (48-byte global records, 32-byte LDS records),
+with the device's conventions: heights in layer units, end points on level
+planes exactly, beta zero in cladding prisms.  Criterion as on the GPU: the
+single-ray gain exp(E_walk - E_oracle) within 1e-5 of 1, the reflectance
+product within 1e-6, the same status, the path length within 2e-5, and the
+prism count equal on 99% of the rays and within 3 on the others (a line
+through a mesh vertex, common on the structured true-cladding mesh, may
+count a zero-length prism on one side and not on the other)."""
+import ctypes as C
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from haseongpu_amd.geometry import GainMedium, MeshTopology
+from haseongpu_amd.laser import CrossSectionData
+from haseongpu_amd.workloads import Workload, plain_small, production_disk
+from oracle import oracle as O
+from tests.helpers import load_true_cladding
+
+HERE = Path(__file__).resolve().parent / "walk_host"
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+pytestmark = pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc is needed to build the host walk")
+
+
+@pytest.fixture(scope="module")
+def walk_lib(tmp_path_factory):
+    out = tmp_path_factory.mktemp("walk") / "walk_host.so"
+    subprocess.run([HIPCC, "-O2", "-fPIC", "-shared", "--cuda-host-only", "-x", "hip", str(HERE / "walk_host.cpp"),
+                    "-o", str(out)], check=True, capture_output=True)
+    lib = C.CDLL(str(out))
+    u32p, f32p, i32p = C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    lib.walk_rays_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, f32p, f32p, f32p, C.c_uint32,
+                                   f32p, u32p, f32p, u32p, u32p, f32p, u32p, i32p]
+    return lib
+
+
+def _f32_exact(w):
+    t = w.topology
+    topo = MeshTopology(t.points.astype(np.float32).astype(np.float64), t.trianglePointIndices, levels=t.levels,
+                        thickness=float(np.float32(t.thickness)))
+    return Workload(w.name, topo, GainMedium(topo, dict(w.medium.physical)), w.spectra, w.settings)
+
+
+def _records(w):
+    """(48-byte records as uint32 [12T], 32-byte records [8T] or None, beta FP32)
+    in the caller's triangle order, packed as the library packs them."""
+    topo, med = w.topology, w.medium
+    T, L = topo.numberOfTriangles, int(topo.levels)
+    d = topo.topology()
+    tri = topo.trianglePointIndices.astype(np.int64)
+    nb = d["triangleNeighbors"].reshape(3, T).T
+    fe = d["forbiddenEdge"].reshape(3, T).T
+    clad = (med.value("claddingCellTypes") == int(med.value("claddingNumber"))).astype(np.uint32)
+    xy = topo.points.astype(np.float32)
+    fb = lambda a: np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)   # noqa: E731
+    nw = np.where(nb >= 0, (nb.astype(np.uint64) << 2) | fe.astype(np.uint64), 0x7FFFFFFF).astype(np.uint64)
+    nw |= clad[:, None].astype(np.uint64) << 31
+    rec = np.zeros((T, 12), np.uint32)
+    for k in range(3):
+        rec[:, 2 * k] = fb(xy[tri[:, k], 0])
+        rec[:, 2 * k + 1] = fb(xy[tri[:, k], 1])
+    rec[:, 6], rec[:, 7], rec[:, 8] = nw[:, 0], nw[:, 1], nw[:, 2]
+    rec[:, 9] = tri[:, 0].astype(np.uint32) | (clad << 31)
+    rec[:, 10], rec[:, 11] = tri[:, 1], tri[:, 2]
+    lds = np.zeros((T, 8), np.uint32)
+    lds[:, :6] = rec[:, :6]
+    W = np.zeros(T, np.uint64)
+    for k in range(3):
+        f = np.where(nb[:, k] >= 0, (nb[:, k].astype(np.uint64) << 2) | fe[:, k].astype(np.uint64), 0x1FFFFF)
+        W |= f.astype(np.uint64) << np.uint64(21 * k)
+    W |= clad.astype(np.uint64) << np.uint64(63)
+    lds[:, 6] = (W & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    lds[:, 7] = (W >> np.uint64(32)).astype(np.uint32)
+    beta = med.value("betaVolume").astype(np.float32).reshape(L - 1, T)
+    beta[:, clad == 1] = 0.0
+    return rec.reshape(-1), lds.reshape(-1), beta.reshape(-1)
+
+
+def _rays(w, om, n, seed, reflections):
+    rng = np.random.default_rng(seed)
+    topo = w.topology
+    T, L, Np = topo.numberOfTriangles, int(topo.levels), topo.numberOfPoints
+    prisms = rng.integers(0, T * (L - 1), n)
+    starts = np.array([O.ray_start(om, seed, 0, 0, i, int(j)) for i, j in enumerate(prisms)])
+    samples = rng.integers(0, Np * L, n)
+    sp = topo.samplePoints()[samples]
+    nr = rng.integers(0, reflections + 1, n) if reflections else np.zeros(n, np.int64)
+    fp = rng.integers(0, 2, n)
+    ends = np.array([O.image_point(om, sp[i], int(nr[i]), int(fp[i])) for i in range(n)])
+    return starts.astype(np.float32).astype(np.float64), prisms, ends.astype(np.float32).astype(np.float64), nr, fp
+
+
+def _walk(lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, staged):
+    topo, med = w.topology, w.medium
+    T, L = topo.numberOfTriangles, int(topo.levels)
+    h = float(topo.thickness)
+    hs = starts.copy()
+    hs[:, 2] = starts[:, 2] / h
+    he = ends.copy()
+    ze = ends[:, 2] / h
+    zr = np.rint(ze)
+    he[:, 2] = np.where(np.abs(ze - zr) < 1e-5, zr, ze)
+    hs, he = hs.astype(np.float32).reshape(-1), he.astype(np.float32).reshape(-1)
+    n = len(prisms)
+    out = np.zeros(5 * n, np.float32)
+    steps = np.zeros(n, np.uint32)
+    status = np.zeros(n, np.int32)
+    refl = np.ascontiguousarray(med.value("reflectivities"), np.float32)
+    nidx = np.ascontiguousarray(med.value("refractiveIndices"), np.float32)
+    P = lambda a, t: np.ascontiguousarray(a, dtype=t).ctypes.data_as(C.POINTER(np.ctypeslib.as_ctypes_type(t)))  # noqa
+    keep = [np.ascontiguousarray(a) for a in (rec, lds, beta, refl, nidx)]
+    lib.walk_rays_host(T, L - 1, np.float32(h), keep[0].ctypes.data_as(C.POINTER(C.c_uint32)),
+                       keep[1].ctypes.data_as(C.POINTER(C.c_uint32)) if staged else None,
+                       keep[2].ctypes.data_as(C.POINTER(C.c_float)), keep[3].ctypes.data_as(C.POINTER(C.c_float)),
+                       keep[4].ctypes.data_as(C.POINTER(C.c_float)), n,
+                       hs.ctypes.data_as(C.POINTER(C.c_float)),
+                       np.ascontiguousarray(prisms, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)),
+                       he.ctypes.data_as(C.POINTER(C.c_float)),
+                       np.ascontiguousarray(nr, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)),
+                       np.ascontiguousarray(fp, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)),
+                       out.ctypes.data_as(C.POINTER(C.c_float)), steps.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       status.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out.reshape(n, 5), steps, status
+
+
+def _case(name):
+    if name == "plain_small":
+        return plain_small(), 0
+    if name == "true_cladding":
+        topo, med, sp, ref = load_true_cladding(1)
+        cs = CrossSectionData.monochromatic(wavelength=1030e-9, crossSectionAbsorption=sp["sigmaA"],
+                                            crossSectionEmission=sp["sigmaE"])
+        return Workload("tc", topo, med, cs, {"monochromatic": True}), 0
+    w = production_disk(levels=6, n_points=400, wavelengths=4)
+    w.medium.set("refractiveIndices", [1.83, 1.0, 1.83, 1.0])
+    return w, 3
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["global", "lds"])
+@pytest.mark.parametrize("name", ["plain_small", "true_cladding", "disk_reflect"])
+def test_host_walk_matches_the_oracle(walk_lib, name, staged):
+    w, refl = _case(name)
+    w = _f32_exact(w)
+    sa, se, _, _ = w.spectra.spectralPairs(True)
+    om = O.OracleMesh(w.topology, w.medium)
+    rec, lds, beta = _records(w)
+    starts, prisms, ends, nr, fp = _rays(w, om, 3000, seed=21, reflections=refl)
+    out, steps, status = _walk(walk_lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, staged)
+    nt = float(w.medium.value("nTot"))
+    alpha = float(w.medium.value("claddingAbsorption"))
+    rel, same_steps, worst = [], 0, 0
+    for i in range(len(prisms)):
+        o = O.walk_ray(om, starts[i], int(prisms[i]), ends[i], int(nr[i]), int(fp[i]))
+        assert o["status"] == 0
+        assert status[i] == 0, f"ray {i} failed with status {status[i]}"
+        bl, gl, cl, rf, ln = (float(v) for v in out[i])
+        Eg = nt * (se[0] + sa[0]) * bl - nt * sa[0] * gl - alpha * cl
+        Eo = nt * (se[0] + sa[0]) * o["beta_len"] - nt * sa[0] * o["gain_len"] - alpha * o["clad_len"]
+        rel.append(abs(np.exp(Eg - Eo) - 1.0))
+        assert rf == pytest.approx(o["refl"], rel=1e-6, abs=1e-7)
+        assert gl + cl == pytest.approx(o["length"], rel=2e-5)
+        same_steps += int(steps[i] == o["steps"])
+        worst = max(worst, abs(int(steps[i]) - int(o["steps"])))
+    rel = np.array(rel)
+    assert rel.max() <= 1e-5, f"max single-ray gain error {rel.max():.3e}"
+    assert same_steps >= 0.99 * len(prisms) and worst <= 3
