@@ -108,7 +108,8 @@ typedef struct hase_compute {
 /* Per-sample outputs, caller-owned, each with S entries.  Samples outside
  * [min_sample_range, max_sample_range] are written as zero.  Mirrors
  * Result (include/core/types.hpp:201-246): mse takes the slot of
- * relativeStandardError, standard_error = mse * phiAse. */
+ * relativeStandardError (mse = relative standard error squared),
+ * standard_error = sqrt(mse) * |phiAse|. */
 typedef struct hase_result {
     float* phi_ase;
     double* mse;

@@ -310,7 +310,9 @@ def test_legacy_mse_is_the_squared_relative_standard_error(index):
     translations (mse 0.1 ~ RSE 0.14 / 0.10, tests/data/compareSerial/
     README.md:83; 0.01087 ~ 0.05, fixed_legacy_reflection_pump3_wedge_reference/
     README.md:76) relate the legacy mse to the forward Tet4 estimator's RSE,
-    a different estimator at a different effort, so they do not bear on it."""
+    a different estimator at a different effort, so they fix no scale.  Their
+    consistency still points the same way (test_threshold_translations_favour
+    _the_square below)."""
     topo, med, sp, ref = load_true_cladding(index)
     om = O.OracleMesh(topo, med)
     op = O.OracleParams(sigma_a=sp["sigmaA"], sigma_e=sp["sigmaE"], n_tot=float(med.value("nTot")),
@@ -328,6 +330,31 @@ def test_legacy_mse_is_the_squared_relative_standard_error(index):
         assert not (1.2 <= scale(wrong) <= 2.0)
     # the legacy values are finite, positive and capped as the fixture states
     assert np.all(ref["mse"] > 0) and ref["mse"].max() < 0.13
+
+
+def test_threshold_translations_favour_the_square():
+    """The reference translates legacy mse thresholds into forward-estimator
+    RSE thresholds at matched effort: 0.1 -> 0.14 (cuboid) and 0.10
+    (cylindrical), tests/data/compareSerial/README.md:83; 0.01087 -> 0.05,
+    fixed_legacy_reflection_pump3_wedge_reference/README.md:76.  If
+    mse = RSE_legacy^p and RSE_legacy is a fixed multiple of the forward RSE
+    (the two estimators' effort ratio), mse^(1/p) / RSE is the same for all
+    three pairs.  Under p = 2 the three ratios spread by 1.5x (2.1-3.2); under
+    p = 1 by 4.6x (0.22-1.0).  The translations therefore reject mse = RSE;
+    they cannot tell RSE^2 from a multiple of it (the F2 interquartile scale
+    above does that)."""
+    pairs = np.array([[0.1, 0.14], [0.1, 0.10], [0.01087, 0.05]])
+
+    def spread(p):
+        r = pairs[:, 0] ** (1.0 / p) / pairs[:, 1]
+        return r.max() / r.min()
+
+    assert spread(2.0) < 1.6
+    assert spread(1.0) > 4.0
+    # a constant factor on mse (mse = 2 RSE^2) leaves the spread as it is
+    two = pairs.copy()
+    two[:, 0] /= 2.0
+    assert (two[:, 0] ** 0.5 / two[:, 1]).max() / (two[:, 0] ** 0.5 / two[:, 1]).min() == pytest.approx(spread(2.0))
 
 
 def test_cladding_absorption_lowers_flux_monotonically():
