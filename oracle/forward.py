@@ -39,7 +39,7 @@ class _Surfaces(C.Structure):
 class Srm(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("max_passes", C.c_uint32), ("divergence_streak", C.c_uint32),
                 ("tolerance", C.c_double), ("status", C.c_int32 * BATCHES), ("passes", C.c_uint32 * BATCHES),
-                ("remaining", C.c_double * BATCHES)]
+                ("remaining", C.c_double * BATCHES), ("order_stride", C.c_uint32)]
 
 
 class Hit(C.Structure):
@@ -222,7 +222,7 @@ class _Reflecting:
     """Surfaces and reservoir controls of the oracle's surface reservoir method."""
 
     def __init__(self, oracle, face_surface, reflectivity, n_in=None, n_out=None, *, slots=32, max_passes=40,
-                 divergence_streak=3, tolerance=1e-4):
+                 divergence_streak=3, tolerance=1e-4, order_stride=0):
         self._fs = np.ascontiguousarray(face_surface, dtype=np.int32).reshape(-1)
         self._refl = np.ascontiguousarray(reflectivity, dtype=np.float32)
         self._nin = None if n_in is None else np.ascontiguousarray(n_in, dtype=np.float32)
@@ -236,6 +236,7 @@ class _Reflecting:
         self.srm = Srm()
         self.srm.slots, self.srm.max_passes = int(slots), int(max_passes)
         self.srm.divergence_streak, self.srm.tolerance = int(divergence_streak), float(tolerance)
+        self.srm.order_stride = int(order_stride)
 
     @property
     def n_faces(self):

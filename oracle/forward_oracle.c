@@ -793,6 +793,20 @@ static double orf_srm_shift(uint32_t seed, uint32_t launch, uint32_t batch, uint
     return orc_u32_to_unit(o[0]);
 }
 
+/* position k of n -> ray index; a permutation only when the stride is coprime
+ * to n, else the ray order is kept */
+static uint32_t orf_deposit_order(const orf_srm* srm, uint32_t k, uint32_t n)
+{
+    uint32_t a = srm->order_stride, b = n;
+    if (a <= 1u || n == 0u) return k;
+    while (b) {
+        const uint32_t r = a % b;
+        a = b;
+        b = r;
+    }
+    return a == 1u ? (uint32_t)(((uint64_t)k * srm->order_stride) % n) : k;
+}
+
 void orf_trace_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, const double* sigma_a,
                    const double* sigma_e, uint32_t n_lambda, uint32_t seed, uint32_t launch, uint32_t rays,
                    double* scores, uint32_t* visits, uint32_t* dropped, uint32_t* batch_counts, uint64_t* segments)
@@ -812,7 +826,8 @@ void orf_trace_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, cons
         for (uint32_t b = 0; b < ORF_BATCHES; ++b) orf_reservoir_alloc(&res[side][b], nb, S);
     uint64_t seg = 0;
     /* pass 0: primary rays, reflections into side 0 */
-    for (uint32_t r = 0; r < rays; ++r) {
+    for (uint32_t k = 0; k < rays; ++k) {
+        const uint32_t r = orf_deposit_order(srm, k, rays);
         orf_ray ray;
         orf_ray_start(m, seed, launch, rays, r, n_lambda, &ray);
         orf_exit ex;
@@ -864,7 +879,8 @@ void orf_trace_srm(const orf_mesh* m, const orf_surfaces* sf, orf_srm* srm, cons
             for (size_t f = 0; f < nb; ++f) cdf[f] = run > 0.0 ? cdf[f] / run : 0.0;
             const double source = previous[b] / (double)count;
             const double shift = orf_srm_shift(seed, launch, b, pass);
-            for (uint32_t i = 0; i < count; ++i) {
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t i = orf_deposit_order(srm, k, count);
                 const uint32_t r = first + i;
                 double u0, u1;
                 orf_srm_uniforms(seed, launch, r, pass, 2, &u0, &u1);

@@ -167,6 +167,11 @@ typedef struct {
     int32_t status[ORF_BATCHES];
     uint32_t passes[ORF_BATCHES];
     double remaining[ORF_BATCHES];
+    /* order in which rays deposit into the reservoirs: ray k of a pass is
+     * processed at position (k * order_stride) mod count (0 or 1: ray order).
+     * The device deposits in atomic arrival order; a stride coprime to the
+     * ray count reproduces the effect of another order on the CPU. */
+    uint32_t order_stride;
 } orf_srm;
 
 int orf_surfaces_build(const orf_mesh* m, const int32_t* face_surface, uint32_t n_surfaces,

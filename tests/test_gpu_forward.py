@@ -181,35 +181,51 @@ def test_reflections_agree_with_the_oracle():
     convergence: the device and the oracle draw the same primary rays, but
     their reservoirs fill in different orders (atomics).  Which deposits a
     32-slot reservoir keeps depends on their arrival order (the first 32 of a
-    face are kept regardless of weight until replaced), so the two agree cell
-    by cell within their errors and in total energy, with a common shift of
-    the reflected part of a few tenths of a percent allowed (|mean z| < 1)."""
+    face are kept regardless of weight), so the device is compared with the
+    oracle run in a permuted deposit order: in ray order the oracle's
+    reflected flux is 0.1-0.4% lower than in any permuted order
+    (test_forward_oracle.py::test_reservoir_results_depend_on_the_deposit_order),
+    while permuted orders agree within |mean z| < 0.2."""
     z = 0.4
     mesh, fs = _slab(nx=6, levels=7, z=z, beta=0.25)
     kw = dict(min_rays=120_000, max_rays=120_000, steps=0, seed=21)
     o = F.ForwardOracle(mesh)
-    refl = F.reflecting(o, fs, [0.0, 0.5, 0.3], tolerance=1e-4)
-    print("[forward] reflections: oracle", flush=True)
-    ref = F.evaluate_srm(o, refl, [SA], [SE], **kw)
+    refs = {}
+    for name, stride in (("ray order", 0), ("permuted order", 1000003)):
+        print(f"[forward] reflections: oracle, {name}", flush=True)
+        refl = F.reflecting(o, fs, [0.0, 0.5, 0.3], tolerance=1e-4, order_stride=stride)
+        refs[name] = F.evaluate_srm(o, refl, [SA], [SE], **kw)
+    status, passes, _ = refl.status()
     with _context(mesh, [SA], [SE], min_rays=kw["min_rays"], max_rays=kw["max_rays"], seed=kw["seed"],
                   reflect=True) as ctx:
         ctx.set_reflections(fs, [0.0, 0.5, 0.3], tolerance=1e-4)
         res = ctx.evaluate()
         st = ctx.stats()
-    status, passes, _ = refl.status()
     print(f"[forward] reflections: oracle passes {passes} status {status}; device {st['srm_passes']} "
           f"{st['srm_status_name']}", flush=True)
     assert st["srm_status_name"] in ("CONVERGED", "STABLE") and abs(st["srm_passes"] - max(passes)) <= 2
-    zs = (res.phiAse.astype(np.float64) - ref["phiAse"]) / np.hypot(res.standardError, ref["standardError"])
     vol = mesh.volumes()
+    for name, ref in refs.items():
+        zs = (res.phiAse.astype(np.float64) - ref["phiAse"]) / np.hypot(res.standardError, ref["standardError"])
+        total = float(np.sum(res.phiAse * vol) / np.sum(ref["phiAse"] * vol))
+        print(f"[forward] reflections vs oracle in {name}: mean z {zs.mean():.3f}, beyond 4 sigma "
+              f"{np.mean(np.abs(zs) > 4):.4f}, energy ratio {total:.5f}", flush=True)
+        assert np.mean(np.abs(zs) > 4) <= 0.01, (name, np.mean(np.abs(zs) > 4))
+    ref = refs["permuted order"]
+    zs = (res.phiAse.astype(np.float64) - ref["phiAse"]) / np.hypot(res.standardError, ref["standardError"])
     total = float(np.sum(res.phiAse * vol) / np.sum(ref["phiAse"] * vol))
-    print(f"[forward] reflections: mean z {zs.mean():.3f}, beyond 4 sigma {np.mean(np.abs(zs) > 4):.4f}, "
-          f"energy ratio {total:.5f}", flush=True)
-    assert np.mean(np.abs(zs) > 4) <= 0.01 and abs(zs.mean()) < 1.0, (np.mean(np.abs(zs) > 4), zs.mean())
-    assert abs(total - 1.0) < 0.01, total
+    assert abs(zs.mean()) < 0.3, zs.mean()
+    assert abs(total - 1.0) < 0.004, total
 
 
 def test_first_order_reflections_against_the_image_method():
+    """One reservoir pass, zero gain, reflectivity 0.4 / 0.2: the centre
+    cells' flux against the first-order image method (angular quadrature
+    averaged by the vertex hat functions), and against the oracle on the same
+    primary rays (seed 5489, permuted deposit order).  The oracle's own ratio
+    to the image method scatters by about 0.3% over seeds (CPU: 0.998-1.0035
+    at 2e6 rays for seeds 1, 2, 3, 5489); seed 5489 sits at +0.35%, so the
+    device is held to the oracle's ratio on its own rays."""
     z, beta = 0.5, SA / (SA + SE)
     mesh, fs = _slab(z=z, beta=beta)
     print("[forward] first-order reflections, 2e6 rays", flush=True)
@@ -218,6 +234,10 @@ def test_first_order_reflections_against_the_image_method():
         res = ctx.evaluate()
         st = ctx.stats()
     assert st["srm_status_name"] == "MAX_PASSES" and st["srm_passes"] == 1
+    o = F.ForwardOracle(mesh)
+    refl = F.reflecting(o, fs, [0.0, 0.4, 0.2], max_passes=1, order_stride=1000003)
+    print("[forward] first-order reflections: oracle, 2e6 rays", flush=True)
+    ref = F.evaluate_srm(o, refl, [SA], [SE], min_rays=2_000_000, max_rays=2_000_000, steps=0, seed=5489)
     cen = mesh.centroids()
     near = np.nonzero(np.linalg.norm((cen - [0.5, 0.5, z / 2]) / [1, 1, z], axis=1) < 0.12)[0]
 
@@ -228,10 +248,11 @@ def test_first_order_reflections_against_the_image_method():
     with np.errstate(invalid="ignore"):
         expect = hat_smoothed_cell_flux(mesh.points, mesh.cells, mesh.volumes(), near, flux)
     ratio = res.phiAse[near].astype(np.float64).sum() / expect.sum()
-    print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}", flush=True)
-    # five runs on one MI355X gave 1.0027-1.0062: the restart from face centroids leaves a small
-    # first-order offset and the reservoir keeps deposits in their (atomic) arrival order
-    assert abs(ratio - 1.0) < 0.01, ratio
+    oracle_ratio = ref["phiAse"][near].astype(np.float64).sum() / expect.sum()
+    print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}, oracle {oracle_ratio:.5f}",
+          flush=True)
+    assert abs(ratio / oracle_ratio - 1.0) < 0.003, (ratio, oracle_ratio)
+    assert abs(ratio - 1.0) < 0.008, ratio
 
 
 def test_edge_cases_zero_beta_few_rays_single_cell():
