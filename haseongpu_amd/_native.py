@@ -24,6 +24,7 @@ _f32p = C.POINTER(C.c_float)
 # every symbol include/hase_ase.h declares
 EXPORTED_SYMBOLS = (
     "hase_calc_phi_ase",
+    "hase_calc_phi_ase_noalloc",
     "hase_context_create",
     "hase_context_update_beta",
     "hase_context_evaluate",
@@ -347,6 +348,8 @@ def load():
     L = C.CDLL(str(LIB_PATH))
     ctx_p = C.c_void_p
     L.hase_calc_phi_ase.argtypes = [C.POINTER(Mesh), C.POINTER(Experiment), C.POINTER(Compute), C.POINTER(ResultC)]
+    L.hase_calc_phi_ase_noalloc.argtypes = [C.POINTER(Mesh), C.POINTER(Experiment), C.POINTER(Compute),
+                                            C.POINTER(ResultC)]
     L.hase_context_create.argtypes = [C.POINTER(Mesh), C.POINTER(Experiment), C.POINTER(Compute),
                                       C.POINTER(ctx_p)]
     L.hase_context_update_beta.argtypes = [ctx_p, _f64p, _f64p]
@@ -398,7 +401,7 @@ def load():
     L.hase_last_error.restype = C.c_char_p
     L.hase_abi_version.restype = C.c_int
     L.hase_build_info.restype = C.c_char_p
-    for name in ("hase_calc_phi_ase", "hase_context_create", "hase_context_update_beta",
+    for name in ("hase_calc_phi_ase", "hase_calc_phi_ase_noalloc", "hase_context_create", "hase_context_update_beta",
                  "hase_context_evaluate", "hase_context_stats", "hase_context_trace_rays",
                               "hase_context_importance", "hase_mlp_forward", "hase_mlp_loss_gradient", "hase_mlp_train",
                  "hase_mlp_forward_bf16", "hase_context_set_proposal", "hase_context_transfer_map",
@@ -842,6 +845,16 @@ def calc_phi_ase(mesh: NativeMesh, experiment: NativeExperiment, compute: Comput
     res = ResultArrays(n)
     check(load().hase_calc_phi_ase(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
                                    C.byref(res.struct)))
+    return res
+
+
+def calc_phi_ase_noalloc(mesh: NativeMesh, experiment: NativeExperiment, compute: Compute):
+    """One ``hase_calc_phi_ase_noalloc`` call: the one-shot evaluation with
+    every device buffer taken from the library's fixed device scratch."""
+    n = mesh.struct.number_of_points * mesh.struct.number_of_levels
+    res = ResultArrays(n)
+    check(load().hase_calc_phi_ase_noalloc(C.byref(mesh.struct), C.byref(experiment.struct), C.byref(compute),
+                                           C.byref(res.struct)))
     return res
 
 

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <new>
@@ -36,26 +37,67 @@ struct HaseError : std::runtime_error {
             throw ::hase::HaseError(HASE_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(err_)); \
     } while (0)
 
-// Owning device allocation that grows on demand.
+// Bump allocator over a caller-provided device region.  While a thread has
+// one installed (ArenaScope), every DevBuf of that thread takes its memory
+// from it instead of hipMalloc, and never frees it: the region is reused as a
+// whole by the next call that installs it.
+struct Arena {
+    char* base = nullptr;
+    size_t capacity = 0;
+    size_t used = 0;
+    size_t remaining() const { return capacity - used; }
+    void* take(size_t bytes)
+    {
+        const size_t start = (used + 255u) & ~(size_t)255u;
+        if (start > capacity || bytes > capacity - start)
+            throw HaseError(HASE_E_MEMORY, "request of " + std::to_string(bytes) + " bytes exceeds the " +
+                                               std::to_string(capacity) + "-byte no-allocation scratch (" +
+                                               std::to_string(capacity - std::min(start, capacity)) +
+                                               " bytes left)");
+        used = start + bytes;
+        return base + start;
+    }
+};
+
+// the arena of the calling thread, or null (DevBuf then uses hipMalloc)
+extern thread_local Arena* g_arena;
+
+struct ArenaScope {
+    Arena* prev;
+    explicit ArenaScope(Arena* a) : prev(g_arena) { g_arena = a; }
+    ~ArenaScope() { g_arena = prev; }
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+};
+
+// Owning device allocation that grows on demand (or a slice of the thread's
+// arena, which it does not own).
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = false;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = false;
     }
     void alloc(size_t count)
     {
         if (count <= n && p) return;
         release();
         if (count == 0) return;
+        if (g_arena) {
+            p = static_cast<T*>(g_arena->take(count * sizeof(T)));
+            n = count;
+            return;
+        }
         hipError_t e = hipMalloc(&p, count * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -63,6 +105,7 @@ struct DevBuf {
                                                " bytes failed: " + hipGetErrorString(e));
         }
         n = count;
+        owned = true;
     }
     void upload(const T* src, size_t count, hipStream_t st)
     {

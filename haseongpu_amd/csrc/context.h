@@ -121,6 +121,13 @@ float elapsed(hipEvent_t a, hipEvent_t b);
 
 // evaluation of the context's sample range into host results (hase_ase.hip)
 void evaluate_context(hase_context* ctx, hase_result_t* res);
+// one-shot evaluation: a context built for the call, evaluated and destroyed
+// (its device buffers come from the calling thread's arena when one is
+// installed)
+void calc_phi_ase(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,
+                  hase_result_t* result);
+// host-side precondition checks of a one-shot evaluation (no HIP call)
+void validate_arguments(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute);
 // Replaces the walk's copies of beta (FP32 beta, beta * V, column-major beta,
 // the sum of beta * V) by those of `beta` [P], FP64 in device prism order,
 // already on the device; block_sums holds ceil(P / 256) doubles.

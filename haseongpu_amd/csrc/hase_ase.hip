@@ -928,7 +928,14 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     HIP_CHECK(hipSetDevice(c->device));
     auto ctx = std::make_unique<hase_context>();
     ctx->device = c->device;
-    HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (g_arena) {
+        // allocation-free evaluation: the null stream, rounds launched one by
+        // one (instantiating a graph allocates), no timing events
+        ctx->stream = nullptr;
+        ctx->tuning.use_graph = 0u;
+    } else {
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    }
     const uint32_t np = mesh->number_of_points, T = mesh->number_of_triangles, L = mesh->number_of_levels;
     ctx->np = np; ctx->T = T; ctx->L = L; ctx->P = T * (L - 1); ctx->S = np * L;
     ctx->h = mesh->thickness;
@@ -1090,7 +1097,26 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
     const uint32_t rpt = c.rays_per_thread ? c.rays_per_thread : 8u;
     const uint32_t rays_per_block = kBlock * rpt;
     const uint32_t max_chunks = (e.max_rays_per_sample + rays_per_block - 1) / rays_per_block;
+    const bool queue = ctx->tuning.mode == HASE_TRACE_QUEUE;
+    // per-slot partial sums: queue items in the work queue, chunks otherwise
+    const size_t chunks = queue ? queue_items_per_slot(ctx, ctx->tuning) : max_chunks;
     uint32_t batch = c.sample_batch;
+    if (g_arena) {
+        // allocation-free call: as many slots as the scratch left after the
+        // mesh holds (64 KiB kept for the per-call tables and alignment)
+        const size_t tables_b = n_ent * 12u + kPrefixBuckets * 4u;
+        const size_t per_slot = (shared ? 0u : tables_b) + chunks * 20u + 256u +
+                                (learned ? (size_t)R * ctx->C * 4u : 0u);
+        size_t left = g_arena->remaining();
+        const size_t keep = ((size_t)64 << 10) + (shared ? tables_b : 0u);
+        left = left > keep ? left - keep : 0u;
+        const size_t fit = std::min<size_t>(left / per_slot, 8192);
+        if (fit == 0)
+            throw HaseError(HASE_E_MEMORY, "the no-allocation scratch cannot hold one sample point's buffers: " +
+                                               std::to_string(per_slot + keep) + " bytes needed, " +
+                                               std::to_string(g_arena->remaining()) + " left after the mesh");
+        batch = batch ? std::min<uint32_t>(batch, (uint32_t)fit) : (uint32_t)fit;
+    }
     if (batch == 0) {
         const uint32_t per_item = ctx->tuning.mode == HASE_TRACE_QUEUE
                                       ? (uint32_t)queue_items_per_slot(ctx, ctx->tuning)
@@ -1134,8 +1160,8 @@ uint32_t reserve_batch(hase_context* ctx, uint32_t want, size_t budget = 0, bool
     ctx->active_next.alloc(batch);
     ctx->active_count.alloc(1);
     ctx->slot_sample.alloc(batch);
-    ctx->partial.alloc((size_t)batch * max_chunks);
-    ctx->partial_drop.alloc((size_t)batch * max_chunks);
+    ctx->partial.alloc((size_t)batch * chunks);
+    ctx->partial_drop.alloc((size_t)batch * chunks);
     ctx->phi.alloc(batch);
     ctx->mse.alloc(batch);
     ctx->se.alloc(batch);
@@ -1456,7 +1482,10 @@ void evaluate(hase_context* ctx, hase_result_t* res)
         run_batch(ctx, res, 0u, deferred.data() + d0, slots, false, 1u, nullptr, tm);
     }
     hipEvent_t ev_end = record(ctx);
-    HIP_CHECK(hipEventSynchronize(ev_end));
+    if (ev_end)
+        HIP_CHECK(hipEventSynchronize(ev_end));
+    else
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
     for (auto& pr : tm.imp) ctx->stats.importance_ms += elapsed(pr.first, pr.second);
     for (auto& pr : tm.trace) ctx->stats.trace_ms += elapsed(pr.first, pr.second);
     for (auto& pr : tm.fin) ctx->stats.finalize_ms += elapsed(pr.first, pr.second);
@@ -1483,6 +1512,18 @@ void evaluate(hase_context* ctx, hase_result_t* res)
 namespace hase {
 
 void evaluate_context(hase_context* ctx, hase_result_t* res) { evaluate(ctx, res); }
+
+void validate_arguments(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute)
+{
+    validate(mesh, experiment, compute);
+}
+
+void calc_phi_ase(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,
+                  hase_result_t* result)
+{
+    auto ctx = create_context(mesh, experiment, compute);
+    evaluate(ctx.get(), result);
+}
 
 void refresh_beta_device(hase_context* ctx, const double* beta, double* block_sums)
 {
@@ -1848,9 +1889,11 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
 
 namespace hase {
 thread_local std::string g_last_error;
+thread_local Arena* g_arena = nullptr;
 
 hipEvent_t record(hase_context* ctx)
 {
+    if (g_arena) return nullptr;   // no events are created in allocation-free calls
     hipEvent_t ev;
     HIP_CHECK(hipEventCreate(&ev));
     HIP_CHECK(hipEventRecord(ev, ctx->stream));
@@ -1861,6 +1904,7 @@ hipEvent_t record(hase_context* ctx)
 float elapsed(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.0f;
+    if (!a || !b) return ms;
     HIP_CHECK(hipEventElapsedTime(&ms, a, b));
     return ms;
 }
@@ -1878,10 +1922,7 @@ extern "C" {
 int hase_calc_phi_ase(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,
                       hase_result_t* result)
 {
-    return guarded([&] {
-        auto ctx = create_context(mesh, experiment, compute);
-        evaluate(ctx.get(), result);
-    });
+    return guarded([&] { calc_phi_ase(mesh, experiment, compute, result); });
 }
 
 int hase_context_create(const hase_mesh_t* mesh, const hase_experiment_t* experiment, const hase_compute_t* compute,

@@ -144,6 +144,28 @@ int hase_calc_phi_ase(const hase_mesh_t* mesh,
                       const hase_compute_t* compute,
                       hase_result_t* result);
 
+/* Bytes of the fixed device scratch behind hase_calc_phi_ase_noalloc. */
+#define HASE_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* hase_calc_phi_ase without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that forbids hipMalloc, tight serving loops).  Same arguments, results and
+ * random streams as hase_calc_phi_ase; every device buffer is carved out of
+ * one static device array of HASE_NOALLOC_SCRATCH_BYTES per GPU, held from
+ * the first call on that GPU until the process ends.  The call creates no
+ * stream, event or graph (it runs on the null stream and returns when the
+ * results are on the host).  A problem whose mesh plus one sample point's
+ * proposal tables (12 B per prism and reflection slice, ~28 KB more) does
+ * not fit returns HASE_E_MEMORY with the sizes in hase_last_error(); larger
+ * problems are evaluated in as many sample batches as the scratch takes.
+ * Calls on one GPU share its array and are serialised by a lock within the
+ * process; the array keeps the last call's contents, and each call
+ * initialises what it reads.  Another process's calls have their own copy. */
+int hase_calc_phi_ase_noalloc(const hase_mesh_t* mesh,
+                              const hase_experiment_t* experiment,
+                              const hase_compute_t* compute,
+                              hase_result_t* result);
+
 /* Persistent context: keeps the mesh resident on the device between
  * evaluations, like ForwardPhiAseContext
  * (include/core/forwardPhiAseEvaluator.hpp:180-204). */

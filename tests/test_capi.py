@@ -92,11 +92,18 @@ def test_invalid_arguments_are_rejected_on_the_host(overrides, message):
     code = lib.hase_calc_phi_ase(C.byref(mesh.struct), C.byref(exp.struct), C.byref(comp), C.byref(res.struct))
     assert code == -1
     assert message in lib.hase_last_error().decode()
+    # the allocation-free entry checks the same preconditions before any device call
+    code = lib.hase_calc_phi_ase_noalloc(C.byref(mesh.struct), C.byref(exp.struct), C.byref(comp),
+                                         C.byref(res.struct))
+    assert code == -1
+    assert message in lib.hase_last_error().decode()
 
 
 def test_null_arguments_are_rejected():
     lib = _native.load()
     assert lib.hase_calc_phi_ase(None, None, None, None) == -1
+    assert b"NULL" in lib.hase_last_error()
+    assert lib.hase_calc_phi_ase_noalloc(None, None, None, None) == -1
     assert b"NULL" in lib.hase_last_error()
     assert lib.hase_context_evaluate(None, None) == -1
     assert lib.hase_context_stats(None, None) == -1
