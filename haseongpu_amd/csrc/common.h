@@ -20,7 +20,6 @@ constexpr int kBlock = 256;
 // whose prefix exceeds k / kPrefixBuckets, so the ray search for a target in
 // [k, k+1) / kPrefixBuckets covers only the entries between two buckets.
 constexpr uint32_t kPrefixBuckets = 2048;
-constexpr float kPi = 3.14159265358979f;
 
 // message of the last failed call on this thread (hase_last_error)
 extern thread_local std::string g_last_error;

@@ -35,6 +35,7 @@
 #include "common.h"
 #include "context.h"
 #include "prism_walk.h"
+#include "importance_walk.h"
 #include "proposal.h"
 #include "device_common.h"
 
@@ -208,153 +209,6 @@ struct ImportanceParams {
     uint32_t parts;           // work-groups per sample; > 1: importance only, hase_prefix_kernel follows
 };
 
-// Levels whose centroid walks share one xy walk in the importance kernel.
-constexpr int kLevelGroup = 4;
-
-// Importance of the prisms (t0, lv0 .. lv0+nlev-1) for one reflection slice.
-// Every level (and every mirror slice) of a triangle column projects onto the
-// same xy segment from the triangle's centroid to the sample point, so the
-// column is walked once in xy and each level's ray is advanced along it,
-// split at that level's own plane crossings.  The per-level sums are those of
-// walk_ray on the same ray: sum(beta l), sum(l) over gain prisms, sum(l) over
-// cladding prisms, the surface factors at the copy boundaries, and the
-// prisms visited.
-template <bool kReflect>
-__device__ __forceinline__ void column_importance(const MeshDev& m, uint32_t t0, int lv0, int nlev, float px,
-                                                  float py, uint32_t pe, int end_plane, uint32_t nrefl, uint32_t first,
-                                                  float out_bl[kLevelGroup], float out_gl[kLevelGroup],
-                                                  float out_cl[kLevelGroup], float out_rf[kLevelGroup],
-                                                  float D[kLevelGroup], unsigned long long& steps, bool& ok)
-{
-    const int layers = (int)m.layers;
-    const float h = m.h;
-    uint4 c0, c1;
-    load_corners(m, t0, c0, c1);
-    const float cx = (bits_to_float(c0.x) + bits_to_float(c0.z) + bits_to_float(c1.x)) * (1.0f / 3.0f);
-    const float cy = (bits_to_float(c0.y) + bits_to_float(c0.w) + bits_to_float(c1.y)) * (1.0f / 3.0f);
-    const float dx = px - cx, dy = py - cy;
-    const float Dxy = sqrtf(dx * dx + dy * dy);
-    const float inv_xy = 1.0f / Dxy;
-    const float vx = dx * inv_xy, vy = dy * inv_xy;
-    const float ze = (float)end_plane;
-    // per level: parameter of a plane = (plane - za) * idz; unfolded layer u = cp * layers + r
-    float idz[kLevelGroup];
-    int r[kLevelGroup], cp[kLevelGroup];
-    uint32_t applied[kLevelGroup];
-#pragma unroll
-    for (int g = 0; g < kLevelGroup; ++g) {
-        const float dz = ze - ((float)(lv0 + g) + 0.5f);
-        idz[g] = 1.0f / dz;
-        D[g] = sqrtf(Dxy * Dxy + dz * h * dz * h);
-        r[g] = lv0 + g;
-        cp[g] = 0;
-        applied[g] = 0;
-        out_bl[g] = 0.0f; out_gl[g] = 0.0f; out_cl[g] = 0.0f; out_rf[g] = 1.0f;
-    }
-    const float tol = 2e-5f * (1.0f + Dxy);
-    uint32_t t = t0;
-    int entry = -1;
-    float s = 0.0f;
-    ok = true;
-    for (uint32_t step = 0; step < kMaxSteps; ++step) {
-        uint4 qa, qb, qc;
-        load_triangle(m, t, qa, qb, qc);
-        const bool clad = record_cladding(qc) != 0u;
-        const float* bcol = m.beta_col + (size_t)t * (uint32_t)layers;
-        // this column's beta at every level's current layer: independent loads, one round trip
-        float bcur[kLevelGroup];
-#pragma unroll
-        for (int g = 0; g < kLevelGroup; ++g) {
-            const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
-            bcur[g] = (g < nlev && !clad) ? bcol[lvl] : 0.0f;
-        }
-        const float X0 = bits_to_float(qa.x) - cx, Y0 = bits_to_float(qa.y) - cy;
-        const float X1 = bits_to_float(qa.z) - cx, Y1 = bits_to_float(qa.w) - cy;
-        const float X2 = bits_to_float(qb.x) - cx, Y2 = bits_to_float(qb.y) - cy;
-        const float ex0 = X1 - X0, ey0 = Y1 - Y0;
-        const float ex1 = X2 - X1, ey1 = Y2 - Y1;
-        const float ex2 = X0 - X2, ey2 = Y0 - Y2;
-        const float nv0 = ey0 * vx - ex0 * vy, nv1 = ey1 * vx - ex1 * vy, nv2 = ey2 * vx - ex2 * vy;
-        float s0 = (ey0 * X0 - ex0 * Y0) * fast_rcp(nv0);
-        float s1 = (ey1 * X1 - ex1 * Y1) * fast_rcp(nv1);
-        float s2 = (ey2 * X2 - ex2 * Y2) * fast_rcp(nv2);
-        s0 = (entry != 0 && nv0 > 0.0f) ? s0 : INFINITY;
-        s1 = (entry != 1 && nv1 > 0.0f) ? s1 : INFINITY;
-        s2 = (entry != 2 && nv2 > 0.0f) ? s2 : INFINITY;
-        const ExitEdge x = exit_edge<GlobalTriangles>(qa, qb, qc, s0, s1, s2, pe, px, py);
-        const float best = (x.touches & (int)(x.s < INFINITY)) ? Dxy : x.s;
-        const float snext = fmaxf(fminf(best, Dxy), s);
-        const bool at_end = snext >= Dxy;
-        const float sig0 = s * inv_xy;
-        const float sig1 = at_end ? 1.0f : snext * inv_xy;
-#pragma unroll
-        for (int g = 0; g < kLevelGroup; ++g) {
-            if (g < nlev) {
-                const float za = (float)(lv0 + g) + 0.5f;
-                const bool up = idz[g] > 0.0f;
-                float a = sig0;
-                float b = bcur[g];
-                for (;;) {
-                    const int plane = cp[g] * layers + r[g] + (up ? 1 : 0);
-                    const float sp = ((float)plane - za) * idz[g];
-                    // the end plane is reached at sigma = 1, never crossed
-                    if (plane == end_plane || !(sp <= sig1)) break;
-                    const float len = fmaxf(sp - a, 0.0f) * D[g];
-                    out_cl[g] += clad ? len : 0.0f;
-                    out_gl[g] += clad ? 0.0f : len;
-                    out_bl[g] = fmaf(b, len, out_bl[g]);
-                    steps++;
-                    a = fmaxf(a, sp);
-                    const int rr = r[g] + (up ? 1 : -1);
-                    if (kReflect && (rr == layers || rr < 0)) {
-                        if (applied[g] < nrefl) {
-                            const float vz = (ze - za) * h / D[g];
-                            const uint32_t pl = up ? ((cp[g] & 1) ? 0u : 1u) : ((cp[g] & 1) ? 1u : 0u);
-                            out_rf[g] *= surface_factor(m, t, pl, fmaxf(0.0f, 1.0f - vz * vz));
-                            applied[g]++;
-                        }
-                        r[g] = up ? 0 : layers - 1;
-                        cp[g] += up ? 1 : -1;
-                    } else {
-                        r[g] = rr;
-                    }
-                    const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
-                    b = clad ? 0.0f : bcol[lvl];
-                }
-                const float len = fmaxf(sig1 - a, 0.0f) * D[g];
-                out_cl[g] += clad ? len : 0.0f;
-                out_gl[g] += clad ? 0.0f : len;
-                out_bl[g] = fmaf(b, len, out_bl[g]);
-                steps++;
-            }
-        }
-        s = snext;
-        const uint32_t nw = x.nw;
-        bool done = at_end;
-        if (!done && nw_boundary(nw)) {
-            if (Dxy - s <= tol) done = true;
-            else { ok = false; return; }
-        }
-        if (done) {
-            if (!kReflect) return;
-#pragma unroll
-            for (int g = 0; g < kLevelGroup; ++g) {
-                const float vz = (ze - ((float)(lv0 + g) + 0.5f)) * h / D[g];
-                const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
-                while (applied[g] < nrefl) {
-                    const uint32_t pl = (applied[g] & 1u) ? 1u - first : first;
-                    out_rf[g] *= surface_factor(m, t, pl, sin2);
-                    applied[g]++;
-                }
-            }
-            return;
-        }
-        t = nw_next(nw);
-        entry = (int)nw_edge(nw);
-    }
-    ok = false;
-}
-
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
     const uint32_t row = blockIdx.x / p.parts, part = blockIdx.x % p.parts;
@@ -383,33 +237,15 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         const uint32_t gi = rest % groups, k = rest / groups;
         const int lv0 = (int)(gi * kLevelGroup);
         const int nlev = min(kLevelGroup, (int)m.layers - lv0);
-        const uint32_t nr = (k + 1u) >> 1, fp = k & 1u;
-        const int end_plane = image_plane(sp.plane, (int)m.layers, nr, fp);
-        float bl[kLevelGroup], gl[kLevelGroup], cl[kLevelGroup], rf[kLevelGroup], len[kLevelGroup];
-        bool ok;
-        if (nr == 0)
-            column_importance<false>(m, t0, lv0, nlev, sp.px, sp.py, sp.p, end_plane, 0u, fp, bl, gl, cl, rf, len,
-                                     steps, ok);
-        else
-            column_importance<true>(m, t0, lv0, nlev, sp.px, sp.py, sp.p, end_plane, nr, fp, bl, gl, cl, rf, len,
-                                    steps, ok);
-
+        float v[kLevelGroup];
+        steps += column_entries(m, GlobalTriangles{}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px,
+                                sp.py, sp.plane, p.transfer != 0, v);
         walks += (unsigned long long)nlev;
 #pragma unroll
         for (int g = 0; g < kLevelGroup; ++g) {
             if (g < nlev) {
-                const uint32_t j = t0 + (uint32_t)(lv0 + g) * T;
-                const float bv = m.bv[j];
-                float value = 0.0f;
-                if (ok && (bv > 0.0f || p.transfer)) {
-                    WalkOut w;
-                    w.beta_len = bl[g]; w.gain_len = gl[g]; w.clad_len = cl[g];
-                    const float gain = spectral_gain(p.c1, p.c2, p.n_lambda, p.inv_lambda, m.clad_alpha, w);
-                    value = p.transfer ? gain * rf[g] : bv * gain * rf[g] / (4.0f * kPi * len[g] * len[g]);
-                    if (!(value >= 0.0f) || isinf(value)) value = 0.0f;
-                }
-                imp[(size_t)k * P + j] = value;
-                local += (double)value;
+                imp[(size_t)k * P + t0 + (uint32_t)(lv0 + g) * T] = v[g];
+                local += (double)v[g];
             }
         }
     }
@@ -422,6 +258,57 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
     mixture_prefix(local, (uint32_t)n, P, T, m.bv, p.eps, p.Bsum, p.R,
                    [&](const EntryIndex&, uint32_t e) { return imp[e]; }, pre, p.Q + slot, red, buf,
                    p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
+}
+
+// Walked importance of whole batches for meshes of up to kLdsTriangles
+// triangles: one persistent 1024-thread work-group per CU copies the compact
+// triangle records into LDS once, then its wavefronts take 64 consecutive
+// column items of the batch at a time, round-robin over the grid (a batch's
+// items are numbered slot-major, so the lanes of a wavefront walk
+// neighbouring triangle columns of one sample and slice).  Only the
+// importance values are written; hase_prefix_kernel forms each sample's
+// mixture prefix afterwards (not for transfer maps).
+constexpr int kStagedBlock = 1024;
+
+__global__ void __launch_bounds__(kStagedBlock) hase_importance_staged_kernel(ImportanceParams p)
+{
+    __shared__ uint4 s_tri[2u * kLdsTriangles];
+    const MeshDev& m = p.m;
+    const uint32_t T = m.T, P = T * m.layers;
+    for (uint32_t i = threadIdx.x; i < 2u * T; i += blockDim.x) s_tri[i] = m.lds_rec[i];
+    __syncthreads();
+    const LdsTriangles fetch{s_tri};
+    const size_t n = (size_t)p.R * P;
+    const uint32_t groups = (m.layers + kLevelGroup - 1) / kLevelGroup;
+    const uint32_t per_slot = T * p.R * groups;
+    const uint64_t total = (uint64_t)p.slots * per_slot;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves_per_block = blockDim.x / 64u;
+    const uint64_t wave = (uint64_t)blockIdx.x * waves_per_block + threadIdx.x / 64u;
+    const uint64_t stride = (uint64_t)gridDim.x * waves_per_block * 64u;
+    unsigned long long walks = 0, steps = 0;
+    for (uint64_t item = wave * 64u + lane; item < total; item += stride) {
+        const uint32_t slot = (uint32_t)(item / per_slot), it = (uint32_t)(item - (uint64_t)slot * per_slot);
+        const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
+        const SampleGeom sp = sample_point(p.pts, m.np, sample);
+        const uint32_t t0 = it % T, rest = it / T;
+        const uint32_t gi = rest % groups, k = rest / groups;
+        const int lv0 = (int)(gi * kLevelGroup);
+        const int nlev = min(kLevelGroup, (int)m.layers - lv0);
+        float v[kLevelGroup];
+        steps += column_entries(m, fetch, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px, sp.py,
+                                sp.plane, p.transfer != 0, v);
+        walks += (unsigned long long)nlev;
+        float* imp = p.imp + (size_t)slot * n + (size_t)k * P + t0;
+#pragma unroll
+        for (int g = 0; g < kLevelGroup; ++g)
+            if (g < nlev) imp[(size_t)(lv0 + g) * T] = v[g];
+    }
+    const unsigned long long ws = wave_sum_u64(walks), ss = wave_sum_u64(steps);
+    if (lane == 0 && ws) {
+        atomicAdd(&p.counters[2], ws);
+        atomicAdd(&p.counters[3], ss);
+    }
 }
 
 // beta * V proposal shared by every sample (one work-group): the mixture with
@@ -1181,6 +1068,35 @@ uint32_t importance_parts(const hase_context* ctx, uint32_t slots)
     return std::min<uint32_t>(64u, (want + slots - 1u) / slots);
 }
 
+// Walked importance of `slots` samples (and, unless ip.transfer is set, their
+// mixture prefixes): the LDS-staged persistent kernel when the mesh's compact
+// records exist, else one work-group per sample (split into ip.parts for
+// small batches).  ip.parts is set here.
+void launch_importance(hase_context* ctx, ImportanceParams& ip, uint32_t slots)
+{
+    if (ctx->lds_rec.p) {
+        int cus = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        ip.parts = 1u;
+        hipLaunchKernelGGL(hase_importance_staged_kernel, dim3((uint32_t)std::max(1, cus)), dim3(kStagedBlock), 0,
+                           ctx->stream, ip);
+        HIP_CHECK(hipGetLastError());
+        if (!ip.transfer) {
+            hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+            HIP_CHECK(hipGetLastError());
+        }
+        return;
+    }
+    // a small batch would leave most of the chip idle: split each sample's walks
+    ip.parts = importance_parts(ctx, slots);
+    hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
+    HIP_CHECK(hipGetLastError());
+    if (ip.parts > 1u && !ip.transfer) {
+        hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
 // The beta * V proposal every sample shares, into slot 0 of the prefix
 // tables (once per evaluation: beta may have changed since the last).
 void build_shared_prefix(hase_context* ctx, BatchTimers& tm)
@@ -1259,15 +1175,8 @@ void build_proposal(hase_context* ctx, uint32_t b0, const uint32_t* sample_of, u
         ip.active = nullptr;
         ip.sample_of = sample_of;
         ip.transfer = 0;
-        // a small batch would leave most of the chip idle: split each sample's walks
-        ip.parts = importance_parts(ctx, slots);
         hipEvent_t i0 = record(ctx);
-        hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
-        HIP_CHECK(hipGetLastError());
-        if (ip.parts > 1u) {
-            hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
-            HIP_CHECK(hipGetLastError());
-        }
+        launch_importance(ctx, ip, slots);
         tm.imp.emplace_back(i0, record(ctx));
         ctx->stats.importance_launches++;
     }
@@ -2043,11 +1952,7 @@ int hase_context_importance(hase_context_t* context, uint32_t sample, float* imp
         ip.active = nullptr;
         ip.sample_of = nullptr;
         ip.transfer = 0;
-        ip.parts = importance_parts(context, 1u);
-        hipLaunchKernelGGL(hase_importance_kernel, dim3(ip.parts), dim3(kBlock), 0, context->stream, ip);
-        HIP_CHECK(hipGetLastError());
-        if (ip.parts > 1u) hipLaunchKernelGGL(hase_prefix_kernel, dim3(1), dim3(kBlock), 0, context->stream, ip);
-        HIP_CHECK(hipGetLastError());
+        launch_importance(context, ip, 1u);
         std::vector<float> dev(n_ent);
         HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, n_ent * sizeof(float), hipMemcpyDeviceToHost,
                                  context->stream));
@@ -2134,9 +2039,7 @@ int hase_context_transfer_map(hase_context_t* context, uint32_t first, uint32_t 
             ip.active = nullptr;
             ip.sample_of = nullptr;
             ip.transfer = 1;
-            ip.parts = importance_parts(context, slots);
-            hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, context->stream, ip);
-            HIP_CHECK(hipGetLastError());
+            launch_importance(context, ip, slots);
             HIP_CHECK(hipMemcpyAsync(dev.data(), context->imp.p, (size_t)slots * n_ent * sizeof(float),
                                      hipMemcpyDeviceToHost, context->stream));
             HIP_CHECK(hipStreamSynchronize(context->stream));

@@ -20,6 +20,7 @@ constexpr uint32_t kKeyTag = 0x48415345u;
 constexpr uint32_t kTagRay = 0u;
 constexpr uint32_t kTagOffset = 1u;
 constexpr uint32_t kMaxSteps = 10000u;
+constexpr float kPi = 3.14159265358979f;
 
 HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float, u); }
 
@@ -355,7 +356,11 @@ HASE_HD inline void walk_begin(const MeshDev& m, WalkState& w, float ax, float a
     }
     const float d2 = dx * dx + dy * dy;
     w.inv_d2 = d2 > 0.0f ? 1.0f / d2 : 0.0f;
-    w.sin2 = d2 / (D * D);
+    // sin^2 of the angle to the plane normal, as 1 - cos^2: never above 1,
+    // so a ratio of refractive indices of 1 never reflects totally (d2 / D^2
+    // can round above 1 for grazing rays)
+    const float vz = dz / D;
+    w.sin2 = fmaxf(0.0f, 1.0f - vz * vz);
     w.tol = 2e-5f * (1.0f + D);
     w.inv_dz = dzeta != 0.0f ? 1.0f / dzeta : 0.0f;
     w.zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);

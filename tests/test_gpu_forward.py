@@ -222,10 +222,15 @@ def test_first_order_reflections_against_the_image_method():
     """One reservoir pass, zero gain, reflectivity 0.4 / 0.2: the centre
     cells' flux against the first-order image method (angular quadrature
     averaged by the vertex hat functions), and against the oracle on the same
-    primary rays (seed 5489, permuted deposit order).  The oracle's own ratio
-    to the image method scatters by about 0.3% over seeds (CPU: 0.998-1.0035
-    at 2e6 rays for seeds 1, 2, 3, 5489); seed 5489 sits at +0.35%, so the
-    device is held to the oracle's ratio on its own rays."""
+    primary rays (seed 5489).  Which deposits the 32-slot reservoirs keep
+    depends on their arrival order, so the oracle's ratio to the image method
+    moves with the deposit order even on identical primaries (CPU, 2e6 rays,
+    seed 5489, eight order strides: mean 1.0038, standard deviation 0.0013,
+    range 1.0023-1.0067) and over seeds (0.998-1.0035 for seeds 1, 2, 3).
+    The device, whose reservoirs fill in atomic order, is held to the mean of
+    three permuted orders within 0.006 (4 standard deviations of one order
+    against a mean of three), and to the image method within 1% (the
+    centroid restart's +0.4% offset plus 4.6 standard deviations)."""
     z, beta = 0.5, SA / (SA + SE)
     mesh, fs = _slab(z=z, beta=beta)
     print("[forward] first-order reflections, 2e6 rays", flush=True)
@@ -235,9 +240,11 @@ def test_first_order_reflections_against_the_image_method():
         st = ctx.stats()
     assert st["srm_status_name"] == "MAX_PASSES" and st["srm_passes"] == 1
     o = F.ForwardOracle(mesh)
-    refl = F.reflecting(o, fs, [0.0, 0.4, 0.2], max_passes=1, order_stride=1000003)
-    print("[forward] first-order reflections: oracle, 2e6 rays", flush=True)
-    ref = F.evaluate_srm(o, refl, [SA], [SE], min_rays=2_000_000, max_rays=2_000_000, steps=0, seed=5489)
+    refs = []
+    for stride in (1000003, 7919, 65537):
+        refl = F.reflecting(o, fs, [0.0, 0.4, 0.2], max_passes=1, order_stride=stride)
+        print(f"[forward] first-order reflections: oracle, 2e6 rays, deposit order stride {stride}", flush=True)
+        refs.append(F.evaluate_srm(o, refl, [SA], [SE], min_rays=2_000_000, max_rays=2_000_000, steps=0, seed=5489))
     cen = mesh.centroids()
     near = np.nonzero(np.linalg.norm((cen - [0.5, 0.5, z / 2]) / [1, 1, z], axis=1) < 0.12)[0]
 
@@ -248,11 +255,12 @@ def test_first_order_reflections_against_the_image_method():
     with np.errstate(invalid="ignore"):
         expect = hat_smoothed_cell_flux(mesh.points, mesh.cells, mesh.volumes(), near, flux)
     ratio = res.phiAse[near].astype(np.float64).sum() / expect.sum()
-    oracle_ratio = ref["phiAse"][near].astype(np.float64).sum() / expect.sum()
-    print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}, oracle {oracle_ratio:.5f}",
-          flush=True)
-    assert abs(ratio / oracle_ratio - 1.0) < 0.003, (ratio, oracle_ratio)
-    assert abs(ratio - 1.0) < 0.008, ratio
+    oracle_ratios = [r["phiAse"][near].astype(np.float64).sum() / expect.sum() for r in refs]
+    oracle_ratio = float(np.mean(oracle_ratios))
+    print(f"[forward] first-order reflections: ratio to the image method {ratio:.5f}, oracle "
+          f"{', '.join(f'{v:.5f}' for v in oracle_ratios)} (mean {oracle_ratio:.5f})", flush=True)
+    assert abs(ratio / oracle_ratio - 1.0) < 0.006, (ratio, oracle_ratios)
+    assert abs(ratio - 1.0) < 0.01, ratio
 
 
 def test_edge_cases_zero_beta_few_rays_single_cell():

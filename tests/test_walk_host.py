@@ -39,6 +39,9 @@ def walk_lib(tmp_path_factory):
     u32p, f32p, i32p = C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
     lib.walk_rays_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, f32p, f32p, f32p, C.c_uint32,
                                    f32p, u32p, f32p, u32p, u32p, f32p, u32p, i32p]
+    lib.importance_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, f32p, f32p, f32p, f32p,
+                                    C.c_float, f32p, f32p, C.c_uint32, C.c_float, C.c_float, C.c_int, C.c_uint32,
+                                    C.c_int, f32p, C.POINTER(C.c_ulonglong)]
     return lib
 
 
@@ -172,3 +175,92 @@ def test_host_walk_matches_the_oracle(walk_lib, name, staged):
     rel = np.array(rel)
     assert rel.max() <= 1e-5, f"max single-ray gain error {rel.max():.3e}"
     assert same_steps >= 0.99 * len(prisms) and worst <= 3
+
+
+def test_grazing_reflections_with_equal_indices_never_reflect_totally(walk_lib):
+    """Refractive indices 1 on both sides of both planes (the production
+    disk's setting): a ray reflected at a plane it runs nearly parallel to
+    takes the triangle's reflectivity, never total reflection.  Starts just
+    above the bottom plane, ends at bottom-level sample points through one
+    reflection at the bottom (the image lies on the sample's own plane), so
+    the rays graze the plane; sin^2 of the incidence angle computed as
+    |xy|^2 / D^2 rounds above 1 in FP32 for such rays."""
+    w = _f32_exact(production_disk(levels=6, n_points=400, wavelengths=4))
+    assert list(w.medium.value("refractiveIndices")) == [1.0, 1.0, 1.0, 1.0]
+    om = O.OracleMesh(w.topology, w.medium)
+    rec, lds, beta = _records(w)
+    topo = w.topology
+    T, Np = topo.numberOfTriangles, topo.numberOfPoints
+    rng = np.random.default_rng(4)
+    n = 4000
+    prisms = rng.integers(0, T, n)                      # bottom layer
+    starts = np.array([O.ray_start(om, 9, 0, 0, i, int(j)) for i, j in enumerate(prisms)])
+    starts[:, 2] = rng.uniform(1e-6, 1e-3, n) * topo.thickness
+    starts = starts.astype(np.float32).astype(np.float64)
+    sp = topo.samplePoints()[rng.integers(0, Np, n)]    # level 0
+    nr, fp = np.ones(n, np.int64), np.zeros(n, np.int64)
+    ends = np.array([O.image_point(om, sp[i], 1, 0) for i in range(n)]).astype(np.float32).astype(np.float64)
+    for staged in (False, True):
+        out, _, status = _walk(walk_lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, staged)
+        for i in range(n):
+            o = O.walk_ray(om, starts[i], int(prisms[i]), ends[i], 1, 0)
+            if o["status"] != 0:
+                continue
+            assert status[i] == 0
+            assert out[i, 3] == pytest.approx(o["refl"], rel=1e-6, abs=1e-7), (i, out[i, 3], o["refl"])
+            assert out[i, 3] < 1.0
+
+
+@pytest.mark.parametrize("staged", [False, True], ids=["global", "lds"])
+@pytest.mark.parametrize("name", ["plain_small", "disk_reflect_2"])
+def test_host_importance_matches_the_oracle(walk_lib, name, staged):
+    """The importance kernel's column walks (one xy side-test walk per
+    triangle column, each level's ray split at its own plane crossings)
+    against the oracle's per-prism centroid walks, FP32 against FP64 on an
+    FP32-exact mesh: rtol 1e-4 per entry, entries below 1e-6 of the largest
+    by absolute tolerance (the GPU test's criterion)."""
+    if name == "plain_small":
+        w, R, reflect = plain_small(), 1, False
+    else:
+        w, R, reflect = production_disk(levels=5, n_points=300, wavelengths=8), 5, True
+        w.medium.set("refractiveIndices", [1.83, 1.0, 1.83, 1.0])
+    w = _f32_exact(w)
+    topo, med = w.topology, w.medium
+    T, L, Np = topo.numberOfTriangles, int(topo.levels), topo.numberOfPoints
+    sa, se, msa, mse_ = w.spectra.spectralPairs(bool(w.settings.get("monochromatic", False)))
+    om = O.OracleMesh(topo, med)
+    op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_, n_tot=float(med.value("nTot")),
+                        tau=float(med.value("crystalTFluo")), beta_cells=med.value("betaCells"), n_samples=w.samples,
+                        min_rays=1, use_reflections=reflect, max_reflections=2)
+    rec, lds, beta = _records(w)
+    h = float(topo.thickness)
+    tri = topo.trianglePointIndices
+    ea = topo.points[tri[:, 1]] - topo.points[tri[:, 0]]
+    eb = topo.points[tri[:, 2]] - topo.points[tri[:, 0]]
+    area = 0.5 * np.abs(ea[:, 0] * eb[:, 1] - ea[:, 1] * eb[:, 0])
+    bvol = med.value("betaVolume").astype(np.float64).reshape(L - 1, T)
+    bv = np.where(bvol > 0, bvol * area[None, :] * h, 0.0).astype(np.float32).reshape(-1)
+    nt = float(med.value("nTot"))
+    log2e = 1.4426950408889634
+    c1 = (nt * (np.asarray(se) + np.asarray(sa)) * log2e).astype(np.float32)
+    c2 = (nt * np.asarray(sa) * log2e).astype(np.float32)
+    refl = np.ascontiguousarray(med.value("reflectivities"), np.float32) if reflect else None
+    nidx = np.ascontiguousarray(med.value("refractiveIndices"), np.float32) if reflect else None
+    pts = topo.points.astype(np.float32)
+    keep = [np.ascontiguousarray(a) for a in (rec, lds, beta, bv, c1, c2)]
+    ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t)) if a is not None else None   # noqa: E731
+    rng = np.random.default_rng(3)
+    for s in rng.integers(0, w.samples, 5):
+        p, lv = int(s) % Np, int(s) // Np
+        out = np.zeros(R * T * (L - 1), np.float32)
+        steps = C.c_ulonglong(0)
+        walk_lib.importance_host(T, L - 1, np.float32(h), ptr(keep[0], C.c_uint32),
+                                 ptr(keep[1], C.c_uint32) if staged else None, ptr(keep[2], C.c_float),
+                                 ptr(keep[3], C.c_float), ptr(refl, C.c_float), ptr(nidx, C.c_float),
+                                 np.float32(med.value("claddingAbsorption")), ptr(keep[4], C.c_float),
+                                 ptr(keep[5], C.c_float), len(c1), pts[p, 0], pts[p, 1], lv, R, 0,
+                                 out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(steps))
+        ref = O.importance(om, op, int(s))
+        np.testing.assert_allclose(out.astype(np.float64), ref, rtol=1e-4, atol=1e-6 * ref.max(),
+                                   err_msg=f"sample {s}")
+        assert steps.value > T * (L - 1)

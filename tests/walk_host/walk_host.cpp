@@ -1,7 +1,11 @@
-// Host build of the device prism walk (haseongpu_amd/csrc/prism_walk.h) for
-// CPU tests: the same source, compiled for the host, walked over meshes the
+// Host build of the device prism walk (haseongpu_amd/csrc/prism_walk.h) and
+// the importance map's column walks (importance_walk.h) for CPU tests: the same source, compiled for the host, walked over meshes the
 // test packs into the device record layouts (48-byte global records and the
 // 32-byte records staged in LDS).  Test infrastructure only.
+#include <algorithm>
+#include <vector>
+
+#include "../../haseongpu_amd/csrc/importance_walk.h"
 #include "../../haseongpu_amd/csrc/prism_walk.h"
 
 using namespace hase;
@@ -39,5 +43,49 @@ extern "C" int walk_rays_host(uint32_t T, uint32_t layers, float h, const uint32
         steps[i] = w.steps;
         status[i] = w.status;
     }
+    return 0;
+}
+
+// Walked importance map of one sample (every reflection slice and prism, in
+// the caller's prism order) by the importance kernel's column walks.
+extern "C" int importance_host(uint32_t T, uint32_t layers, float h, const uint32_t* rec, const uint32_t* lds_rec,
+                               const float* beta, const float* bv, const float* refl, const float* nidx, float clad_alpha,
+                               const float* c1, const float* c2, uint32_t n_lambda, float px, float py, int plane,
+                               uint32_t R, int transfer, float* out, unsigned long long* steps)
+{
+    MeshDev m{};
+    m.rec = reinterpret_cast<const uint4*>(rec);
+    m.beta = beta;
+    m.bv = bv;
+    m.refl = refl;
+    m.T = T;
+    m.layers = layers;
+    m.h = h;
+    m.clad_alpha = clad_alpha;
+    if (nidx) {
+        m.has_nidx = 1;
+        m.nin_b = nidx[0]; m.nout_b = nidx[1]; m.nin_t = nidx[2]; m.nout_t = nidx[3];
+    }
+    // column-major copy of beta, as the library keeps it for the column walks
+    std::vector<float> col((size_t)T * layers);
+    for (uint32_t l = 0; l < layers; ++l)
+        for (uint32_t t = 0; t < T; ++t) col[(size_t)t * layers + l] = beta[t + (size_t)l * T];
+    m.beta_col = col.data();
+    const uint32_t P = T * layers, groups = (layers + kLevelGroup - 1) / kLevelGroup;
+    *steps = 0;
+    for (uint32_t k = 0; k < R; ++k)
+        for (uint32_t gi = 0; gi < groups; ++gi)
+            for (uint32_t t0 = 0; t0 < T; ++t0) {
+                const int lv0 = (int)(gi * kLevelGroup);
+                const int nlev = std::min(kLevelGroup, (int)layers - lv0);
+                float v[kLevelGroup];
+                if (lds_rec)
+                    *steps += column_entries(m, LdsTriangles{reinterpret_cast<const uint4*>(lds_rec)}, c1, c2, n_lambda,
+                                             1.0f / (float)n_lambda, t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
+                else
+                    *steps += column_entries(m, GlobalTriangles{}, c1, c2, n_lambda, 1.0f / (float)n_lambda, t0, lv0,
+                                             nlev, k, px, py, plane, transfer != 0, v);
+                for (int g = 0; g < nlev; ++g) out[(size_t)k * P + t0 + (size_t)(lv0 + g) * T] = v[g];
+            }
     return 0;
 }
