@@ -227,16 +227,15 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
 
     double local = 0.0;
     unsigned long long walks = 0, steps = 0;
-    const uint32_t groups = (m.layers + kLevelGroup - 1) / kLevelGroup;
-    const uint32_t items = T * p.R * groups;
+    const ColumnItems ci(T, m.layers, p.R);
+    const uint32_t items = ci.per_sample();
     // this work-group's share of the sample's column walks
     const uint32_t it0 = (uint32_t)((uint64_t)items * part / p.parts);
     const uint32_t it1 = (uint32_t)((uint64_t)items * (part + 1u) / p.parts);
     for (uint32_t it = it0 + tid; it < it1; it += kBlock) {
-        const uint32_t t0 = it % T, rest = it / T;
-        const uint32_t gi = rest % groups, k = rest / groups;
-        const int lv0 = (int)(gi * kLevelGroup);
-        const int nlev = min(kLevelGroup, (int)m.layers - lv0);
+        uint32_t t0, k;
+        int lv0, nlev;
+        ci.decode(it, t0, k, lv0, nlev);
         float v[kLevelGroup];
         steps += column_entries(m, GlobalTriangles{}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px,
                                 sp.py, sp.plane, p.transfer != 0, v);
@@ -279,8 +278,8 @@ __global__ void __launch_bounds__(kStagedBlock) hase_importance_staged_kernel(Im
     __syncthreads();
     const LdsTriangles fetch{s_tri};
     const size_t n = (size_t)p.R * P;
-    const uint32_t groups = (m.layers + kLevelGroup - 1) / kLevelGroup;
-    const uint32_t per_slot = T * p.R * groups;
+    const ColumnItems ci(T, m.layers, p.R);
+    const uint32_t per_slot = ci.per_sample();
     const uint64_t total = (uint64_t)p.slots * per_slot;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves_per_block = blockDim.x / 64u;
@@ -291,10 +290,9 @@ __global__ void __launch_bounds__(kStagedBlock) hase_importance_staged_kernel(Im
         const uint32_t slot = (uint32_t)(item / per_slot), it = (uint32_t)(item - (uint64_t)slot * per_slot);
         const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
         const SampleGeom sp = sample_point(p.pts, m.np, sample);
-        const uint32_t t0 = it % T, rest = it / T;
-        const uint32_t gi = rest % groups, k = rest / groups;
-        const int lv0 = (int)(gi * kLevelGroup);
-        const int nlev = min(kLevelGroup, (int)m.layers - lv0);
+        uint32_t t0, k;
+        int lv0, nlev;
+        ci.decode(it, t0, k, lv0, nlev);
         float v[kLevelGroup];
         steps += column_entries(m, fetch, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px, sp.py,
                                 sp.plane, p.transfer != 0, v);

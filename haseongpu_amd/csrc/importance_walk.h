@@ -18,26 +18,66 @@
 
 namespace hase {
 
-// Levels whose centroid walks share one xy walk in the importance kernel.
-constexpr int kLevelGroup = 4;
+// Levels whose centroid walks share one xy walk in the importance kernels:
+// more for the direct slice than for the mirror slices, whose per-level
+// copy and surface-factor state costs more registers (both groups keep the
+// kernels within 128 VGPRs, four wavefronts per SIMD).
+constexpr int kGroupDirect = 8;
+constexpr int kGroupMirror = 4;
+constexpr int kLevelGroup = kGroupDirect;   // the larger of the two
+
+// Column items of one sample: for each reflection slice k and each group of
+// that slice's levels, one item per triangle column t0 (items of slice 0
+// first, then the mirror slices; columns fastest, so the lanes of a
+// wavefront walk neighbouring columns).
+struct ColumnItems {
+    uint32_t T, layers, R, g0, g1;
+    HASE_HD ColumnItems(uint32_t T_, uint32_t layers_, uint32_t R_)
+        : T(T_), layers(layers_), R(R_), g0((layers_ + kGroupDirect - 1) / kGroupDirect),
+          g1((layers_ + kGroupMirror - 1) / kGroupMirror)
+    {
+    }
+    HASE_HD uint32_t per_sample() const { return T * (g0 + (R - 1u) * g1); }
+    HASE_HD void decode(uint32_t it, uint32_t& t0, uint32_t& k, int& lv0, int& nlev) const
+    {
+        t0 = it % T;
+        uint32_t rest = it / T;
+        int group;
+        if (rest < g0) {
+            k = 0u;
+            group = kGroupDirect;
+        } else {
+            rest -= g0;
+            k = 1u + rest / g1;
+            rest %= g1;
+            group = kGroupMirror;
+        }
+        lv0 = (int)rest * group;
+        nlev = (int)layers - lv0 < group ? (int)layers - lv0 : group;
+    }
+};
 
 // Importance of the prisms (t0, lv0 .. lv0+nlev-1) for one reflection slice.
 // Every level (and every mirror slice) of a triangle column projects onto the
 // same xy segment from the triangle's centroid to the sample point, so the
-// column is walked once in xy and each level's ray is advanced along it,
-// split at that level's own plane crossings.  The per-level sums are those of
-// walk_ray on the same ray: sum(beta l), sum(l) over gain prisms, sum(l) over
-// cladding prisms, the surface factors at the copy boundaries, and the
-// prisms visited.  The xy walk is the side-test walk of prism_walk.h (one
-// new vertex and one side test per triangle), parametrised by sigma in
-// [0, 1] from the centroid to the sample point.
-template <bool kReflect, class Fetch>
-HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint32_t t0, int lv0,
-                                                  int nlev, float px, float py, int end_plane, uint32_t nrefl,
-                                                  uint32_t first, float out_bl[kLevelGroup],
-                                                  float out_gl[kLevelGroup], float out_cl[kLevelGroup],
-                                                  float out_rf[kLevelGroup], float D[kLevelGroup],
-                                                  unsigned long long& steps, bool& ok)
+// column is walked once in xy and each level's ray is advanced along it.
+// All of them are parametrised by the same sigma in [0, 1] (xy and height are
+// both linear in sigma, lengths are sigma * D of each level), so what a
+// triangle step adds is shared except for beta: the cladding share of sigma
+// is one sum for the column, and per level only sum(beta dsigma) is kept,
+// split at that level's own plane crossings (a crossing is where the level's
+// ray enters the next layer; the next one is tracked as its sigma).  The
+// per-level results are those of walk_ray on the same ray: sum(beta l),
+// sum(l) over gain prisms, sum(l) over cladding prisms, the surface factors
+// at the copy boundaries, and the prisms visited.  The xy walk is the
+// side-test walk of prism_walk.h (one new vertex and one side test per
+// triangle).
+template <int G, bool kReflect, class Fetch>
+HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint32_t t0, int lv0, int nlev, float px,
+                                      float py, int end_plane, uint32_t nrefl, uint32_t first,
+                                      float out_bl[G], float out_gl[G],
+                                      float out_cl[G], float out_rf[G], float D[G],
+                                      unsigned long long& steps, bool& ok)
 {
     const int layers = (int)m.layers;
     const float h = m.h;
@@ -49,19 +89,22 @@ HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint
     const float Dxy = sqrtf(d2);
     const float inv_d2 = d2 > 0.0f ? 1.0f / d2 : 0.0f;
     const float ze = (float)end_plane;
-    // per level: parameter of a plane = (plane - za) * idz; unfolded layer u = cp * layers + r
-    float idz[kLevelGroup];
-    int r[kLevelGroup], cp[kLevelGroup];
-    uint32_t applied[kLevelGroup];
+    // per level: unfolded layer u = cp * layers + r (the prism it is in has
+    // folded level r, or layers - 1 - r in odd mirror copies), the sigma of
+    // its next plane crossing (measured back from the end, so the end's
+    // plane sits at exactly 1 and is never crossed), sum(beta dsigma) and the
+    // surface factors so far (one per copy boundary crossed: |cp| of them)
+    int r[G], cp[G];
+    float snp[G], bsum[G];
 #pragma unroll
-    for (int g = 0; g < kLevelGroup; ++g) {
+    for (int g = 0; g < G; ++g) {
         const float dz = ze - ((float)(lv0 + g) + 0.5f);
-        idz[g] = 1.0f / dz;
-        D[g] = sqrtf(d2 + dz * h * dz * h);
         r[g] = lv0 + g;
         cp[g] = 0;
-        applied[g] = 0;
-        out_bl[g] = 0.0f; out_gl[g] = 0.0f; out_cl[g] = 0.0f; out_rf[g] = 1.0f;
+        const int plane = lv0 + g + (dz > 0.0f ? 1 : 0);
+        snp[g] = fmaf((float)plane - ze, fast_rcp(dz), 1.0f);
+        bsum[g] = 0.0f;
+        out_rf[g] = 1.0f;
     }
     const float tol = 2e-5f * (1.0f + Dxy);
     // exit of the start triangle: the edge whose start vertex is right of the
@@ -90,60 +133,56 @@ HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint
                            d2 == 0.0f;
     float sxy = start_end ? 1.0f : edge_sigma(cl, wl, cr, wr, inv_d2);
     uint32_t t = t0;
-    float s = 0.0f;
+    float s = 0.0f;          // sigma reached
+    float clad_s = 0.0f;     // sum of dsigma over cladding triangles
     ok = true;
     for (uint32_t step = 0; step < kMaxSteps; ++step) {
         const float* bcol = m.beta_col + (size_t)t * (uint32_t)layers;
         // this column's beta at every level's current layer: independent loads, one round trip
-        float bcur[kLevelGroup];
+        float b[G];
 #pragma unroll
-        for (int g = 0; g < kLevelGroup; ++g) {
-            const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
-            bcur[g] = (g < nlev && !clad) ? bcol[lvl] : 0.0f;
+        for (int g = 0; g < G; ++g) {
+            const int lvl = (kReflect && (cp[g] & 1)) ? (layers - 1 - r[g]) : r[g];
+            b[g] = (g < nlev && !clad) ? bcol[lvl] : 0.0f;
         }
         const float snext = fmaxf(fminf(sxy, 1.0f), s);
         const bool at_end = snext >= 1.0f;
         const float sig0 = s;
         const float sig1 = at_end ? 1.0f : snext;
+        clad_s = fmaf((float)clad, sig1 - sig0, clad_s);
+        steps += (unsigned long long)nlev;
 #pragma unroll
-        for (int g = 0; g < kLevelGroup; ++g) {
+        for (int g = 0; g < G; ++g) {
             if (g < nlev) {
-                const float za = (float)(lv0 + g) + 0.5f;
-                const bool up = idz[g] > 0.0f;
                 float a = sig0;
-                float b = bcur[g];
-                for (;;) {
-                    const int plane = cp[g] * layers + r[g] + (up ? 1 : 0);
-                    const float sp = ((float)plane - za) * idz[g];
-                    // the end plane is reached at sigma = 1, never crossed
-                    if (plane == end_plane || !(sp <= sig1)) break;
-                    const float len = fmaxf(sp - a, 0.0f) * D[g];
-                    out_cl[g] += clad ? len : 0.0f;
-                    out_gl[g] += clad ? 0.0f : len;
-                    out_bl[g] = fmaf(b, len, out_bl[g]);
+                float bg = b[g];
+                while (snp[g] < sig1) {
+                    // the level's ray enters the next layer inside this triangle
+                    const float sp = fmaxf(snp[g], a);
+                    bsum[g] = fmaf(bg, sp - a, bsum[g]);
+                    a = sp;
                     steps++;
-                    a = fmaxf(a, sp);
+                    const float dz = ze - ((float)(lv0 + g) + 0.5f);
+                    const bool up = dz > 0.0f;
                     const int rr = r[g] + (up ? 1 : -1);
                     if (kReflect && (rr == layers || rr < 0)) {
-                        if (applied[g] < nrefl) {
-                            const float vz = (ze - za) * h / D[g];
+                        if ((uint32_t)abs(cp[g]) < nrefl) {
+                            const float Dg = sqrtf(d2 + dz * h * dz * h);
+                            const float vz = dz * h / Dg;
                             const uint32_t pl_ = up ? ((cp[g] & 1) ? 0u : 1u) : ((cp[g] & 1) ? 1u : 0u);
                             out_rf[g] *= surface_factor(m, t, pl_, fmaxf(0.0f, 1.0f - vz * vz));
-                            applied[g]++;
                         }
                         r[g] = up ? 0 : layers - 1;
                         cp[g] += up ? 1 : -1;
                     } else {
                         r[g] = rr;
                     }
-                    const int lvl = (cp[g] & 1) ? (layers - 1 - r[g]) : r[g];
-                    b = clad ? 0.0f : bcol[lvl];
+                    const int lvl = (kReflect && (cp[g] & 1)) ? (layers - 1 - r[g]) : r[g];
+                    bg = clad ? 0.0f : bcol[lvl];
+                    const int plane = cp[g] * layers + r[g] + (up ? 1 : 0);
+                    snp[g] = fmaf((float)plane - ze, fast_rcp(dz), 1.0f);
                 }
-                const float len = fmaxf(sig1 - a, 0.0f) * D[g];
-                out_cl[g] += clad ? len : 0.0f;
-                out_gl[g] += clad ? 0.0f : len;
-                out_bl[g] = fmaf(b, len, out_bl[g]);
-                steps++;
+                bsum[g] = fmaf(bg, sig1 - a, bsum[g]);
             }
         }
         s = snext;
@@ -155,15 +194,22 @@ HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint
             else { ok = false; return; }
         }
         if (done) {
-            if (!kReflect) return;
+            const float stot = fminf(s, 1.0f);
 #pragma unroll
-            for (int g = 0; g < kLevelGroup; ++g) {
-                const float vz = (ze - ((float)(lv0 + g) + 0.5f)) * h / D[g];
-                const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
-                while (applied[g] < nrefl) {
-                    const uint32_t pl_ = (applied[g] & 1u) ? 1u - first : first;
-                    out_rf[g] *= surface_factor(m, t, pl_, sin2);
-                    applied[g]++;
+            for (int g = 0; g < G; ++g) {
+                const float dz = ze - ((float)(lv0 + g) + 0.5f);
+                D[g] = sqrtf(d2 + dz * h * dz * h);
+                out_bl[g] = bsum[g] * D[g];
+                out_cl[g] = clad_s * D[g];
+                out_gl[g] = (stot - clad_s) * D[g];
+                if (kReflect) {
+                    // an end point on a reflecting plane takes its last factors here
+                    const float vz = dz * h / D[g];
+                    const float sin2 = fmaxf(0.0f, 1.0f - vz * vz);
+                    for (uint32_t n = (uint32_t)abs(cp[g]); n < nrefl; ++n) {
+                        const uint32_t pl_ = (n & 1u) ? 1u - first : first;
+                        out_rf[g] *= surface_factor(m, t, pl_, sin2);
+                    }
                 }
             }
             return;
@@ -189,7 +235,6 @@ HASE_HD inline void column_importance(const MeshDev& m, const Fetch& fetch, uint
     ok = false;
 }
 
-
 // Importance values of one column item: prisms (t0, lv0 .. lv0 + nlev - 1)
 // of reflection slice k for the sample at (px, py) on level `plane`; out[g]
 // receives the entry of level lv0 + g (the transfer factor gain * surface
@@ -206,9 +251,11 @@ HASE_HD inline unsigned long long column_entries(const MeshDev& m, const Fetch& 
     bool ok;
     unsigned long long steps = 0;
     if (nr == 0)
-        column_importance<false>(m, fetch, t0, lv0, nlev, px, py, end_plane, 0u, fp, bl, gl, cl, rf, len, steps, ok);
+        column_importance<kGroupDirect, false>(m, fetch, t0, lv0, nlev, px, py, end_plane, 0u, fp, bl, gl, cl, rf,
+                                               len, steps, ok);
     else
-        column_importance<true>(m, fetch, t0, lv0, nlev, px, py, end_plane, nr, fp, bl, gl, cl, rf, len, steps, ok);
+        column_importance<kGroupMirror, true>(m, fetch, t0, lv0, nlev, px, py, end_plane, nr, fp, bl, gl, cl, rf,
+                                              len, steps, ok);
 #pragma unroll
     for (int g = 0; g < kLevelGroup; ++g) {
         float value = 0.0f;

@@ -296,7 +296,8 @@ struct WalkState {
     float ex, ey;             // end (x, y)
     float dx, dy, inv_d2;     // e - a in xy and 1 / |e - a|^2
     float D, inv_dz, tol, sin2;
-    int zdir, end_plane, end_on_plane;
+    float ze;                 // end height (layer units)
+    int zdir;
     uint32_t nrefl, first;
     uint32_t t;               // current triangle
     uint32_t clad;            // its cladding flag
@@ -364,8 +365,7 @@ HASE_HD inline void walk_begin(const MeshDev& m, WalkState& w, float ax, float a
     w.tol = 2e-5f * (1.0f + D);
     w.inv_dz = dzeta != 0.0f ? 1.0f / dzeta : 0.0f;
     w.zdir = dzeta > 0.0f ? 1 : (dzeta < 0.0f ? -1 : 0);
-    w.end_plane = (int)floorf(ze + 0.5f);
-    w.end_on_plane = (float)w.end_plane == ze;
+    w.ze = ze;
     // exit of the start triangle: the edge whose start vertex is right of the
     // line and whose end vertex is left of it (counter-clockwise order)
     const float c0 = side_of(dx, dy, p0.x - ax, p0.y - ay), c1 = side_of(dx, dy, p1.x - ax, p1.y - ay);
@@ -411,15 +411,17 @@ HASE_HD inline void walk_advance(const MeshDev& m, WalkState& w, uint32_t budget
             return;
         }
         w.steps++;
-        // next level plane along the line (exactly sigma 1 for the end's plane)
+        // next level plane along the line, measured back from the end: the
+        // end's own plane (every sample point and mirror image lies on one) is
+        // reached at sigma = 1 exactly, and a level ray (inv_dz = 0) meets no
+        // plane before the end
         const int plane = w.u + (w.zdir > 0 ? 1 : 0);
-        float sz = w.zdir != 0 ? ((float)plane - w.za) * w.inv_dz : INFINITY;
-        sz = (w.end_on_plane && plane == w.end_plane && w.zdir != 0) ? 1.0f : sz;
+        const float sz = fmaf((float)plane - w.ze, w.inv_dz, 1.0f);
         const bool zstep = sz <= w.sxy;
         float snext = fminf(fminf(w.sxy, sz), 1.0f);
         snext = fmaxf(snext, w.sig);
         const float seg = snext - w.sig;
-        w.clad_sig += w.clad ? seg : 0.0f;
+        w.clad_sig = fmaf((float)w.clad, seg, w.clad_sig);
         // the walk's beta copy is zero in cladding prisms
         w.beta_sig = fmaf(w.beta, seg, w.beta_sig);
         w.sig = snext;

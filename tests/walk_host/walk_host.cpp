@@ -71,21 +71,21 @@ extern "C" int importance_host(uint32_t T, uint32_t layers, float h, const uint3
     for (uint32_t l = 0; l < layers; ++l)
         for (uint32_t t = 0; t < T; ++t) col[(size_t)t * layers + l] = beta[t + (size_t)l * T];
     m.beta_col = col.data();
-    const uint32_t P = T * layers, groups = (layers + kLevelGroup - 1) / kLevelGroup;
+    const uint32_t P = T * layers;
+    const ColumnItems ci(T, layers, R);
     *steps = 0;
-    for (uint32_t k = 0; k < R; ++k)
-        for (uint32_t gi = 0; gi < groups; ++gi)
-            for (uint32_t t0 = 0; t0 < T; ++t0) {
-                const int lv0 = (int)(gi * kLevelGroup);
-                const int nlev = std::min(kLevelGroup, (int)layers - lv0);
-                float v[kLevelGroup];
-                if (lds_rec)
-                    *steps += column_entries(m, LdsTriangles{reinterpret_cast<const uint4*>(lds_rec)}, c1, c2, n_lambda,
-                                             1.0f / (float)n_lambda, t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
-                else
-                    *steps += column_entries(m, GlobalTriangles{}, c1, c2, n_lambda, 1.0f / (float)n_lambda, t0, lv0,
-                                             nlev, k, px, py, plane, transfer != 0, v);
-                for (int g = 0; g < nlev; ++g) out[(size_t)k * P + t0 + (size_t)(lv0 + g) * T] = v[g];
-            }
+    for (uint32_t it = 0; it < ci.per_sample(); ++it) {
+        uint32_t t0, k;
+        int lv0, nlev;
+        ci.decode(it, t0, k, lv0, nlev);
+        float v[kLevelGroup];
+        if (lds_rec)
+            *steps += column_entries(m, LdsTriangles{reinterpret_cast<const uint4*>(lds_rec)}, c1, c2, n_lambda,
+                                     1.0f / (float)n_lambda, t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
+        else
+            *steps += column_entries(m, GlobalTriangles{}, c1, c2, n_lambda, 1.0f / (float)n_lambda, t0, lv0, nlev, k,
+                                     px, py, plane, transfer != 0, v);
+        for (int g = 0; g < nlev; ++g) out[(size_t)k * P + t0 + (size_t)(lv0 + g) * T] = v[g];
+    }
     return 0;
 }
