@@ -399,3 +399,30 @@ def test_flat_schedule_terminates():
     r = O.calc_phi_ase(om, op)
     assert np.all(r["totalRays"][:21] == 500)
     assert O.adaptive_target(500, 5000, 0, 7) == 500
+
+
+def test_high_budget_fixture_matches_the_oracle_at_a_small_budget():
+    """tests/golden/production_disk_1e9.npz (the FP64 oracle at 1e9 rays per
+    sample on six production-disk samples) against the oracle rerun here at
+    1e6 rays with another seed: within 4 combined standard errors, and its
+    own relative standard errors below 1e-3."""
+    from pathlib import Path
+
+    from haseongpu_amd.workloads import production_disk
+
+    hb = np.load(Path(__file__).resolve().parent / "golden" / "production_disk_1e9.npz")
+    assert np.all(hb["totalRays"] == 1_000_000_000) and np.all(np.sqrt(hb["mse"]) < 1e-3)
+    w = production_disk()
+    s = w.settings
+    sa, se, msa, mse_ = w.spectra.spectralPairs(False)
+    om = O.OracleMesh(w.topology, w.medium)
+    op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_,
+                        n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")),
+                        beta_cells=w.medium.value("betaCells"), min_rays=1_000_000, max_rays=1_000_000,
+                        mse_threshold=0.0, use_reflections=True, max_reflections=s["maxReflections"],
+                        adaptive_steps=0, seed=4321, n_samples=w.samples, samples=hb["samples"])
+    ref = O.calc_phi_ase(om, op)
+    smp = hb["samples"]
+    z = (ref["phiAse"][smp] - hb["phiAse"]) / np.sqrt(ref["mse"][smp] * ref["phiAse"][smp] ** 2
+                                                     + hb["mse"] * hb["phiAse"] ** 2)
+    assert np.all(np.abs(z) < 4), z
