@@ -39,10 +39,16 @@ sys.path.insert(0, str(ROOT))
 METRIC = "wall-time to max MSE<=0.005 + rays/s: learned-proposal vs plain IS, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# MI355X: 256 CUs at 2.4 GHz; one wave64 VALU instruction per CU per cycle
+# (4 SIMDs of 16 lanes, four cycles per wave64 instruction)
+CUS = 256
+CLOCK_HZ = 2.4e9
 # the dominant kernel's launches are queue rounds: the HIP events of the
 # library bracket plan + trace + finalize of every round (the trace kernel is
 # > 97% of a round in the kernel traces under profiles/)
 DOMINANT = "hase_queue_trace_kernel"
+# its instantiation on meshes staged in LDS (<refill lanes, problem table, LDS records>)
+STAGED_TRACE = "hase_queue_trace_kernel<false, false, true>"
 # algorithmic cost of one prism step of the walk (rays and importance walks)
 BYTES_PER_STEP = 56
 FLOPS_PER_STEP = 40
@@ -547,7 +553,8 @@ def traffic_pass(w, args):
     copyBuffer kernels) whose known byte counts calibrate the counters."""
     from haseongpu_amd import proposal as prop
 
-    ctx, _ = build_context(w)
+    batch = int(args.sample_batches.split(",")[-1])
+    ctx, _ = build_context(w, sample_batch=batch)
     if args.load_proposal:
         ct, cl = (int(v) for v in args.clusters.lower().split("x"))
         ctx.set_proposal(prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl))
@@ -556,7 +563,7 @@ def traffic_pass(w, args):
         ctx.close()
     else:
         learned = None
-    ctx, _ = build_context(w, max_sample_range=w.topology.numberOfPoints - 1)
+    ctx, _ = build_context(w, sample_batch=batch, max_sample_range=w.topology.numberOfPoints - 1)
     ctx.evaluate()
     plain = ctx.stats()
     ctx.close()
@@ -580,6 +587,90 @@ def traffic_pass(w, args):
         hip.hipFree(dst)
     print(json.dumps({"learned": learned, "plain_first_level": plain, "fill_bytes": nbytes, "copy_bytes": nbytes}),
           flush=True)
+
+
+def reduce_counters(args):
+    """profiles/traffic.json from the rocprofv3 databases of four --traffic-pass
+    runs under DIR (pmc_fetch/, pmc_write/, pmc_valu/, pmc_mem/, each with the
+    pass's JSON output as pmc_NAME.json beside it): calibrated HBM bytes and
+    SQ instruction counts per launch, per ray and per prism step of the
+    work-queue trace kernel (the learned evaluation's launches, the ones
+    before the plain evaluation's first importance launch) and per launch of
+    the other kernels."""
+    import sqlite3
+
+    base = Path(args.counters)
+
+    def rows(name):
+        db = sqlite3.connect(str(base / f"pmc_{name}" / f"{name}_results.db"))
+        return db.execute("select kernel_name, counter_name, value, dispatch_id, start, end "
+                          "from counters_collection").fetchall()
+
+    def short(k):
+        import re
+
+        for ident, targs in re.findall(r"([A-Za-z_]\w*)(<[^()]*>)?\(", k):
+            if ident != "anonymous":
+                return ident + (targs or "")
+        return k
+
+    passes = {p: rows(p) for p in ("fetch", "write", "valu", "mem")}
+    stats = json.loads((base / "pmc_valu.json").read_text())
+    learned = stats["learned"]
+    out = {"calibration": {}, "kernels": {}}
+    per = {}
+    for p, rs in passes.items():
+        # the learned evaluation's trace launches come first (its launch count
+        # is in the pass's statistics); the plain evaluation's follow
+        starts = sorted({(r[4], r[3]) for r in rs if short(r[0]) == STAGED_TRACE})
+        plain_disp = {d for _, d in starts[learned["trace_launches"]:]}
+        agg = {}
+        for name, ctr, val, disp, start, end in rs:
+            k = short(name)
+            if k == STAGED_TRACE and disp in plain_disp:
+                k += " (plain, first level)"
+            a = agg.setdefault(k, {"ctr": {}, "disp": {}})
+            a["ctr"][ctr] = a["ctr"].get(ctr, 0.0) + val
+            a["disp"][disp] = end - start
+        per[p] = agg
+    copy = per["fetch"].get("__amd_rocclr_copyBuffer")
+    fetch_scale = stats["copy_bytes"] / (copy["ctr"]["FETCH_SIZE"] * 1024.0)
+    wcopy = per["write"].get("__amd_rocclr_copyBuffer")
+    write_scale = (stats["copy_bytes"] + stats["fill_bytes"]) / (
+        (wcopy["ctr"]["WRITE_SIZE"] + per["write"]["__amd_rocclr_fillBufferAligned"]["ctr"]["WRITE_SIZE"]) * 1024.0)
+    out["calibration"] = {"FETCH_SIZE": fetch_scale, "WRITE_SIZE": write_scale,
+                          "rule": "bytes moved by the runtime's 1 GiB copy (and fill) over the counter's KiB"}
+    for k in per["valu"]:
+        if k.startswith("__amd"):
+            continue
+        v = per["valu"][k]
+        n = len(v["disp"])
+        dur = sum(v["disp"].values()) * 1e-9
+        c = dict(v["ctr"])
+        for p in ("fetch", "write", "mem"):
+            c.update(per[p].get(k, {"ctr": {}})["ctr"])
+        e = {"launches": n, "seconds": dur,
+             "hbm_read_bytes_per_launch": c.get("FETCH_SIZE", 0.0) * 1024.0 * fetch_scale / n,
+             "hbm_write_bytes_per_launch": c.get("WRITE_SIZE", 0.0) * 1024.0 * write_scale / n,
+             "valu_wave_instructions_per_launch": c["SQ_INSTS_VALU"] / n,
+             "salu_wave_instructions_per_launch": c["SQ_INSTS_SALU"] / n,
+             "lds_instructions_per_launch": c.get("SQ_INSTS_LDS", 0.0) / n,
+             "vmem_read_instructions_per_launch": c.get("SQ_INSTS_VMEM_RD", 0.0) / n,
+             "lane_utilisation": c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU"]),
+             "wait_fraction": c["SQ_WAIT_INST_ANY"] / c["SQ_WAVE_CYCLES"],
+             "valu_issue_per_cu_cycle": c["SQ_INSTS_VALU"] / (dur * CUS * CLOCK_HZ)}
+        e["hbm_bytes_per_launch"] = e["hbm_read_bytes_per_launch"] + e["hbm_write_bytes_per_launch"]
+        if k == STAGED_TRACE:
+            rays, steps = learned["rays"], learned["ray_steps"]
+            e.update({"rays": rays, "prism_steps": steps, "hbm_bytes_per_ray": e["hbm_bytes_per_launch"] * n / rays,
+                      "valu_per_step": c["SQ_INSTS_VALU"] * 64.0 / steps,
+                      "active_valu_per_step": c["SQ_THREAD_CYCLES_VALU"] / steps,
+                      "salu_per_step": c["SQ_INSTS_SALU"] * 64.0 / steps})
+        out["kernels"][k] = e
+    out["command"] = ("rocprofv3 --pmc <counters> -d DIR/pmc_NAME -o NAME -- python3 bench.py --traffic-pass "
+                      "--load-proposal NET > DIR/pmc_NAME.json, for FETCH_SIZE, WRITE_SIZE and two SQ sets")
+    (ROOT / "profiles" / "traffic.json").write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out, indent=1))
 
 
 def main():
@@ -621,6 +712,8 @@ def main():
                          "1e5 -> 1e8 rays), one share of its sample points, STEPS depletion steps, both forms")
     ap.add_argument("--share", type=int, default=1024,
                     help="sample points of the --combined share (>= 315300: all of them, no warm evaluation)")
+    ap.add_argument("--counters", default=None, metavar="DIR",
+                    help="reduce the rocprofv3 counter runs under DIR into profiles/traffic.json")
     ap.add_argument("--traffic-pass", action="store_true",
                     help="counter pass: one learned evaluation, one plain evaluation of the first level's "
                          "samples and a 1 GiB device copy (the HBM byte-counter calibration)")
@@ -649,6 +742,9 @@ def main():
     w = production_disk(levels=args.levels)
     if args.traffic_pass:
         traffic_pass(w, args)
+        return
+    if args.counters:
+        reduce_counters(args)
         return
     topo = w.topology
     log(f"workload: {topo.numberOfPoints} points, {topo.numberOfTriangles} triangles, {topo.levels} levels, "
@@ -721,24 +817,35 @@ def main():
     kernels = results[head]["kernels"]
     dom = kernels[DOMINANT] if DOMINANT in kernels else kernels["hase_trace_kernel"]
     achieved = dom["flops_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e12
-    traffic, counters = None, {}
+    rays_per_launch = dom["rays"] / max(dom["launches"], 1)
+    steps_per_launch = dom["prism_steps"] / max(dom["launches"], 1)
+    traffic, issue = None, None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
-        stored = json.loads(tf.read_text())
-        traffic = stored.get(DOMINANT)
-        counters = {k: v for k, v in stored.items() if k.startswith(DOMINANT + "_")}
+        stored = json.loads(tf.read_text()).get("kernels", {}).get(STAGED_TRACE)
+        if stored:
+            # stored per-ray bytes and per-step instructions, scaled to this run's launches
+            traffic = stored["hbm_bytes_per_ray"] * rays_per_launch
+            wave_insts = stored["valu_per_step"] / 64.0 * steps_per_launch
+            issue = {"valu_wave_instructions_per_launch": wave_insts,
+                     "valu_per_step": stored["valu_per_step"],
+                     "active_valu_per_step": stored["active_valu_per_step"],
+                     "lane_utilisation": stored["lane_utilisation"],
+                     "issue_per_cu_cycle": wave_insts / (dom["avg_ms"] * 1e-3 * CUS * CLOCK_HZ),
+                     "issue_peak_per_cu_cycle": 1.0,
+                     "counter_run_issue_per_cu_cycle": stored["valu_issue_per_cu_cycle"]}
     roofline = {
         "bound": "valu", "kernel": DOMINANT, "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
         "work": f"algorithmic FP32: {FLOPS_PER_STEP} flops per prism step + {FLOPS_PER_RAY} per ray, "
-                f"{dom['prism_steps'] / max(dom['launches'], 1):.4g} steps and "
-                f"{dom['rays'] / max(dom['launches'], 1):.4g} rays per launch (device-counted)",
+                f"{steps_per_launch:.4g} steps and {rays_per_launch:.4g} rays per launch (device-counted)",
         "avg_launch_ms": dom["avg_ms"],
-        "traffic_source": "stored: rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel at the committed code, "
-                          "calibrated by a 1 GiB copy (profiles/traffic.json); not measured in this run",
+        "valu_issue": issue,
+        "traffic_source": "stored counters (profiles/traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per ray, "
+                          "calibrated by a 1 GiB copy; SQ_INSTS_VALU per prism step) scaled to this run's launches; "
+                          "not measured in this run",
         "hbm_frac": (traffic / (dom["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
     }
-    roofline.update(counters)
     out = {
         "metric": METRIC,
         "value": hl["rays_per_s"],
