@@ -2,7 +2,8 @@
 
 * Production disk, whole (4072 triangles, 20 levels, 77,368 prisms, 42,240
   sample points, 64 wavelengths, first-order reflections, cladding ring,
-  adaptive 1e5 -> 1e7 rays per sample, mse <= 0.005): about 200 samples
+  adaptive 1e5 -> 1e7 rays per sample, mse <= 0.005): six samples against
+  the FP64 oracle at 1e9 rays per sample (a committed fixture), and about 200 samples
   stratified over levels 0, 9 and 19 - the rim points of the cladding ring,
   the points near the disk centre and random others - are compared with the
   FP64 oracle run on the same settings with an independent seed.  Every form
@@ -19,6 +20,8 @@ Acceptance, statistical form of "every phi within 3 sigma":
 z = (phi_gpu - phi_oracle) / sqrt(SE_gpu^2 + SE_oracle^2); at most 2% of the
 checked samples beyond 3 (0.27% expected by chance), none beyond 5, and
 |mean z| < 0.3 (a bias of a third of a standard error would fail)."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -29,6 +32,8 @@ from haseongpu_amd.workloads import batch_problems, production_disk
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
 def _z(phi_a, se_a, phi_b, se_b):
@@ -93,6 +98,19 @@ def test_production_disk_whole_against_the_oracle():
     phi_o, se_o, ref = _oracle_at(w, picks, seed=424242)
     assert np.all(ref["droppedRays"][picks] == 0)
     assert np.all(ref["mse"][picks] <= 0.005)
+    # the high-budget reference: the FP64 oracle at 1e9 rays per sample on the
+    # rim and centre points of levels 0, 9 and 19 (tests/golden/make_high_budget.py)
+    hb = np.load(GOLDEN / "production_disk_1e9.npz")
+    r = np.hypot(w.topology.points[:, 0], w.topology.points[:, 1])
+    Np = w.topology.numberOfPoints
+    hb_picks = np.array([p + lv * Np for lv in (0, 9, 19) for p in (int(np.argmax(r)), int(np.argmin(r)))])
+    assert np.array_equal(hb["samples"], hb_picks), "the fixture belongs to another production disk"
+    hb_se = np.sqrt(hb["mse"]) * hb["phiAse"]
+    for name in ("walked map", "learned"):
+        res = forms[name]
+        zh = _z(res.phiAse[hb_picks].astype(np.float64), res.standardError[hb_picks], hb["phiAse"], hb_se)
+        print(f"production disk, {name} against the 1e9-ray oracle: z {np.array2string(zh, precision=2)}", flush=True)
+        assert np.max(np.abs(zh)) < 4 and np.sum(np.abs(zh) > 3) <= 1, (name, zh)
     cap = w.settings["maxRaysPerSample"]
     for name, res in forms.items():
         # every sample converged, or (beta*V sampling, no map) stopped at the ray cap
