@@ -115,11 +115,13 @@ def host_description():
 
 
 def cpu_baseline(w, spectra, seconds=15.0):
-    """The FP64 oracle (a port of the estimator, OpenMP) on this host's cores,
-    over the stratified sample points (levels 0, 9 and 19; rim, centre and
-    others), in blocks until `seconds` of CPU time are spent, same settings
-    and an independent seed.  Its estimates also serve as the FP64 reference
-    the GPU estimates are checked against."""
+    """The oracle's C restatement of the estimator built with FP32 ray
+    geometry and FP64 sums (OpenMP, this host's cores), timed over the
+    stratified sample points (levels 0, 9 and 19; rim, centre and others) in
+    blocks until `seconds` of wall time are spent, same settings and an
+    independent seed.  Each block is then rerun, untimed, by the FP64 oracle
+    on the same streams: those estimates are the FP64 reference the GPU
+    estimates are checked against."""
     from oracle import oracle as O
 
     s = w.settings
@@ -131,7 +133,7 @@ def cpu_baseline(w, spectra, seconds=15.0):
     rng = np.random.default_rng(7)
     order = rng.permutation(picks.size)
     block = max(threads, 8)
-    done_rays, elapsed, at = 0, 0.0, 0
+    done_rays, elapsed, at, geometry_dev = 0, 0.0, 0, 0.0
     idx, phi, mse = [], [], []
     while elapsed < seconds and at < picks.size:
         chunk = picks[order[at:at + block]]
@@ -143,9 +145,11 @@ def cpu_baseline(w, spectra, seconds=15.0):
                             adaptive_steps=s["adaptiveSteps"], seed=424242, threads=threads,
                             n_samples=w.samples, samples=chunk)
         t0 = time.perf_counter()
-        r = O.calc_phi_ase(om, op)
+        r32 = O.calc_phi_ase(om, op, geometry="f32")
         elapsed += time.perf_counter() - t0
-        done_rays += int(r["totalRays"][chunk].astype(np.int64).sum())
+        done_rays += int(r32["totalRays"][chunk].astype(np.int64).sum())
+        r = O.calc_phi_ase(om, op)
+        geometry_dev = max(geometry_dev, float(np.max(np.abs(r32["phiAse"][chunk] / r["phiAse"][chunk] - 1))))
         idx.append(chunk)
         phi.append(r["phiAse"][chunk])
         mse.append(r["mse"][chunk])
@@ -154,9 +158,10 @@ def cpu_baseline(w, spectra, seconds=15.0):
     n = int(sum(c.size for c in idx))
     return {"value": done_rays / elapsed, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{n} of the {picks.size} stratified sample points (levels 0, 9, 19: rim, centre, others) of "
-                      f"the same workload, full importance map + adaptive rays, FP64, {elapsed:.1f}s on "
-                      f"{threads} threads",
+                      f"the same workload, full importance map + adaptive rays, FP32 ray geometry and FP64 "
+                      f"sums, {elapsed:.1f}s on {threads} threads",
             "samples_per_s": n / elapsed, "host": host_description(),
+            "fp32_geometry_vs_fp64_max_rel": geometry_dev,
             "_oracle": (np.concatenate(idx), np.concatenate(phi), np.concatenate(mse))}
 
 
@@ -232,6 +237,37 @@ def batch_bench(args):
     for w in ws:
         mesh, exp, comp, _ = native_problem(w)
         ctxs.append(_native.Context(mesh, exp, comp))
+    net, train_s = None, 0.0
+    if args.batch_learned:
+        # one network for every problem: fitted on transfer-map rows of random
+        # pump fields on eight of the batch's meshes, smallest to largest
+        # (never on the problems' own beta fields)
+        from haseongpu_amd import proposal as prop
+
+        t0 = time.perf_counter()
+        order = np.argsort([w.topology.numberOfTriangles for w in ws])
+        picks = order[np.linspace(0, len(ws) - 1, min(8, len(ws))).astype(int)]
+        rng = np.random.default_rng(11)
+        Xs, Ys = [], []
+        for i in picks:
+            w = ws[i]
+            mesh, exp, comp, log_sigma = native_problem(w)
+            clad = np.asarray(w.medium.value("claddingCellTypes")) == int(w.medium.value("claddingNumber"))
+            with _native.Context(mesh, exp, comp) as c:
+                for _ in range(2):
+                    beta = prop.random_pump_beta(w.topology, rng, cladding=clad)
+                    c.update_beta(beta)
+                    X, Y = prop.training_rows(c, w.topology, beta, log_sigma=log_sigma, rng=rng, samples=48,
+                                              rows_per_sample=1024)
+                    Xs.append(X)
+                    Ys.append(Y)
+        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+        net = prop.fit(np.concatenate(Xs), np.concatenate(Ys), epochs=12, batch=4096, seed=1,
+                       cluster_triangles=ct, cluster_levels=cl)
+        train_s = time.perf_counter() - t0
+        for c in ctxs:
+            c.set_proposal(net)
+        log(f"network fitted on {sum(x.shape[0] for x in Xs)} rows of {len(picks)} problems in {train_s:.1f}s")
     tuning = ctxs[0].autotune(512)
     for c in ctxs[1:]:
         c.set_tuning(**{k: v for k, v in tuning.items() if k != "probe_ms"})
@@ -257,6 +293,7 @@ def batch_bench(args):
     tb, rb = together()
     log(f"together: {tb:.2f}s")
     st = batch.stats()
+    fallback = sum(int(c.proposal_stats()["fallback_samples"]) for c in ctxs) if net else 0
     rays_total = sum(int(r.totalRays.astype(np.int64).sum()) for r in rb)
     same = all(np.array_equal(a.totalRays, b.totalRays) for a, b in zip(ra, rb))
     batch.close()
@@ -269,7 +306,9 @@ def batch_bench(args):
            "all_converged": bool(all(np.all(r.mse <= 0.005) for r in rb)),
            "samples_at_ray_cap": int(sum(np.count_nonzero(~(r.mse <= 0.005) & (r.totalRays >= rays[1])) for r in rb)),
            "importance_ms": st["importance_ms"],
-           "trace_ms": st["trace_ms"], "tuning": tuning}
+           "trace_ms": st["trace_ms"], "tuning": tuning,
+           "proposal": "learned (one network for all problems, walked-map fallback)" if net else "walked map",
+           "training_s": train_s if net else None, "fallback_samples": fallback}
     print(json.dumps(out), flush=True)
 
 
@@ -696,6 +735,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batching measurement instead of the bench line: N independent problems evaluated one "
                          "after another and together in one work queue")
+    ap.add_argument("--batch-learned", action="store_true",
+                    help="with --batch: every problem uses one learned proposal (walked-map fallback)")
     ap.add_argument("--batch-triangles", default="200,2000")
     ap.add_argument("--batch-levels", default="5,20")
     ap.add_argument("--batch-rays", default="10000,10000000")

@@ -34,6 +34,18 @@
 #include <omp.h>
 #endif
 
+/* Ray geometry type of the walk.  The oracle proper is FP64 throughout; the
+ * build with ORC_GEOMETRY_FP32 (oracle/build/libhase_oracle_f32.so) walks in
+ * FP32 with FP64 sums, the CPU restatement the benchmark's CPU baseline is
+ * timed on. */
+#ifdef ORC_GEOMETRY_FP32
+typedef float orc_g;
+#define ORC_SQRT sqrtf
+#else
+typedef double orc_g;
+#define ORC_SQRT sqrt
+#endif
+
 #define ORC_MAX_STEPS 10000u
 #define ORC_KEY_TAG 0x48415345u /* key word 1 of every stream */
 #define ORC_TAG_RAY 0u
@@ -143,46 +155,49 @@ int orc_walk_ray(const orc_mesh* m, const double a[3], uint32_t prism, const dou
     }
     uint32_t t = prism % T;
     int64_t u = (int64_t)(prism / T);
-    const double h = m->h;
+    const orc_g h = (orc_g)m->h;
     /* heights in layer units; an end point on a level plane sits on it exactly */
-    double za = a[2] / h, ze = e[2] / h, zr = nearbyint(ze);
-    if (fabs(ze - zr) < 1e-5) ze = zr;
-    double dx = e[0] - a[0], dy = e[1] - a[1], dzeta = ze - za, dz = dzeta * h;
-    double D = sqrt(dx * dx + dy * dy + dz * dz);
+    const orc_g ax = (orc_g)a[0], ay = (orc_g)a[1], ex0 = (orc_g)e[0], ey0 = (orc_g)e[1];
+    orc_g za = (orc_g)a[2] / h, ze = (orc_g)e[2] / h, zr = (orc_g)nearbyint((double)ze);
+    if (fabs((double)(ze - zr)) < 1e-5) ze = zr;
+    orc_g dx = ex0 - ax, dy = ey0 - ay, dzeta = ze - za, dz = dzeta * h;
+    orc_g D = ORC_SQRT(dx * dx + dy * dy + dz * dz);
     w->length = D;
     if (!(D > 0.0)) return 0;
-    double vx = dx / D, vy = dy / D, vz = dz / D;
-    double sin2 = 1.0 - vz * vz;
-    if (sin2 < 0.0) sin2 = 0.0;
-    double tol = 1e-10 * (1.0 + D);
+    orc_g vx = dx / D, vy = dy / D, vz = dz / D;
+    orc_g sin2 = 1 - vz * vz;
+    if (sin2 < 0) sin2 = 0;
+    const orc_g tol = (orc_g)(sizeof(orc_g) == 4 ? 2e-5 : 1e-10) * (1 + D);
     int entry = -1;
-    double s = 0.0;
+    orc_g s = 0;
     uint32_t applied = 0;
     for (uint32_t step = 0; step < ORC_MAX_STEPS; ++step) {
         w->steps++;
-        double best = INFINITY;
+        orc_g best = INFINITY;
         int bestk = -1;
         for (int k = 0; k < 3; ++k) {
             if (k == entry) continue;
             uint32_t i0 = m->tri[t + (size_t)k * T];
             uint32_t i1 = m->tri[t + (size_t)((k + 1) % 3) * T];
-            double ex = m->px[i1] - m->px[i0], ey = m->py[i1] - m->py[i0];
-            double nx = ey, ny = -ex;
-            double nv = nx * vx + ny * vy;
+            const orc_g x0 = (orc_g)m->px[i0], y0 = (orc_g)m->py[i0];
+            const orc_g x1 = (orc_g)m->px[i1], y1 = (orc_g)m->py[i1];
+            orc_g ex = x1 - x0, ey = y1 - y0;
+            orc_g nx = ey, ny = -ex;
+            orc_g nv = nx * vx + ny * vy;
             if (!(nv > 0.0)) continue;
             /* an edge through the end point's (x, y) meets the ray exactly at the end */
-            int touches = (m->px[i0] == e[0] && m->py[i0] == e[1]) || (m->px[i1] == e[0] && m->py[i1] == e[1]);
-            double sk = touches ? D : (nx * (m->px[i0] - a[0]) + ny * (m->py[i0] - a[1])) / nv;
+            int touches = (x0 == ex0 && y0 == ey0) || (x1 == ex0 && y1 == ey0);
+            orc_g sk = touches ? D : (nx * (x0 - ax) + ny * (y0 - ay)) / nv;
             if (sk < best) { best = sk; bestk = k; }
         }
-        double sz = INFINITY;
+        orc_g sz = INFINITY;
         int dir = 0;
-        if (dzeta > 0.0) { sz = D * (((double)(u + 1) - za) / dzeta); dir = 1; }
-        else if (dzeta < 0.0) { sz = D * (((double)u - za) / dzeta); dir = -1; }
-        double snext = best < sz ? best : sz;
+        if (dzeta > 0) { sz = D * (((orc_g)(u + 1) - za) / dzeta); dir = 1; }
+        else if (dzeta < 0) { sz = D * (((orc_g)u - za) / dzeta); dir = -1; }
+        orc_g snext = best < sz ? best : sz;
         if (D < snext) snext = D;
         if (snext < s) snext = s;
-        double seg = snext - s;
+        orc_g seg = snext - s;
         uint32_t level = orc_real_level(u, layers);
         int cladding = m->clad && m->clad[t] == m->clad_number;
         if (cladding) {

@@ -15,6 +15,8 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "build" / "libhase_oracle.so"
+# the same restatement built with FP32 ray geometry and FP64 sums (CPU baseline)
+LIB_F32_PATH = HERE / "build" / "libhase_oracle_f32.so"
 
 _u32p = C.POINTER(C.c_uint32)
 _i32p = C.POINTER(C.c_int32)
@@ -54,6 +56,7 @@ class _Params(C.Structure):
 
 
 _lib = None
+_lib_f32 = None
 
 
 def build():
@@ -61,12 +64,25 @@ def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
-def lib():
-    global _lib
+def lib(geometry="f64"):
+    """The oracle library: FP64 throughout (default), or with FP32 ray
+    geometry (`geometry="f32"`)."""
+    global _lib, _lib_f32
+    if geometry == "f32":
+        if _lib_f32 is None:
+            if not LIB_F32_PATH.exists():
+                build()
+            _lib_f32 = _declare(C.CDLL(str(LIB_F32_PATH)))
+        return _lib_f32
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        _lib = _declare(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def _declare(L):
+    if True:
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.orc_u32_to_unit.argtypes = [C.c_uint32]
         L.orc_u32_to_unit.restype = C.c_double
@@ -89,8 +105,7 @@ def lib():
         L.orc_calc_phi_ase.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params), _f64p, _f64p, _u32p,
                                        _f64p, _u32p, _f64p, _f64p]
         L.orc_calc_phi_ase.restype = C.c_int
-        _lib = L
-    return _lib
+    return L
 
 
 def _p(arr, ctype):
@@ -220,7 +235,7 @@ def systematic_offset(seed, sample, launch):
     return lib().orc_systematic_offset(seed, sample, launch)
 
 
-def calc_phi_ase(mesh, params):
+def calc_phi_ase(mesh, params, geometry="f64"):
     S = mesh.np_ * mesh.nl
     phi = np.zeros(S)
     mse = np.zeros(S)
@@ -229,7 +244,8 @@ def calc_phi_ase(mesh, params):
     dropped = np.zeros(S, dtype=np.uint32)
     s1 = np.zeros(S)
     s2 = np.zeros(S)
-    rc = lib().orc_calc_phi_ase(C.byref(mesh.s), C.byref(params.s), _p(phi, C.c_double), _p(mse, C.c_double),
+    rc = lib(geometry).orc_calc_phi_ase(C.byref(mesh.s), C.byref(params.s), _p(phi, C.c_double),
+                                         _p(mse, C.c_double),
                                 _p(rays, C.c_uint32), _p(dndt, C.c_double), _p(dropped, C.c_uint32),
                                 _p(s1, C.c_double), _p(s2, C.c_double))
     if rc != 0:

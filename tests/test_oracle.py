@@ -426,3 +426,29 @@ def test_high_budget_fixture_matches_the_oracle_at_a_small_budget():
     z = (ref["phiAse"][smp] - hb["phiAse"]) / np.sqrt(ref["mse"][smp] * ref["phiAse"][smp] ** 2
                                                      + hb["mse"] * hb["phiAse"] ** 2)
     assert np.all(np.abs(z) < 4), z
+
+
+def test_fp32_geometry_build_stays_within_a_tenth_of_a_standard_error():
+    """The CPU baseline's build (oracle/build/libhase_oracle_f32.so: FP32 ray
+    geometry, FP64 sums) against the FP64 oracle on identical streams, eight
+    production-disk samples at 20,000 rays with reflections and the cladding
+    ring: the builds differ (the FP32 walk is really loaded), and no estimate
+    moves by more than a tenth of its standard error."""
+    from haseongpu_amd.workloads import production_disk
+
+    w = production_disk()
+    s = w.settings
+    sa, se, msa, mse_ = w.spectra.spectralPairs(False)
+    om = O.OracleMesh(w.topology, w.medium)
+    picks = np.sort(np.random.default_rng(1).choice(w.samples, 8, replace=False)).astype(np.int64)
+    op = O.OracleParams(sigma_a=sa, sigma_e=se, max_sigma_a=msa, max_sigma_e=mse_,
+                        n_tot=float(w.medium.value("nTot")), tau=float(w.medium.value("crystalTFluo")),
+                        beta_cells=w.medium.value("betaCells"), min_rays=20000, max_rays=20000,
+                        mse_threshold=0.0, use_reflections=True, max_reflections=s["maxReflections"],
+                        adaptive_steps=0, seed=99, n_samples=w.samples, samples=picks)
+    r64 = O.calc_phi_ase(om, op)
+    r32 = O.calc_phi_ase(om, op, geometry="f32")
+    a, b = r64["phiAse"][picks], r32["phiAse"][picks]
+    assert np.all(r32["droppedRays"][picks] == 0) and np.all(r32["totalRays"][picks] == 20000)
+    assert np.any(a != b)
+    assert np.all(np.abs(b / a - 1) < 0.1 * np.sqrt(r64["mse"][picks])), (b / a - 1, np.sqrt(r64["mse"][picks]))
