@@ -160,18 +160,18 @@ __device__ __forceinline__ void mixture_prefix(double local, uint32_t n, uint32_
 #pragma unroll
         for (int i = 0; i < kScanPer; ++i) buf[scan_slot(tid * kScanPer + i)] = excl + v[i];
         if (bucket) {
-            // entry e takes the buckets k with prefix(e - 1) <= k / B < prefix(e)
-            const double B = (double)kPrefixBuckets, inv_b = 1.0 / B;
-            double lo = excl;
+            // entry e takes the buckets k with prefix(e - 1) <= k / B < prefix(e),
+            // i.e. k in [ceil(prefix(e - 1) B), ceil(prefix(e) B)): B is a power of
+            // two, so the products are exact and one ceil per entry suffices
+            const double B = (double)kPrefixBuckets;
+            uint32_t k0 = (uint32_t)fmin(ceil(excl * B), B);
 #pragma unroll
             for (int i = 0; i < kScanPer; ++i) {
                 const uint32_t e = base + (uint32_t)(tid * kScanPer + i);
-                const double hi = excl + v[i];
-                if (e < n && hi > lo) {
-                    for (uint32_t k = (uint32_t)ceil(lo * B); k < kPrefixBuckets && (double)k * inv_b < hi; ++k)
-                        bucket[k] = e;
-                }
-                lo = hi;
+                const uint32_t k1 = (uint32_t)fmin(ceil((excl + v[i]) * B), B);
+                if (e < n)
+                    for (uint32_t k = k0; k < k1; ++k) bucket[k] = e;
+                k0 = k1;
             }
         }
         __syncthreads();
@@ -359,16 +359,36 @@ struct LearnedParams {
     uint32_t* buckets = nullptr;   // [slots * kPrefixBuckets] or null
 };
 
+// An entry's slice and layer change only every T entries of a thread's
+// ascending run, so the slice's image plane, the layer's row of cluster
+// logits and the height difference are kept per thread and recomputed on a
+// change; per entry remain the cluster column, the logit, the centroid and
+// beta * V.  The reciprocal is the hardware one (1 ulp).
+struct LearnedRow {
+    uint32_t k = ~0u, l = ~0u;
+    float zi = 0.0f, dz = 0.0f;
+    const float* row = nullptr;
+};
+
 __device__ __forceinline__ float learned_importance(const LearnedParams& p, const SampleGeom& sp, const float* lg,
-                                                    const EntryIndex& x, float zi)
+                                                    const EntryIndex& x, LearnedRow& r)
 {
     const MeshDev& m = p.m;
-    const uint32_t c = p.cl_lev[x.l] + p.cl_tri[x.t];
-    const float y = fminf(lg[(size_t)x.k * p.C + c], 80.0f);
+    if (x.k != r.k) {
+        r.k = x.k;
+        r.zi = (float)image_plane(sp.plane, (int)m.layers, (x.k + 1u) >> 1, x.k & 1u);
+        r.l = ~0u;
+    }
+    if (x.l != r.l) {
+        r.l = x.l;
+        r.row = lg + (size_t)x.k * p.C + p.cl_lev[x.l];
+        r.dz = ((float)x.l + 0.5f - r.zi) * m.h;
+    }
+    const float y = fminf(r.row[p.cl_tri[x.t]], 80.0f);
     const float2 cxy = p.centroid[x.t];
-    const float dx = cxy.x - sp.px, dy = cxy.y - sp.py, dz = ((float)x.l + 0.5f - zi) * m.h;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    float v = m.bv[x.j] * __expf(y) / (4.0f * kPi * d2);
+    const float dx = cxy.x - sp.px, dy = cxy.y - sp.py;
+    const float d2 = fmaf(dx, dx, fmaf(dy, dy, r.dz * r.dz));
+    const float v = m.bv[x.j] * __expf(y) * __builtin_amdgcn_rcpf(4.0f * kPi * d2);
     return (v >= 0.0f && !isinf(v)) ? v : 0.0f;
 }
 
@@ -381,10 +401,8 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
     const uint32_t n = p.R * P;
     const SampleGeom sp = sample_point(p.pts, m.np, p.sample_base + slot);
     const float* lg = p.logits + (size_t)slot * p.R * p.C;
-    auto value = [&](const EntryIndex& x, uint32_t) {
-        const float zi = (float)image_plane(sp.plane, (int)m.layers, (x.k + 1u) >> 1, x.k & 1u);
-        return learned_importance(p, sp, lg, x, zi);
-    };
+    LearnedRow row;
+    auto value = [&](const EntryIndex& x, uint32_t) { return learned_importance(p, sp, lg, x, row); };
     __shared__ double red[kBlock / 64 + 2];
     __shared__ double buf[kScanLds];
     double local = 0.0;
@@ -394,6 +412,7 @@ __global__ void __launch_bounds__(kBlock) hase_learned_importance_kernel(Learned
         local += (double)value(cur.x, e);
         cur.advance((uint32_t)kBlock);
     }
+    row = LearnedRow{};
     mixture_prefix(local, n, P, T, m.bv, p.eps, p.Bsum, p.R, value, p.prefix + (size_t)slot * n, p.Q + slot, red,
                    buf, p.buckets ? p.buckets + (size_t)slot * kPrefixBuckets : nullptr);
 }
