@@ -1509,6 +1509,10 @@ struct hase_batch {
 
 namespace {
 
+// Samples from which a problem of a batch whose records fit in LDS runs on
+// its own queue (one resident batch of sample points).
+constexpr uint32_t kSoloSamples = 8192;
+
 void batch_evaluate(hase_batch* b, hase_result_t* const* results)
 {
     if (!results) throw HaseError(HASE_E_INVALID, "results must not be NULL");
@@ -1524,6 +1528,34 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
     check_tuning(tu);
     b->stats = hase_stats_t{};
     b->generation++;
+    // A problem whose triangle records fit in LDS and whose samples fill a
+    // resident batch of their own (kSoloSamples) runs on its own queue, as
+    // when evaluated alone: its trace kernel then reads the records from LDS,
+    // which the shared queue (items of several problems in one work-group)
+    // cannot.  The shared queue takes the rest, the problems too small to
+    // fill the chip by themselves, where one queue for all pays.
+    std::vector<char> solo(n, 0);
+    hase_stats_t solo_stats{};
+    for (size_t p = 0; p < n; ++p) {
+        hase_context* ctx = b->ctx[p];
+        const hase_tuning_t& ct = ctx->tuning;
+        const uint32_t count = ctx->comp.max_sample_range - ctx->comp.min_sample_range + 1u;
+        solo[p] = ctx->lds_rec.p != nullptr && ct.mode == HASE_TRACE_QUEUE && ct.stage_lds && ct.refill_lanes == 0u &&
+                  count >= kSoloSamples;
+        if (!solo[p]) continue;
+        evaluate(ctx, results[p]);
+        const hase_stats_t& st = ctx->stats;
+        solo_stats.importance_ms += st.importance_ms + ctx->pstats.inference_ms + ctx->pstats.proposal_ms;
+        solo_stats.trace_ms += st.trace_ms;
+        solo_stats.finalize_ms += st.finalize_ms;
+        solo_stats.rays += st.rays;
+        solo_stats.ray_steps += st.ray_steps;
+        solo_stats.importance_walks += st.importance_walks;
+        solo_stats.importance_steps += st.importance_steps;
+        solo_stats.dropped_rays += st.dropped_rays;
+        solo_stats.trace_launches += st.trace_launches;
+        solo_stats.importance_launches += st.importance_launches;
+    }
     // Batch memory per problem: what it would take evaluated alone (all its
     // samples, at most 8192 within 6 GiB), scaled down evenly when the
     // problems together would need more than half the free device memory
@@ -1538,6 +1570,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         size_t want_sum = 0;
         for (size_t p = 0; p < n; ++p) {
             const hase_context* ctx = b->ctx[p];
+            if (solo[p]) continue;
             const size_t count = std::min<size_t>(8192, ctx->comp.max_sample_range - ctx->comp.min_sample_range + 1u);
             const bool shared = ctx->proposal_kind == HASE_PROPOSAL_BETA_VOLUME;
             unit[p] = (shared ? 0u : (size_t)slices(ctx) * ctx->P * 12u + kPrefixBuckets * 4u) +
@@ -1558,6 +1591,11 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
     std::vector<BatchTimers> tm(n);
     for (size_t p = 0; p < n; ++p) {
         hase_context* ctx = b->ctx[p];
+        if (solo[p]) {   // evaluated above
+            next[p] = ctx->comp.max_sample_range + 1u;
+            last[p] = ctx->comp.max_sample_range;
+            continue;
+        }
         hase_result_t* r = results[p];
         for (uint32_t s = 0; s < ctx->S; ++s) {
             r->phi_ase[s] = 0.0f;
@@ -1716,6 +1754,7 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
     b->stats.trace_ms = queue_ms;
     for (size_t p = 0; p < n; ++p) {
         hase_context* ctx = b->ctx[p];
+        if (solo[p]) continue;
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (auto& pr : tm[p].imp) b->stats.importance_ms += elapsed(pr.first, pr.second);
         for (auto& pr : tm[p].trace) b->stats.trace_ms += elapsed(pr.first, pr.second);
@@ -1735,6 +1774,16 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
             for (uint32_t s = ctx->comp.min_sample_range; s <= ctx->comp.max_sample_range; ++s) dr += results[p]->dropped_rays[s];
         b->stats.dropped_rays += dr;
     }
+    b->stats.importance_ms += solo_stats.importance_ms;
+    b->stats.trace_ms += solo_stats.trace_ms;
+    b->stats.finalize_ms += solo_stats.finalize_ms;
+    b->stats.rays += solo_stats.rays;
+    b->stats.ray_steps += solo_stats.ray_steps;
+    b->stats.importance_walks += solo_stats.importance_walks;
+    b->stats.importance_steps += solo_stats.importance_steps;
+    b->stats.dropped_rays += solo_stats.dropped_rays;
+    b->stats.trace_launches += solo_stats.trace_launches;
+    b->stats.importance_launches += solo_stats.importance_launches;
 }
 
 // Autotuner of the persistent work queue: coordinate descent over the

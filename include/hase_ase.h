@@ -361,7 +361,12 @@ int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_
  * round (uneven budgets, different meshes, spectra and schedules), so small
  * problems no longer leave the GPU idle.  Each context keeps its proposal
  * (walked or learned) and its outputs; ray scheduling follows the first
- * context's tuning.  Contexts must be on one device, stay owned by the
+ * context's tuning.  A context whose triangle records fit in LDS (at most
+ * 4800 triangles), whose own tuning stages them there, and which has at
+ * least 8192 sample points is evaluated on its own queue first, as
+ * hase_context_evaluate would (its samples fill the chip alone and its
+ * trace kernel reads the records from LDS); the shared queue takes the
+ * others.  Contexts must be on one device, stay owned by the
  * caller and outlive the batch.  No counterpart in the reference (one
  * problem per calcPhiASE run, include/core/simulation.hpp:42-333). */
 typedef struct hase_batch hase_batch_t;

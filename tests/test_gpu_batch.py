@@ -59,3 +59,25 @@ def test_batch_with_an_empty_problem_and_sample_ranges():
     assert np.all(together[1].phiAse == 0) and np.all(together[1].totalRays == 0)
     for c in ctxs:
         c.close()
+
+
+def test_batch_runs_large_lds_problems_on_their_own_queue():
+    """A problem of at least 8192 sample points whose records fit in LDS runs
+    on its own queue inside the batch; results and ray counts are still
+    those of each problem alone, and the batch statistics count its rays."""
+    ws = batch_problems(3, seed=3, triangles=(200, 500), levels=(4, 8), wavelengths=(1, 4), min_rays=2000,
+                        max_rays=16000, mse_threshold=2e-3, adaptive_steps=2)
+    disk = production_disk(levels=4, wavelengths=4, min_rays=2048, max_rays=8192, mse_threshold=2e-3)
+    assert disk.samples >= 8192 and disk.topology.numberOfTriangles <= 4800
+    ctxs = _contexts([ws[0], disk] + ws[1:])
+    ctxs[1].set_tuning(stage_lds=1)
+    alone = [c.evaluate() for c in ctxs]
+    rays_alone = sum(int(c.stats()["rays"]) for c in ctxs)
+    with _native.Batch(ctxs) as batch:
+        together = batch.evaluate()
+        st = batch.stats()
+    for a, b in zip(alone, together):
+        _same(a, b)
+    assert st["rays"] == rays_alone
+    for c in ctxs:
+        c.close()
