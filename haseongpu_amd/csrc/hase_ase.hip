@@ -269,14 +269,15 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
 // mixture prefix afterwards (not for transfer maps).
 constexpr int kStagedBlock = 1024;
 
+template <uint32_t kKind>
 __global__ void __launch_bounds__(kStagedBlock) hase_importance_staged_kernel(ImportanceParams p)
 {
-    __shared__ uint4 s_tri[2u * kLdsTriangles];
+    __shared__ uint4 s_tri[kLdsStageWords];
     const MeshDev& m = p.m;
     const uint32_t T = m.T, P = T * m.layers;
-    for (uint32_t i = threadIdx.x; i < 2u * T; i += blockDim.x) s_tri[i] = m.lds_rec[i];
+    for (uint32_t i = threadIdx.x; i < m.lds_words; i += blockDim.x) s_tri[i] = m.lds_rec[i];
     __syncthreads();
-    const LdsTriangles fetch{s_tri};
+    const auto fetch = staged_fetch<kKind>(s_tri, T);
     const size_t n = (size_t)p.R * P;
     const ColumnItems ci(T, m.layers, p.R);
     const uint32_t per_slot = ci.per_sample();
@@ -912,6 +913,30 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
             lrec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
         }
         ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
+        ctx->lds_kind = hase::kLdsCompact;
+    } else if (T < (1u << 19) && np <= 0x10000u && (size_t)T + (np + 1u) / 2u <= hase::kLdsStageWords) {
+        // indexed records (point indices + 21-bit neighbour fields + cladding
+        // bit) followed by the vertex table
+        std::vector<uint4> lrec((size_t)T + (np + 1u) / 2u, make_uint4(0u, 0u, 0u, 0u));
+        for (uint32_t td = 0; td < T; ++td) {
+            const uint4 b = rec[3u * td + 1u], c = rec[3u * td + 2u];
+            const uint32_t nwk[3] = {b.z, b.w, c.x};
+            uint64_t W = 0;
+            for (int k = 0; k < 3; ++k) {
+                const uint64_t f = nw_boundary(nwk[k]) ? 0x1FFFFFull
+                                                       : (uint64_t)(nw_next(nwk[k]) << 2 | nw_edge(nwk[k]));
+                W |= f << (21 * k);
+            }
+            const uint32_t clad = c.y >> 31;
+            W |= (uint64_t)clad << 63;
+            lrec[td] = make_uint4((c.y & 0x7FFFFFFFu) | c.z << 16, c.w | clad << 31, (uint32_t)W, (uint32_t)(W >> 32));
+        }
+        float2* vtx = reinterpret_cast<float2*>(lrec.data() + T);
+        for (uint32_t i = 0; i < np; ++i) vtx[i] = make_float2((float)px[i], (float)py[i]);
+        ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
+        ctx->lds_kind = hase::kLdsIndexed;
+    }
+    if (ctx->lds_kind) {
         // staged walks run one 1024-thread work-group per CU
         ctx->tuning.stage_lds = 1u;
         ctx->tuning.block_size = 1024u;
@@ -950,6 +975,8 @@ MeshDev mesh_view(const hase_context* ctx)
     MeshDev m;
     m.rec = ctx->rec.p;
     m.lds_rec = ctx->lds_rec.p;
+    m.lds_kind = ctx->lds_rec.p ? ctx->lds_kind : 0u;
+    m.lds_words = (uint32_t)ctx->lds_rec.n;
     m.beta = ctx->beta.p;
     m.beta_col = ctx->beta_col.p;
     m.bv = ctx->bv.p;
@@ -1095,8 +1122,12 @@ void launch_importance(hase_context* ctx, ImportanceParams& ip, uint32_t slots)
         int cus = 0;
         HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         ip.parts = 1u;
-        hipLaunchKernelGGL(hase_importance_staged_kernel, dim3((uint32_t)std::max(1, cus)), dim3(kStagedBlock), 0,
-                           ctx->stream, ip);
+        if (ctx->lds_kind == kLdsIndexed)
+            hipLaunchKernelGGL(hase_importance_staged_kernel<kLdsIndexed>, dim3((uint32_t)std::max(1, cus)),
+                               dim3(kStagedBlock), 0, ctx->stream, ip);
+        else
+            hipLaunchKernelGGL(hase_importance_staged_kernel<kLdsCompact>, dim3((uint32_t)std::max(1, cus)),
+                               dim3(kStagedBlock), 0, ctx->stream, ip);
         HIP_CHECK(hipGetLastError());
         if (!ip.transfer) {
             hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);

@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #define HASE_HD __host__ __device__
 
@@ -36,11 +37,21 @@ HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float
 //   {x0, y0, x1, y1}  {x2, y2, W & 0xFFFFFFFF, W >> 32}
 // with W = f0 | f1 << 21 | f2 << 42 | cladding << 63 and f_k = neighbour << 2 |
 // shared edge (21 bits: neighbour < 2^19) or 0x1FFFFF at the boundary.
-constexpr uint32_t kLdsTriangles = 4800;   // 150 KiB of LDS
+// Indexed records, for meshes beyond kLdsTriangles: 16 B per triangle,
+//   {v0 | v1 << 16, v2 | cladding << 31, W & 0xFFFFFFFF, W >> 32}
+// with v_k the point index of vertex k (< 2^16) and W as above, followed by
+// the vertex table, the FP32 (x, y) of every mesh point, two per 16 bytes.
+// One more LDS read per new vertex than the compact records, but a Delaunay
+// disk (about T / 2 points) fits up to ~8,100 triangles instead of 4800.
+constexpr uint32_t kLdsTriangles = 4800;     // compact records: 150 KiB of LDS
+constexpr uint32_t kLdsStageWords = 10176;   // 16-byte words of LDS a staged kernel holds (159 KiB)
+constexpr uint32_t kLdsCompact = 1u, kLdsIndexed = 2u;
 
 struct MeshDev {
     const uint4* rec;      // 3 records per triangle
-    const uint4* lds_rec;  // 2 compact records per triangle, or null when the mesh exceeds kLdsTriangles
+    const uint4* lds_rec;  // records for LDS staging (kind lds_kind), or null when the mesh does not fit
+    uint32_t lds_kind;     // 0: none, kLdsCompact, kLdsIndexed
+    uint32_t lds_words;    // 16-byte words to stage
     const float* beta;     // [P] level-major: t + l * T, zero in cladding prisms
     const float* beta_col; // [P] column-major: t * layers + l, zero in cladding prisms
     const float* bv;       // [P] beta * prism volume, level-major
@@ -232,6 +243,52 @@ struct LdsTriangles {
         return vertex & ((int)(ox != ex) | (int)(oy != ey));
     }
 };
+
+// From the 16-byte indexed records and the vertex table staged in LDS
+// (vtx = the table, right after the T records); words, decoders and the end
+// test are those of the compact records.
+struct LdsIndexedTriangles : LdsTriangles {
+    const float2* vtx;
+    HASE_HD uint32_t point(uint32_t t, uint32_t v) const
+    {
+        const uint2 q = reinterpret_cast<const uint2*>(rec)[2u * t];
+        const uint32_t i01 = v == 1u ? q.x >> 16 : q.x & 0xFFFFu;
+        return v == 2u ? (q.y & 0xFFFFu) : i01;
+    }
+    HASE_HD void operator()(const MeshDev&, uint32_t t, uint4& a, uint4& b, uint4& c) const
+    {
+        const uint4 q = rec[t];
+        const float2 p0 = vtx[q.x & 0xFFFFu], p1 = vtx[q.x >> 16], p2 = vtx[q.y & 0xFFFFu];
+        a = make_uint4(__builtin_bit_cast(uint32_t, p0.x), __builtin_bit_cast(uint32_t, p0.y),
+                       __builtin_bit_cast(uint32_t, p1.x), __builtin_bit_cast(uint32_t, p1.y));
+        b = make_uint4(__builtin_bit_cast(uint32_t, p2.x), __builtin_bit_cast(uint32_t, p2.y), q.z & 0x1FFFFFu,
+                       ((q.z >> 21) | (q.w << 11)) & 0x1FFFFFu);
+        c = make_uint4((q.w >> 10) & 0x1FFFFFu, q.w & 0x80000000u, 0u, 0u);
+    }
+    HASE_HD float2 vertex(const MeshDev&, uint32_t t, uint32_t v) const { return vtx[point(t, v)]; }
+    HASE_HD void words(const MeshDev&, uint32_t t, uint32_t e1, uint32_t e2, uint32_t& w1, uint32_t& w2,
+                       uint32_t& clad) const
+    {
+        const uint64_t W = reinterpret_cast<const uint64_t*>(rec)[2u * t + 1u];
+        w1 = (uint32_t)(W >> (21u * e1)) & 0x1FFFFFu;
+        w2 = (uint32_t)(W >> (21u * e2)) & 0x1FFFFFu;
+        clad = (uint32_t)(W >> 63);
+    }
+};
+
+// Fetch of records staged in LDS at s (kind kLdsCompact or kLdsIndexed: T
+// records, then the indexed format's vertex table).
+template <uint32_t kKind>
+using StagedFetch = std::conditional_t<kKind == kLdsIndexed, LdsIndexedTriangles, LdsTriangles>;
+
+template <uint32_t kKind>
+HASE_HD inline StagedFetch<kKind> staged_fetch(const uint4* s, uint32_t T)
+{
+    if constexpr (kKind == kLdsIndexed)
+        return LdsIndexedTriangles{{s}, reinterpret_cast<const float2*>(s + T)};
+    else
+        return LdsTriangles{s};
+}
 
 // Exit edge among the candidates s_k (edge k joins corners k and k+1): its
 // parameter, the neighbour word behind it, and whether it ends at the end

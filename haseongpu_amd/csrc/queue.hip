@@ -183,18 +183,19 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
 //
 // kTable: per-problem constants from the device table (batches of several
 // problems); otherwise from the kernel argument `one` (a single problem), so
-// they sit in scalar registers.  kLds (single problem, mesh of at most
-// kLdsTriangles triangles): the work-group first copies the compact triangle
-// records into LDS and the walks read them there; beta stays in global memory.
-template <bool kRefill, bool kTable, bool kLds>
+// they sit in scalar registers.  kLds (single problem whose records fit in
+// LDS: kLdsCompact or kLdsIndexed, 0 for none): the work-group first copies
+// the staged triangle records into LDS and the walks read them there; beta
+// stays in global memory.
+template <bool kRefill, bool kTable, uint32_t kLds>
 __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q, ProblemDev one)
 {
     const uint32_t n_active = q.scalars[kNActive];
     if (n_active == 0u) return;
     const uint4* lds_rec = nullptr;
-    if constexpr (kLds) {
-        __shared__ uint4 s_tri[2u * kLdsTriangles];
-        const uint32_t n = 2u * one.m.T;
+    if constexpr (kLds != 0u) {
+        __shared__ uint4 s_tri[kLdsStageWords];
+        const uint32_t n = one.m.lds_words;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_tri[i] = one.m.lds_rec[i];
         __syncthreads();
         lds_rec = s_tri;
@@ -262,8 +263,8 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                     double qprob;
                     uint32_t j;
                     WalkOut w;
-                    if constexpr (kLds) {
-                        const LdsTriangles fetch{lds_rec};
+                    if constexpr (kLds != 0u) {
+                        const auto fetch = staged_fetch<kLds>(lds_rec, m.T);
                         const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
                         w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
                                      r.fp, fetch);
@@ -502,10 +503,11 @@ float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const Q
     const bool refill = tu.refill_lanes != 0u;
     const bool table = single == nullptr;
     // LDS staging: single problem, no lane refill, mesh small enough
-    const bool lds = tu.stage_lds && !table && !refill && single->m.lds_rec != nullptr;
-    auto kernel = lds ? hase_queue_trace_kernel<false, false, true>
-                : refill ? (table ? hase_queue_trace_kernel<true, true, false> : hase_queue_trace_kernel<true, false, false>)
-                         : (table ? hase_queue_trace_kernel<false, true, false> : hase_queue_trace_kernel<false, false, false>);
+    const uint32_t lds = tu.stage_lds && !table && !refill && single->m.lds_rec != nullptr ? single->m.lds_kind : 0u;
+    auto kernel = lds == kLdsIndexed ? hase_queue_trace_kernel<false, false, kLdsIndexed>
+                : lds == kLdsCompact ? hase_queue_trace_kernel<false, false, kLdsCompact>
+                : refill ? (table ? hase_queue_trace_kernel<true, true, 0u> : hase_queue_trace_kernel<true, false, 0u>)
+                         : (table ? hase_queue_trace_kernel<false, true, 0u> : hase_queue_trace_kernel<false, false, 0u>);
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)tu.block_size, 0));
     ProblemDev one;
     std::memset(&one, 0, sizeof(one));

@@ -138,3 +138,31 @@ def test_results_do_not_depend_on_the_batch_size(learned):
                 assert ctx.proposal_stats()["fallback_samples"] > 0
     assert w.samples > 97 * 20
     _same(*out)
+
+
+def test_indexed_lds_records_of_a_larger_mesh():
+    """A 6720-triangle disk (beyond the 4800 triangles of the compact LDS
+    records, within the indexed ones: 16 B per triangle plus the vertex
+    table): the staged trace walks the same rays as the global-record trace,
+    bit for bit, and the staged importance map matches the FP64 oracle."""
+    from haseongpu_amd.workloads import batch_problems
+    from tests.test_gpu_parity import _f32_exact
+
+    w = _f32_exact(batch_problems(1, seed=5, triangles=(6000, 7000), levels=(4, 4), wavelengths=(2, 2),
+                                  min_rays=1000, max_rays=4000)[0])
+    T, Np = w.topology.numberOfTriangles, w.topology.numberOfPoints
+    assert 4800 < T and T + (Np + 1) // 2 <= 10176
+    ctx, spectra = _context(w, min_rays=2048, max_rays=8192, mse_threshold=1e-3, adaptive_steps=2, rng_seed=6,
+                            min_sample_range=0, max_sample_range=2999)
+    om, op = _oracle(w, spectra, min_rays=1)
+    with ctx:
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=0, block_size=512)
+        glob = ctx.evaluate()
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=1, block_size=1024)
+        lds = ctx.evaluate()
+        np.testing.assert_array_equal(lds.totalRays, glob.totalRays)
+        np.testing.assert_array_equal(lds.phiAse, glob.phiAse)
+        for s in np.random.default_rng(8).integers(0, 3000, 3):
+            g = ctx.importance(int(s)).astype(np.float64)
+            c = O.importance(om, op, int(s))
+            np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6 * c.max())

@@ -37,9 +37,9 @@ def walk_lib(tmp_path_factory):
                     "-o", str(out)], check=True, capture_output=True)
     lib = C.CDLL(str(out))
     u32p, f32p, i32p = C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
-    lib.walk_rays_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, f32p, f32p, f32p, C.c_uint32,
-                                   f32p, u32p, f32p, u32p, u32p, f32p, u32p, i32p]
-    lib.importance_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, f32p, f32p, f32p, f32p,
+    lib.walk_rays_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, C.c_int, f32p, f32p, f32p,
+                                   C.c_uint32, f32p, u32p, f32p, u32p, u32p, f32p, u32p, i32p]
+    lib.importance_host.argtypes = [C.c_uint32, C.c_uint32, C.c_float, u32p, u32p, C.c_int, f32p, f32p, f32p, f32p,
                                     C.c_float, f32p, f32p, C.c_uint32, C.c_float, C.c_float, C.c_int, C.c_uint32,
                                     C.c_int, f32p, C.POINTER(C.c_ulonglong)]
     return lib
@@ -52,9 +52,15 @@ def _f32_exact(w):
     return Workload(w.name, topo, GainMedium(topo, dict(w.medium.physical)), w.spectra, w.settings)
 
 
+LAYOUTS = {"global": 0, "lds": 1, "indexed": 2}
+
+
 def _records(w):
-    """(48-byte records as uint32 [12T], 32-byte records [8T] or None, beta FP32)
-    in the caller's triangle order, packed as the library packs them."""
+    """(48-byte records as uint32 [12T], {layout: staged records as uint32},
+    beta FP32) in the caller's triangle order, packed as the library packs
+    them: "lds" the 32-byte compact records [8T], "indexed" the 16-byte
+    indexed records [4T] followed by the FP32 vertex table (two points per
+    16 bytes)."""
     topo, med = w.topology, w.medium
     T, L = topo.numberOfTriangles, int(topo.levels)
     d = topo.topology()
@@ -82,9 +88,18 @@ def _records(w):
     W |= clad.astype(np.uint64) << np.uint64(63)
     lds[:, 6] = (W & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     lds[:, 7] = (W >> np.uint64(32)).astype(np.uint32)
+    Np = topo.numberOfPoints
+    assert Np <= 0x10000
+    idx = np.zeros((T + (Np + 1) // 2, 4), np.uint32)
+    idx[:T, 0] = tri[:, 0].astype(np.uint32) | (tri[:, 1].astype(np.uint32) << np.uint32(16))
+    idx[:T, 1] = tri[:, 2].astype(np.uint32) | (clad << np.uint32(31))
+    idx[:T, 2], idx[:T, 3] = lds[:, 6], lds[:, 7]
+    vtx = np.zeros(2 * ((Np + 1) // 2) * 2, np.float32)
+    vtx[:2 * Np] = xy.reshape(-1)
+    idx[T:] = vtx.view(np.uint32).reshape(-1, 4)
     beta = med.value("betaVolume").astype(np.float32).reshape(L - 1, T)
     beta[:, clad == 1] = 0.0
-    return rec.reshape(-1), lds.reshape(-1), beta.reshape(-1)
+    return rec.reshape(-1), {"lds": lds.reshape(-1), "indexed": idx.reshape(-1)}, beta.reshape(-1)
 
 
 def _rays(w, om, n, seed, reflections):
@@ -119,10 +134,12 @@ def _walk(lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, staged):
     refl = np.ascontiguousarray(med.value("reflectivities"), np.float32)
     nidx = np.ascontiguousarray(med.value("refractiveIndices"), np.float32)
     P = lambda a, t: np.ascontiguousarray(a, dtype=t).ctypes.data_as(C.POINTER(np.ctypeslib.as_ctypes_type(t)))  # noqa
-    keep = [np.ascontiguousarray(a) for a in (rec, lds, beta, refl, nidx)]
+    staged_rec = lds.get(staged) if staged != "global" else None
+    keep = [np.ascontiguousarray(a) if a is not None else None for a in (rec, staged_rec, beta, refl, nidx)]
     lib.walk_rays_host(T, L - 1, np.float32(h), keep[0].ctypes.data_as(C.POINTER(C.c_uint32)),
-                       keep[1].ctypes.data_as(C.POINTER(C.c_uint32)) if staged else None,
-                       keep[2].ctypes.data_as(C.POINTER(C.c_float)), keep[3].ctypes.data_as(C.POINTER(C.c_float)),
+                       keep[1].ctypes.data_as(C.POINTER(C.c_uint32)) if keep[1] is not None else None,
+                       LAYOUTS[staged], keep[2].ctypes.data_as(C.POINTER(C.c_float)),
+                       keep[3].ctypes.data_as(C.POINTER(C.c_float)),
                        keep[4].ctypes.data_as(C.POINTER(C.c_float)), n,
                        hs.ctypes.data_as(C.POINTER(C.c_float)),
                        np.ascontiguousarray(prisms, np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -147,7 +164,7 @@ def _case(name):
     return w, 3
 
 
-@pytest.mark.parametrize("staged", [False, True], ids=["global", "lds"])
+@pytest.mark.parametrize("staged", ["global", "lds", "indexed"])
 @pytest.mark.parametrize("name", ["plain_small", "true_cladding", "disk_reflect"])
 def test_host_walk_matches_the_oracle(walk_lib, name, staged):
     w, refl = _case(name)
@@ -177,6 +194,22 @@ def test_host_walk_matches_the_oracle(walk_lib, name, staged):
     assert same_steps >= 0.99 * len(prisms) and worst <= 3
 
 
+@pytest.mark.parametrize("name", ["true_cladding", "disk_reflect"])
+def test_staged_layouts_walk_bit_for_bit_alike(walk_lib, name):
+    """The same rays over the three record layouts (global 48-byte, compact
+    and indexed LDS records) give the same FP32 results and step counts bit
+    for bit: the layouts hold the same FP32 corners and neighbour links."""
+    w, refl = _case(name)
+    w = _f32_exact(w)
+    om = O.OracleMesh(w.topology, w.medium)
+    rec, lds, beta = _records(w)
+    starts, prisms, ends, nr, fp = _rays(w, om, 2000, seed=5, reflections=refl)
+    outs = {k: _walk(walk_lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, k) for k in LAYOUTS}
+    for k in ("lds", "indexed"):
+        assert np.array_equal(outs[k][0], outs["global"][0]), k
+        assert np.array_equal(outs[k][1], outs["global"][1]) and np.array_equal(outs[k][2], outs["global"][2]), k
+
+
 def test_grazing_reflections_with_equal_indices_never_reflect_totally(walk_lib):
     """Refractive indices 1 on both sides of both planes (the production
     disk's setting): a ray reflected at a plane it runs nearly parallel to
@@ -200,7 +233,7 @@ def test_grazing_reflections_with_equal_indices_never_reflect_totally(walk_lib):
     sp = topo.samplePoints()[rng.integers(0, Np, n)]    # level 0
     nr, fp = np.ones(n, np.int64), np.zeros(n, np.int64)
     ends = np.array([O.image_point(om, sp[i], 1, 0) for i in range(n)]).astype(np.float32).astype(np.float64)
-    for staged in (False, True):
+    for staged in ("global", "lds", "indexed"):
         out, _, status = _walk(walk_lib, w, rec, lds, beta, starts, prisms, ends, nr, fp, staged)
         for i in range(n):
             o = O.walk_ray(om, starts[i], int(prisms[i]), ends[i], 1, 0)
@@ -211,7 +244,7 @@ def test_grazing_reflections_with_equal_indices_never_reflect_totally(walk_lib):
             assert out[i, 3] < 1.0
 
 
-@pytest.mark.parametrize("staged", [False, True], ids=["global", "lds"])
+@pytest.mark.parametrize("staged", ["global", "lds", "indexed"])
 @pytest.mark.parametrize("name", ["plain_small", "disk_reflect_2"])
 def test_host_importance_matches_the_oracle(walk_lib, name, staged):
     """The importance kernel's column walks (one xy side-test walk per
@@ -247,7 +280,8 @@ def test_host_importance_matches_the_oracle(walk_lib, name, staged):
     refl = np.ascontiguousarray(med.value("reflectivities"), np.float32) if reflect else None
     nidx = np.ascontiguousarray(med.value("refractiveIndices"), np.float32) if reflect else None
     pts = topo.points.astype(np.float32)
-    keep = [np.ascontiguousarray(a) for a in (rec, lds, beta, bv, c1, c2)]
+    staged_rec = lds.get(staged) if staged != "global" else np.zeros(4, np.uint32)
+    keep = [np.ascontiguousarray(a) for a in (rec, staged_rec, beta, bv, c1, c2)]
     ptr = lambda a, t: a.ctypes.data_as(C.POINTER(t)) if a is not None else None   # noqa: E731
     rng = np.random.default_rng(3)
     for s in rng.integers(0, w.samples, 5):
@@ -255,7 +289,8 @@ def test_host_importance_matches_the_oracle(walk_lib, name, staged):
         out = np.zeros(R * T * (L - 1), np.float32)
         steps = C.c_ulonglong(0)
         walk_lib.importance_host(T, L - 1, np.float32(h), ptr(keep[0], C.c_uint32),
-                                 ptr(keep[1], C.c_uint32) if staged else None, ptr(keep[2], C.c_float),
+                                 ptr(keep[1], C.c_uint32) if staged != "global" else None, LAYOUTS[staged],
+                                 ptr(keep[2], C.c_float),
                                  ptr(keep[3], C.c_float), ptr(refl, C.c_float), ptr(nidx, C.c_float),
                                  np.float32(med.value("claddingAbsorption")), ptr(keep[4], C.c_float),
                                  ptr(keep[5], C.c_float), len(c1), pts[p, 0], pts[p, 1], lv, R, 0,

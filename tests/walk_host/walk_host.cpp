@@ -1,7 +1,8 @@
 // Host build of the device prism walk (haseongpu_amd/csrc/prism_walk.h) and
 // the importance map's column walks (importance_walk.h) for CPU tests: the same source, compiled for the host, walked over meshes the
-// test packs into the device record layouts (48-byte global records and the
-// 32-byte records staged in LDS).  Test infrastructure only.
+// test packs into the device record layouts (48-byte global records, and the
+// 32-byte compact or 16-byte indexed records staged in LDS; lds_kind 0, 1
+// or 2).  Test infrastructure only.
 #include <algorithm>
 #include <vector>
 
@@ -11,7 +12,7 @@
 using namespace hase;
 
 extern "C" int walk_rays_host(uint32_t T, uint32_t layers, float h, const uint32_t* rec, const uint32_t* lds_rec,
-                              const float* beta, const float* refl, const float* nidx, uint32_t n,
+                              int lds_kind, const float* beta, const float* refl, const float* nidx, uint32_t n,
                               const float* start, const uint32_t* prism, const float* end, const uint32_t* nrefl,
                               const uint32_t* first, float* out, uint32_t* steps, int32_t* status)
 {
@@ -29,9 +30,13 @@ extern "C" int walk_rays_host(uint32_t T, uint32_t layers, float h, const uint32
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t t = prism[i] % T, l = prism[i] / T;
         WalkOut w;
-        if (lds_rec)
+        const uint4* staged = reinterpret_cast<const uint4*>(lds_rec);
+        if (lds_kind == (int)kLdsCompact)
             w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
-                         end[3 * i + 2], nrefl[i], first[i], LdsTriangles{reinterpret_cast<const uint4*>(lds_rec)});
+                         end[3 * i + 2], nrefl[i], first[i], staged_fetch<kLdsCompact>(staged, T));
+        else if (lds_kind == (int)kLdsIndexed)
+            w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
+                         end[3 * i + 2], nrefl[i], first[i], staged_fetch<kLdsIndexed>(staged, T));
         else
             w = walk_ray(m, start[3 * i], start[3 * i + 1], start[3 * i + 2], t, l, end[3 * i], end[3 * i + 1],
                          end[3 * i + 2], nrefl[i], first[i]);
@@ -49,7 +54,7 @@ extern "C" int walk_rays_host(uint32_t T, uint32_t layers, float h, const uint32
 // Walked importance map of one sample (every reflection slice and prism, in
 // the caller's prism order) by the importance kernel's column walks.
 extern "C" int importance_host(uint32_t T, uint32_t layers, float h, const uint32_t* rec, const uint32_t* lds_rec,
-                               const float* beta, const float* bv, const float* refl, const float* nidx, float clad_alpha,
+                               int lds_kind, const float* beta, const float* bv, const float* refl, const float* nidx, float clad_alpha,
                                const float* c1, const float* c2, uint32_t n_lambda, float px, float py, int plane,
                                uint32_t R, int transfer, float* out, unsigned long long* steps)
 {
@@ -79,9 +84,13 @@ extern "C" int importance_host(uint32_t T, uint32_t layers, float h, const uint3
         int lv0, nlev;
         ci.decode(it, t0, k, lv0, nlev);
         float v[kLevelGroup];
-        if (lds_rec)
-            *steps += column_entries(m, LdsTriangles{reinterpret_cast<const uint4*>(lds_rec)}, c1, c2, n_lambda,
-                                     1.0f / (float)n_lambda, t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
+        const uint4* staged = reinterpret_cast<const uint4*>(lds_rec);
+        if (lds_kind == (int)kLdsCompact)
+            *steps += column_entries(m, staged_fetch<kLdsCompact>(staged, T), c1, c2, n_lambda, 1.0f / (float)n_lambda,
+                                     t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
+        else if (lds_kind == (int)kLdsIndexed)
+            *steps += column_entries(m, staged_fetch<kLdsIndexed>(staged, T), c1, c2, n_lambda, 1.0f / (float)n_lambda,
+                                     t0, lv0, nlev, k, px, py, plane, transfer != 0, v);
         else
             *steps += column_entries(m, GlobalTriangles{}, c1, c2, n_lambda, 1.0f / (float)n_lambda, t0, lv0, nlev, k,
                                      px, py, plane, transfer != 0, v);
