@@ -47,8 +47,9 @@ CLOCK_HZ = 2.4e9
 # library bracket plan + trace + finalize of every round (the trace kernel is
 # > 97% of a round in the kernel traces under profiles/)
 DOMINANT = "hase_queue_trace_kernel"
-# its instantiation on meshes staged in LDS (<refill lanes, problem table, LDS records>)
-STAGED_TRACE = "hase_queue_trace_kernel<false, false, true>"
+# its instantiation on meshes staged in LDS as compact records (<refill lanes,
+# problem table, LDS record kind: 1 compact, 2 indexed>)
+STAGED_TRACE = "hase_queue_trace_kernel<false, false, 1u>"
 # algorithmic cost of one prism step of the walk (rays and importance walks)
 BYTES_PER_STEP = 56
 FLOPS_PER_STEP = 40
