@@ -35,7 +35,8 @@ struct hase_context {
     std::vector<double> sigma_a, sigma_e;
     hase_compute_t comp{};
     hase::DevBuf<uint4> rec, lds_rec;
-    uint32_t lds_kind = 0;             // format of lds_rec: 0 (none), kLdsCompact or kLdsIndexed
+    uint32_t lds_kind = 0;             // format of lds_rec: 0 (none), kLdsCompact, kLdsIndexed or kCompactGlobal
+    bool lds_staged() const { return lds_kind == hase::kLdsCompact || lds_kind == hase::kLdsIndexed; }
     hase::DevBuf<float2> pts;
     hase::DevBuf<float> beta, beta_col, bv, refl, c1, c2;
     hase::DevBuf<float> imp;

@@ -209,6 +209,7 @@ struct ImportanceParams {
     uint32_t parts;           // work-groups per sample; > 1: importance only, hase_prefix_kernel follows
 };
 
+template <bool kCompact>
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
     const uint32_t row = blockIdx.x / p.parts, part = blockIdx.x % p.parts;
@@ -237,8 +238,12 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         int lv0, nlev;
         ci.decode(it, t0, k, lv0, nlev);
         float v[kLevelGroup];
-        steps += column_entries(m, GlobalTriangles{}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px,
-                                sp.py, sp.plane, p.transfer != 0, v);
+        if constexpr (kCompact)
+            steps += column_entries(m, LdsTriangles{m.lds_rec}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k,
+                                    sp.px, sp.py, sp.plane, p.transfer != 0, v);
+        else
+            steps += column_entries(m, GlobalTriangles{}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px,
+                                    sp.py, sp.plane, p.transfer != 0, v);
         walks += (unsigned long long)nlev;
 #pragma unroll
         for (int g = 0; g < kLevelGroup; ++g) {
@@ -896,8 +901,11 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
     ctx->user2dev_dev.upload(ctx->user2dev.data(), T, ctx->stream);
-    if (T <= hase::kLdsTriangles) {
-        // compact records for LDS staging: corners + 21-bit neighbour fields + cladding bit
+    const bool indexed_fits = T < (1u << 19) && np <= 0x10000u && (size_t)T + (np + 1u) / 2u <= hase::kLdsStageWords;
+    if (T <= hase::kLdsTriangles || (!indexed_fits && T < (1u << 19))) {
+        // compact records: corners + 21-bit neighbour fields + cladding bit,
+        // staged in LDS or, for meshes too large for either LDS format, read
+        // from global memory
         std::vector<uint4> lrec(2u * (size_t)T);
         for (uint32_t td = 0; td < T; ++td) {
             const uint4 a = rec[3u * td], b = rec[3u * td + 1u], c = rec[3u * td + 2u];
@@ -913,8 +921,8 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
             lrec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
         }
         ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
-        ctx->lds_kind = hase::kLdsCompact;
-    } else if (T < (1u << 19) && np <= 0x10000u && (size_t)T + (np + 1u) / 2u <= hase::kLdsStageWords) {
+        ctx->lds_kind = T <= hase::kLdsTriangles ? hase::kLdsCompact : hase::kCompactGlobal;
+    } else if (indexed_fits) {
         // indexed records (point indices + 21-bit neighbour fields + cladding
         // bit) followed by the vertex table
         std::vector<uint4> lrec((size_t)T + (np + 1u) / 2u, make_uint4(0u, 0u, 0u, 0u));
@@ -936,11 +944,13 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
         ctx->lds_kind = hase::kLdsIndexed;
     }
-    if (ctx->lds_kind) {
+    if (ctx->lds_staged()) {
         // staged walks run one 1024-thread work-group per CU
         ctx->tuning.stage_lds = 1u;
         ctx->tuning.block_size = 1024u;
         ctx->tuning.rays_per_item = 512u;
+    } else if (ctx->lds_kind == hase::kCompactGlobal) {
+        ctx->tuning.stage_lds = 1u;
     }
     ctx->centroid.upload(ctx->centroid_host.data(), T, ctx->stream);
     std::vector<float2>& pts = ctx->pts_host;
@@ -1118,7 +1128,7 @@ uint32_t importance_parts(const hase_context* ctx, uint32_t slots)
 // small batches).  ip.parts is set here.
 void launch_importance(hase_context* ctx, ImportanceParams& ip, uint32_t slots)
 {
-    if (ctx->lds_rec.p) {
+    if (ctx->lds_rec.p && ctx->lds_staged()) {
         int cus = 0;
         HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         ip.parts = 1u;
@@ -1137,7 +1147,10 @@ void launch_importance(hase_context* ctx, ImportanceParams& ip, uint32_t slots)
     }
     // a small batch would leave most of the chip idle: split each sample's walks
     ip.parts = importance_parts(ctx, slots);
-    hipLaunchKernelGGL(hase_importance_kernel, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
+    if (ctx->lds_rec.p && ctx->lds_kind == kCompactGlobal)
+        hipLaunchKernelGGL(hase_importance_kernel<true>, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
+    else
+        hipLaunchKernelGGL(hase_importance_kernel<false>, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
     HIP_CHECK(hipGetLastError());
     if (ip.parts > 1u && !ip.transfer) {
         hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
@@ -1571,7 +1584,8 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         hase_context* ctx = b->ctx[p];
         const hase_tuning_t& ct = ctx->tuning;
         const uint32_t count = ctx->comp.max_sample_range - ctx->comp.min_sample_range + 1u;
-        solo[p] = ctx->lds_rec.p != nullptr && ct.mode == HASE_TRACE_QUEUE && ct.stage_lds && ct.refill_lanes == 0u &&
+        solo[p] = ctx->lds_rec.p != nullptr && ctx->lds_staged() && ct.mode == HASE_TRACE_QUEUE && ct.stage_lds &&
+                  ct.refill_lanes == 0u &&
                   count >= kSoloSamples;
         if (!solo[p]) continue;
         evaluate(ctx, results[p]);
@@ -1869,10 +1883,11 @@ void autotune(hase_context* ctx, uint32_t probe, hase_tuning_t* best)
             }
         };
         if (ctx->lds_rec.p) {
-            // one work-group per CU holds the staged mesh: try it with the largest block
+            // one work-group per CU holds a staged mesh: try it with the largest
+            // block (compact records in global memory keep the block size)
             hase_tuning_t t = cur;
             t.stage_lds = cur.stage_lds ? 0u : 1u;
-            t.block_size = t.stage_lds ? 1024u : cur.block_size;
+            t.block_size = t.stage_lds && ctx->lds_staged() ? 1024u : cur.block_size;
             const double ms = time_it(t);
             if (ms < 0.98 * best_ms) {
                 best_ms = ms;

@@ -46,11 +46,16 @@ HASE_HD inline float bits_to_float(uint32_t u) { return __builtin_bit_cast(float
 constexpr uint32_t kLdsTriangles = 4800;     // compact records: 150 KiB of LDS
 constexpr uint32_t kLdsStageWords = 10176;   // 16-byte words of LDS a staged kernel holds (159 KiB)
 constexpr uint32_t kLdsCompact = 1u, kLdsIndexed = 2u;
+// Meshes that fit neither LDS format keep the compact records in global
+// memory: the walk reads one vertex and one neighbour word per step from
+// them (two loads, 32 B per triangle) instead of three loads from the
+// 48-byte records.
+constexpr uint32_t kCompactGlobal = 3u;
 
 struct MeshDev {
     const uint4* rec;      // 3 records per triangle
-    const uint4* lds_rec;  // records for LDS staging (kind lds_kind), or null when the mesh does not fit
-    uint32_t lds_kind;     // 0: none, kLdsCompact, kLdsIndexed
+    const uint4* lds_rec;  // records of kind lds_kind (staged in LDS, or read from global memory), or null
+    uint32_t lds_kind;     // 0: none, kLdsCompact, kLdsIndexed, kCompactGlobal
     uint32_t lds_words;    // 16-byte words to stage
     const float* beta;     // [P] level-major: t + l * T, zero in cladding prisms
     const float* beta_col; // [P] column-major: t * layers + l, zero in cladding prisms

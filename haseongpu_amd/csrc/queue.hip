@@ -186,14 +186,17 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
 // they sit in scalar registers.  kLds (single problem whose records fit in
 // LDS: kLdsCompact or kLdsIndexed, 0 for none): the work-group first copies
 // the staged triangle records into LDS and the walks read them there; beta
-// stays in global memory.
+// stays in global memory.  kCompactGlobal: the walks read the compact
+// records from global memory.
 template <bool kRefill, bool kTable, uint32_t kLds>
 __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q, ProblemDev one)
 {
     const uint32_t n_active = q.scalars[kNActive];
     if (n_active == 0u) return;
     const uint4* lds_rec = nullptr;
-    if constexpr (kLds != 0u) {
+    if constexpr (kLds == kCompactGlobal) {
+        lds_rec = one.m.lds_rec;
+    } else if constexpr (kLds != 0u) {
         __shared__ uint4 s_tri[kLdsStageWords];
         const uint32_t n = one.m.lds_words;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_tri[i] = one.m.lds_rec[i];
@@ -506,6 +509,7 @@ float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const Q
     const uint32_t lds = tu.stage_lds && !table && !refill && single->m.lds_rec != nullptr ? single->m.lds_kind : 0u;
     auto kernel = lds == kLdsIndexed ? hase_queue_trace_kernel<false, false, kLdsIndexed>
                 : lds == kLdsCompact ? hase_queue_trace_kernel<false, false, kLdsCompact>
+                : lds == kCompactGlobal ? hase_queue_trace_kernel<false, false, kCompactGlobal>
                 : refill ? (table ? hase_queue_trace_kernel<true, true, 0u> : hase_queue_trace_kernel<true, false, 0u>)
                          : (table ? hase_queue_trace_kernel<false, true, 0u> : hase_queue_trace_kernel<false, false, 0u>);
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)tu.block_size, 0));

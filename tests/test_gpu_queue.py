@@ -166,3 +166,31 @@ def test_indexed_lds_records_of_a_larger_mesh():
             g = ctx.importance(int(s)).astype(np.float64)
             c = O.importance(om, op, int(s))
             np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6 * c.max())
+
+
+def test_compact_records_in_global_memory_of_a_mesh_beyond_lds():
+    """A 10,027-triangle disk (beyond both LDS formats): its walks read the
+    32-byte compact records from global memory (stage_lds 1) or the 48-byte
+    records (stage_lds 0); the same rays bit for bit, and the importance map
+    from the compact records matches the FP64 oracle."""
+    from haseongpu_amd.workloads import batch_problems
+    from tests.test_gpu_parity import _f32_exact
+
+    w = _f32_exact(batch_problems(1, seed=7, triangles=(9500, 10500), levels=(3, 3), wavelengths=(2, 2),
+                                  min_rays=1000, max_rays=4000)[0])
+    T, Np = w.topology.numberOfTriangles, w.topology.numberOfPoints
+    assert T + (Np + 1) // 2 > 10176
+    ctx, spectra = _context(w, min_rays=2048, max_rays=8192, mse_threshold=1e-3, adaptive_steps=2, rng_seed=8,
+                            min_sample_range=0, max_sample_range=2999)
+    om, op = _oracle(w, spectra, min_rays=1)
+    with ctx:
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=0, block_size=512)
+        wide = ctx.evaluate()
+        ctx.set_tuning(mode=_native.TRACE_QUEUE, use_graph=1, stage_lds=1, block_size=512)
+        compact = ctx.evaluate()
+        np.testing.assert_array_equal(compact.totalRays, wide.totalRays)
+        np.testing.assert_array_equal(compact.phiAse, wide.phiAse)
+        for s in np.random.default_rng(9).integers(0, 3000, 3):
+            g = ctx.importance(int(s)).astype(np.float64)
+            c = O.importance(om, op, int(s))
+            np.testing.assert_allclose(g, c, rtol=1e-4, atol=1e-6 * c.max())
