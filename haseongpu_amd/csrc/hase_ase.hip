@@ -209,7 +209,9 @@ struct ImportanceParams {
     uint32_t parts;           // work-groups per sample; > 1: importance only, hase_prefix_kernel follows
 };
 
-template <bool kCompact>
+// kKind: the records the walks read from global memory: 0 the 48-byte
+// records, kCompactGlobal the 32-byte compact records.
+template <uint32_t kKind>
 __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParams p)
 {
     const uint32_t row = blockIdx.x / p.parts, part = blockIdx.x % p.parts;
@@ -238,9 +240,9 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
         int lv0, nlev;
         ci.decode(it, t0, k, lv0, nlev);
         float v[kLevelGroup];
-        if constexpr (kCompact)
-            steps += column_entries(m, LdsTriangles{m.lds_rec}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k,
-                                    sp.px, sp.py, sp.plane, p.transfer != 0, v);
+        if constexpr (kKind != 0u)
+            steps += column_entries(m, staged_fetch<kKind>(m.lds_rec, T), p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0,
+                                    nlev, k, sp.px, sp.py, sp.plane, p.transfer != 0, v);
         else
             steps += column_entries(m, GlobalTriangles{}, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px,
                                     sp.py, sp.plane, p.transfer != 0, v);
@@ -1148,9 +1150,10 @@ void launch_importance(hase_context* ctx, ImportanceParams& ip, uint32_t slots)
     // a small batch would leave most of the chip idle: split each sample's walks
     ip.parts = importance_parts(ctx, slots);
     if (ctx->lds_rec.p && ctx->lds_kind == kCompactGlobal)
-        hipLaunchKernelGGL(hase_importance_kernel<true>, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
+        hipLaunchKernelGGL(hase_importance_kernel<kCompactGlobal>, dim3(slots * ip.parts), dim3(kBlock), 0,
+                           ctx->stream, ip);
     else
-        hipLaunchKernelGGL(hase_importance_kernel<false>, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
+        hipLaunchKernelGGL(hase_importance_kernel<0u>, dim3(slots * ip.parts), dim3(kBlock), 0, ctx->stream, ip);
     HIP_CHECK(hipGetLastError());
     if (ip.parts > 1u && !ip.transfer) {
         hipLaunchKernelGGL(hase_prefix_kernel, dim3(slots), dim3(kBlock), 0, ctx->stream, ip);
