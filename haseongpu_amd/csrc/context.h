@@ -35,6 +35,7 @@ struct hase_context {
     std::vector<double> sigma_a, sigma_e;
     hase_compute_t comp{};
     hase::DevBuf<uint4> rec, lds_rec;
+    hase::DevBuf<uint4> cmp_rec;       // compact records beside indexed ones (kind kLdsIndexed)
     uint32_t lds_kind = 0;             // format of lds_rec: 0 (none), kLdsCompact, kLdsIndexed or kCompactGlobal
     bool lds_staged() const { return lds_kind == hase::kLdsCompact || lds_kind == hase::kLdsIndexed; }
     hase::DevBuf<float2> pts;

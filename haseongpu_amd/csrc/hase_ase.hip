@@ -904,11 +904,12 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
     ctx->user2dev_dev.upload(ctx->user2dev.data(), T, ctx->stream);
     const bool indexed_fits = T < (1u << 19) && np <= 0x10000u && (size_t)T + (np + 1u) / 2u <= hase::kLdsStageWords;
-    if (T <= hase::kLdsTriangles || (!indexed_fits && T < (1u << 19))) {
-        // compact records: corners + 21-bit neighbour fields + cladding bit,
-        // staged in LDS or, for meshes too large for either LDS format, read
-        // from global memory
-        std::vector<uint4> lrec(2u * (size_t)T);
+    // compact records (corners + 21-bit neighbour fields + cladding bit) of
+    // every mesh below 2^19 triangles: staged in LDS up to kLdsTriangles,
+    // else read from global memory by the walks that do not stage
+    std::vector<uint4> crec;
+    if (T < (1u << 19)) {
+        crec.resize(2u * (size_t)T);
         for (uint32_t td = 0; td < T; ++td) {
             const uint4 a = rec[3u * td], b = rec[3u * td + 1u], c = rec[3u * td + 2u];
             const uint32_t nwk[3] = {b.z, b.w, c.x};
@@ -919,32 +920,28 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
                 W |= f << (21 * k);
             }
             W |= (uint64_t)(c.y >> 31) << 63;
-            lrec[2u * td] = a;
-            lrec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
+            crec[2u * td] = a;
+            crec[2u * td + 1u] = make_uint4(b.x, b.y, (uint32_t)W, (uint32_t)(W >> 32));
         }
-        ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
+    }
+    if (T <= hase::kLdsTriangles || (!indexed_fits && !crec.empty())) {
+        ctx->lds_rec.upload(crec.data(), crec.size(), ctx->stream);
         ctx->lds_kind = T <= hase::kLdsTriangles ? hase::kLdsCompact : hase::kCompactGlobal;
     } else if (indexed_fits) {
-        // indexed records (point indices + 21-bit neighbour fields + cladding
-        // bit) followed by the vertex table
+        // indexed records (point indices + the compact records' neighbour
+        // word) followed by the vertex table; the compact records stay in
+        // global memory beside them
         std::vector<uint4> lrec((size_t)T + (np + 1u) / 2u, make_uint4(0u, 0u, 0u, 0u));
         for (uint32_t td = 0; td < T; ++td) {
-            const uint4 b = rec[3u * td + 1u], c = rec[3u * td + 2u];
-            const uint32_t nwk[3] = {b.z, b.w, c.x};
-            uint64_t W = 0;
-            for (int k = 0; k < 3; ++k) {
-                const uint64_t f = nw_boundary(nwk[k]) ? 0x1FFFFFull
-                                                       : (uint64_t)(nw_next(nwk[k]) << 2 | nw_edge(nwk[k]));
-                W |= f << (21 * k);
-            }
+            const uint4 c = rec[3u * td + 2u], q = crec[2u * td + 1u];
             const uint32_t clad = c.y >> 31;
-            W |= (uint64_t)clad << 63;
-            lrec[td] = make_uint4((c.y & 0x7FFFFFFFu) | c.z << 16, c.w | clad << 31, (uint32_t)W, (uint32_t)(W >> 32));
+            lrec[td] = make_uint4((c.y & 0x7FFFFFFFu) | c.z << 16, c.w | clad << 31, q.z, q.w);
         }
         float2* vtx = reinterpret_cast<float2*>(lrec.data() + T);
         for (uint32_t i = 0; i < np; ++i) vtx[i] = make_float2((float)px[i], (float)py[i]);
         ctx->lds_rec.upload(lrec.data(), lrec.size(), ctx->stream);
         ctx->lds_kind = hase::kLdsIndexed;
+        ctx->cmp_rec.upload(crec.data(), crec.size(), ctx->stream);
     }
     if (ctx->lds_staged()) {
         // staged walks run one 1024-thread work-group per CU
@@ -988,6 +985,7 @@ MeshDev mesh_view(const hase_context* ctx)
     m.rec = ctx->rec.p;
     m.lds_rec = ctx->lds_rec.p;
     m.lds_kind = ctx->lds_rec.p ? ctx->lds_kind : 0u;
+    m.cmp_rec = ctx->lds_kind == kLdsIndexed ? ctx->cmp_rec.p : (ctx->lds_rec.p ? ctx->lds_rec.p : nullptr);
     m.lds_words = (uint32_t)ctx->lds_rec.n;
     m.beta = ctx->beta.p;
     m.beta_col = ctx->beta_col.p;
@@ -1750,7 +1748,10 @@ void batch_evaluate(hase_batch* b, hase_result_t* const* results)
         qs.refill = tu.refill_lanes;
         qs.budget = tu.step_budget;
         b->generation++;   // buffers may have moved: capture again
-        queue_ms += queue_run(b->stream, b->device, tu, qs, total, rounds, b->graph, b->generation);
+        bool table_compact = true;
+        for (size_t w = 0; w < part.size(); ++w) table_compact = table_compact && table[w].m.cmp_rec != nullptr;
+        queue_ms += queue_run(b->stream, b->device, tu, qs, total, rounds, b->graph, b->generation, nullptr,
+                              table_compact);
         b->stats.trace_launches += rounds;
         std::vector<float> h_phi(total);
         std::vector<double> h_mse(total), h_se(total);

@@ -56,6 +56,7 @@ struct MeshDev {
     const uint4* rec;      // 3 records per triangle
     const uint4* lds_rec;  // records of kind lds_kind (staged in LDS, or read from global memory), or null
     uint32_t lds_kind;     // 0: none, kLdsCompact, kLdsIndexed, kCompactGlobal
+    const uint4* cmp_rec;  // the compact records in global memory (meshes below 2^19 triangles), or null
     uint32_t lds_words;    // 16-byte words to stage
     const float* beta;     // [P] level-major: t + l * T, zero in cladding prisms
     const float* beta_col; // [P] column-major: t * layers + l, zero in cladding prisms

@@ -93,8 +93,10 @@ size_t queue_items_per_slot(const hase_context* ctx, const hase_tuning_t& tu);
 // Runs `rounds` adaptive rounds over `slots` global slots on `stream`
 // (the state is reset first); returns the device time in ms.  `single`
 // (a batch of one problem) passes the problem as a kernel argument instead
-// of the device table.
+// of the device table; `table_compact`: every problem of the table has its
+// compact records in global memory (the walks then read those).
 float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const QueueState& qs, uint32_t slots,
-                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single = nullptr);
+                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single = nullptr,
+                bool table_compact = false);
 
 }  // namespace hase

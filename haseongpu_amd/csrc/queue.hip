@@ -187,16 +187,15 @@ __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, ui
 // LDS: kLdsCompact or kLdsIndexed, 0 for none): the work-group first copies
 // the staged triangle records into LDS and the walks read them there; beta
 // stays in global memory.  kCompactGlobal: the walks read the compact
-// records from global memory.
+// records of the item's problem from global memory (single problems too
+// large for LDS, and the problem table of a batch).
 template <bool kRefill, bool kTable, uint32_t kLds>
 __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueState q, ProblemDev one)
 {
     const uint32_t n_active = q.scalars[kNActive];
     if (n_active == 0u) return;
     const uint4* lds_rec = nullptr;
-    if constexpr (kLds == kCompactGlobal) {
-        lds_rec = one.m.lds_rec;
-    } else if constexpr (kLds != 0u) {
+    if constexpr (kLds == kLdsCompact || kLds == kLdsIndexed) {
         __shared__ uint4 s_tri[kLdsStageWords];
         const uint32_t n = one.m.lds_words;
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s_tri[i] = one.m.lds_rec[i];
@@ -267,7 +266,8 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                     uint32_t j;
                     WalkOut w;
                     if constexpr (kLds != 0u) {
-                        const auto fetch = staged_fetch<kLds>(lds_rec, m.T);
+                        // staged records, or the item's problem's compact records in global memory
+                        const auto fetch = staged_fetch<kLds>(kLds == kCompactGlobal ? m.cmp_rec : lds_rec, m.T);
                         const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
                         w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
                                      r.fp, fetch);
@@ -499,7 +499,7 @@ ProblemDev queue_problem(const hase_context* ctx, uint32_t slot0, uint32_t base,
 }
 
 float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const QueueState& qs, uint32_t slots,
-                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single)
+                uint32_t rounds, GraphCache& gc, uint64_t generation, const ProblemDev* single, bool table_compact)
 {
     int cus = 0, per = 0;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -510,6 +510,7 @@ float queue_run(hipStream_t stream, int device, const hase_tuning_t& tu, const Q
     auto kernel = lds == kLdsIndexed ? hase_queue_trace_kernel<false, false, kLdsIndexed>
                 : lds == kLdsCompact ? hase_queue_trace_kernel<false, false, kLdsCompact>
                 : lds == kCompactGlobal ? hase_queue_trace_kernel<false, false, kCompactGlobal>
+                : table && table_compact && tu.stage_lds && !refill ? hase_queue_trace_kernel<false, true, kCompactGlobal>
                 : refill ? (table ? hase_queue_trace_kernel<true, true, 0u> : hase_queue_trace_kernel<true, false, 0u>)
                          : (table ? hase_queue_trace_kernel<false, true, 0u> : hase_queue_trace_kernel<false, false, 0u>);
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, (int)tu.block_size, 0));
