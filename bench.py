@@ -115,6 +115,19 @@ def host_description():
             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
+def cluster_shape(args, topology):
+    """(triangles, layers) per network evaluation of the learned proposal:
+    `--clusters AxB`, or with `auto` about 150 clusters per reflection slice
+    (prisms per cluster P / 150, as about 8 triangles per layer: the
+    production disk gets 64 x 8)."""
+    if args.clusters.lower() != "auto":
+        return tuple(int(v) for v in args.clusters.lower().split("x"))
+    layers = int(topology.levels) - 1
+    ppc = max(1, round(topology.numberOfTriangles * layers / 150))
+    cl = min(layers, max(1, round((ppc / 8) ** 0.5)))
+    return max(1, round(ppc / cl)), cl
+
+
 def cpu_baseline(w, spectra, seconds=15.0):
     """The oracle's C restatement of the estimator built with FP32 ray
     geometry and FP64 sums (OpenMP, this host's cores), timed over the
@@ -262,12 +275,13 @@ def batch_bench(args):
                                               rows_per_sample=1024)
                     Xs.append(X)
                     Ys.append(Y)
-        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+        ct, cl = cluster_shape(args, ws[order[-1]].topology)
         net = prop.fit(np.concatenate(Xs), np.concatenate(Ys), epochs=12, batch=4096, seed=1,
                        cluster_triangles=ct, cluster_levels=cl)
         train_s = time.perf_counter() - t0
-        for c in ctxs:
-            c.set_proposal(net)
+        # each problem evaluates the network on clusters of its own size
+        for c, w in zip(ctxs, ws):
+            c.set_proposal(net.with_clusters(*cluster_shape(args, w.topology)))
         log(f"network fitted on {sum(x.shape[0] for x in Xs)} rows of {len(picks)} problems in {train_s:.1f}s")
     tuning = ctxs[0].autotune(512)
     for c in ctxs[1:]:
@@ -413,7 +427,7 @@ def time_bench(args):
         ctx, spectra = build_context(w)
         best = ctx.autotune(512)
         if form == "learned":
-            ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+            ct, cl = cluster_shape(args, w.topology)
             if args.load_proposal:
                 net = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
             else:
@@ -497,7 +511,7 @@ def combined_bench(args):
         setup = time.perf_counter() - t0
         train_s = 0.0
         if form == "learned":
-            ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+            ct, cl = cluster_shape(args, w.topology)
             t0 = time.perf_counter()
             if args.load_proposal:
                 net = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
@@ -596,7 +610,7 @@ def traffic_pass(w, args):
     batch = int(args.sample_batches.split(",")[-1])
     ctx, _ = build_context(w, sample_batch=batch)
     if args.load_proposal:
-        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+        ct, cl = cluster_shape(args, w.topology)
         ctx.set_proposal(prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl))
         ctx.evaluate()
         learned = ctx.stats()
@@ -722,7 +736,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--levels", type=int, default=20, help="override for quick local runs")
     ap.add_argument("--fields", type=int, default=16, help="random pump fields the network trains on")
-    ap.add_argument("--clusters", default="16x4", help="triangles x layers per network evaluation")
+    ap.add_argument("--clusters", default="auto",
+                    help="triangles x layers per network evaluation (AxB), or auto: ~150 clusters per slice")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--learned-only", action="store_true", help="with --combined: time the learned form only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default queue launch shape")
@@ -797,7 +812,7 @@ def main():
     net, train_s, rows = None, 0.0, 0
     forms = ("plain",) if args.plain_only else ("plain", "beta_volume", "learned")
     if "learned" in forms:
-        ct, cl = (int(v) for v in args.clusters.lower().split("x"))
+        ct, cl = cluster_shape(args, w.topology)
         t0 = time.perf_counter()
         if args.load_proposal:
             net = prop.LearnedProposal.load(args.load_proposal).with_clusters(ct, cl)
@@ -913,7 +928,8 @@ def main():
             "refractive_indices": [float(v) for v in w.medium.value("refractiveIndices")],
             "reflectivity": "per triangle, top and bottom, U(0, 0.1)",
             "proposal": "plain importance map" if head == "plain" else
-                        f"learned (4x128 SiLU, BF16 MFMA, clusters {args.clusters}) with walked-map fallback",
+                        "learned (4x128 SiLU, BF16 MFMA, clusters of {}x{} triangles x layers) with walked-map "
+                        "fallback".format(*cluster_shape(args, w.topology)),
         },
         "wall_time_to_mse_s": hl["wall_time_to_mse_s"],
         "cold_first_call_s": hl["cold_first_call_s"],
