@@ -343,7 +343,9 @@ typedef struct hase_tuning {
     uint32_t refill_lanes;    /* idle lanes that trigger a refill, 1..64; 0: each lane walks its rays to the end */
     uint32_t step_budget;     /* prism steps between refill checks, >= 1 */
     uint32_t use_graph;       /* 1: replay the rounds of a batch as a captured hipGraph */
-    uint32_t stage_lds;       /* 1: stage the triangle records in LDS (meshes up to 4800 triangles, no refill) */
+    uint32_t stage_lds;       /* 1: walk compact triangle records, staged in LDS when they fit (32-byte records
+                                 up to 4800 triangles, 16-byte indexed ones up to ~8,100), else read from
+                                 global memory; 0: the 48-byte global records (refill always uses these) */
     double probe_ms;          /* hase_context_autotune: device time of the chosen setting's probe */
 } hase_tuning_t;
 
@@ -361,8 +363,8 @@ int hase_context_autotune(hase_context_t* context, uint32_t probe_samples, hase_
  * round (uneven budgets, different meshes, spectra and schedules), so small
  * problems no longer leave the GPU idle.  Each context keeps its proposal
  * (walked or learned) and its outputs; ray scheduling follows the first
- * context's tuning.  A context whose triangle records fit in LDS (at most
- * 4800 triangles), whose own tuning stages them there, and which has at
+ * context's tuning.  A context whose triangle records fit in LDS (up to
+ * ~8,100 triangles), whose own tuning stages them there, and which has at
  * least 8192 sample points is evaluated on its own queue first, as
  * hase_context_evaluate would (its samples fill the chip alone and its
  * trace kernel reads the records from LDS); the shared queue takes the
