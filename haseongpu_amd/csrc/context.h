@@ -23,6 +23,7 @@ struct hase_context {
     bool has_refl = false, has_nidx = false;
     float nidx[4] = {1, 1, 1, 1};
     std::vector<double> tri_area;      // [T] device order
+    double steps_per_cm = 0.0;         // 1.2 sqrt(T / mesh area): triangle crossings per cm, estimated
     std::vector<uint32_t> clad_flag;   // [T] device order
     std::vector<uint32_t> dev2user;    // device triangle -> caller triangle
     std::vector<uint32_t> user2dev;    // caller triangle -> device triangle

@@ -901,6 +901,11 @@ std::unique_ptr<hase_context> create_context(const hase_mesh_t* mesh, const hase
         const float y0 = (float)py[v[0]], y1 = (float)py[v[1]], y2 = (float)py[v[2]];
         ctx->centroid_host[td] = make_float2((x0 + x1 + x2) * (1.0f / 3.0f), (y0 + y1 + y2) * (1.0f / 3.0f));
     }
+    {
+        double area = 0.0;
+        for (uint32_t td = 0; td < T; ++td) area += ctx->tri_area[td];
+        ctx->steps_per_cm = 1.2 * std::sqrt((double)T / area);
+    }
     ctx->rec.upload(rec.data(), rec.size(), ctx->stream);
     ctx->user2dev_dev.upload(ctx->user2dev.data(), T, ctx->stream);
     const bool indexed_fits = T < (1u << 19) && np <= 0x10000u && (size_t)T + (np + 1u) / 2u <= hase::kLdsStageWords;
@@ -998,6 +1003,7 @@ MeshDev mesh_view(const hase_context* ctx)
     m.np = ctx->np;
     m.h = (float)ctx->h;
     m.clad_alpha = (float)ctx->clad_alpha;
+    m.steps_per_cm = (float)ctx->steps_per_cm;
     return m;
 }
 

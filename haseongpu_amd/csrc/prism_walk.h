@@ -66,6 +66,7 @@ struct MeshDev {
     int has_nidx;
     uint32_t T, layers, np;
     float h, clad_alpha;
+    float steps_per_cm;    // estimated triangle crossings per cm of xy path (1.2 sqrt(T / mesh area))
 };
 
 struct WalkOut {

@@ -135,13 +135,11 @@ struct RayStart {
     uint32_t t, l, nr, fp;
 };
 
-template <class Fetch>
-__device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, double& q, uint32_t& jout,
-                                             const Fetch& fetch)
+// Proposal entry of ray `ray` of an item: the first entry whose prefix
+// exceeds the ray's systematic target, searched between the target's buckets.
+__device__ __forceinline__ size_t ray_entry(const ItemCtx& c, uint32_t ray)
 {
-    const uint32_t T = m.T, P = T * m.layers;
     const double target = ((double)ray + c.U) * c.scale;
-    // first entry whose prefix exceeds the target, between the target's buckets
     const double kt = target * (double)kPrefixBuckets;
     const uint32_t kb = kt >= (double)(kPrefixBuckets - 1u) ? kPrefixBuckets - 1u : (uint32_t)kt;
     size_t lo = c.bk[kb], hi = (kb + 1u < kPrefixBuckets ? (size_t)c.bk[kb + 1u] : c.n - 1) + 1;
@@ -149,7 +147,30 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
         const size_t mid = (lo + hi) >> 1;
         if (c.pre[mid] > target) hi = mid; else lo = mid + 1;
     }
-    const size_t e = lo < c.n ? lo : c.n - 1;
+    return lo < c.n ? lo : c.n - 1;
+}
+
+// Estimated prism steps of a ray from entry e to the item's sample: triangle
+// crossings over the xy distance from the start triangle's first corner,
+// plus the level planes between the start layer's middle and the end image.
+// Only used to order the rays of a wavefront (longer walks together).
+template <class Fetch>
+__device__ __forceinline__ float ray_steps_estimate(const MeshDev& m, const ItemCtx& c, size_t e, const Fetch& fetch)
+{
+    const uint32_t T = m.T, P = T * m.layers;
+    const uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
+    const uint32_t t = j % T, l = j / T;
+    const float2 p0 = fetch.vertex(m, t, 0u);
+    const float ez = (float)image_plane(c.sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
+    const float dx = p0.x - c.sp.px, dy = p0.y - c.sp.py;
+    return sqrtf(dx * dx + dy * dy) * m.steps_per_cm + fabsf((float)l + 0.5f - ez);
+}
+
+template <class Fetch>
+__device__ __forceinline__ RayStart draw_ray_at(const MeshDev& m, const ItemCtx& c, uint32_t ray, size_t e, double& q,
+                                                uint32_t& jout, const Fetch& fetch)
+{
+    const uint32_t T = m.T, P = T * m.layers;
     q = c.pre[e] - (e ? c.pre[e - 1] : 0.0);
     const uint32_t k = (uint32_t)(e / P), j = (uint32_t)(e % P);
     const uint32_t t = j % T, l = j / T;
@@ -171,6 +192,48 @@ __device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c,
     r.fp = k & 1u;
     r.ez = (float)image_plane(c.sp.plane, (int)m.layers, r.nr, r.fp);
     return r;
+}
+
+template <class Fetch>
+__device__ __forceinline__ RayStart draw_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, double& q, uint32_t& jout,
+                                             const Fetch& fetch)
+{
+    return draw_ray_at(m, c, ray, ray_entry(c, ray), q, jout, fetch);
+}
+
+// Bitonic sort of a wavefront's 256 keys, ascending, key r * 64 + lane in
+// v[r] of that lane.  Every lane of the wavefront takes part (the exchanges
+// read the partner lane's register).
+__device__ __forceinline__ void wave_sort256(uint32_t (&v)[4], int lane)
+{
+#pragma unroll
+    for (uint32_t k = 2u; k <= 256u; k <<= 1) {
+#pragma unroll
+        for (uint32_t d = k >> 1; d > 0u; d >>= 1) {
+            if (d >= 64u) {
+                // partners in the same lane: registers r and r + d / 64
+                const uint32_t rd = d >> 6;
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; ++r) {
+                    if (r & rd) continue;
+                    const bool asc = ((r * 64u + (uint32_t)lane) & k) == 0u;
+                    const uint32_t a = v[r], b = v[r | rd];
+                    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                    v[r] = asc ? lo : hi;
+                    v[r | rd] = asc ? hi : lo;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t r = 0; r < 4u; ++r) {
+                    const uint32_t w = (uint32_t)__shfl_xor((int)v[r], (int)d, 64);
+                    const bool lower = ((uint32_t)lane & d) == 0u;
+                    const bool asc = ((r * 64u + (uint32_t)lane) & k) == 0u;
+                    const uint32_t lo = v[r] < w ? v[r] : w, hi = v[r] < w ? w : v[r];
+                    v[r] = lower == asc ? lo : hi;
+                }
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ void start_ray(const MeshDev& m, const ItemCtx& c, uint32_t ray, LaneRay& lr)
@@ -261,23 +324,65 @@ __global__ void __launch_bounds__(kQueueMaxBlock) hase_queue_trace_kernel(QueueS
                 drops += ok ? 0u : 1u;
             };
             if (!kRefill) {
-                for (uint32_t i = (uint32_t)lane; i < cnt; i += 64u) {
-                    double qprob;
-                    uint32_t j;
-                    WalkOut w;
+                // Rays in groups of 256: each lane finds the entries of four,
+                // the wavefront sorts the group by estimated walk length and
+                // walks it as four rounds of 64 rays of similar length (a
+                // round lasts as long as its longest walk).
+                auto run_group = [&](uint32_t g0, const auto& fetch) {
+                    uint32_t key[4];
+                    uint32_t ent[4];
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; ++r) {
+                        const uint32_t li = r * 64u + (uint32_t)lane;
+                        key[r] = 0xFFFFFF00u | li;   // past the item's end: sorted last, skipped
+                        ent[r] = 0u;
+                        if (g0 + li < cnt) {
+                            const size_t e = ray_entry(c, c.r0 + g0 + li);
+                            ent[r] = (uint32_t)e;
+                            const float est = fminf(ray_steps_estimate(m, c, e, fetch) * 16.0f, 16777214.0f);
+                            key[r] = ((uint32_t)est << 8) | li;
+                        }
+                    }
+                    wave_sort256(key, lane);
+                    // round r: the ray of sorted key r of this lane, whose
+                    // entry lane li % 64 found in its register li / 64
+                    uint32_t mine[4];
+                    uint32_t rays4 = 0u;   // the four group indices, 8 bits each
+#pragma unroll
+                    for (uint32_t r = 0; r < 4u; ++r) {
+                        const int src = (int)(key[r] & 63u);
+                        const uint32_t e0 = (uint32_t)__shfl((int)ent[0], src, 64);
+                        const uint32_t e1 = (uint32_t)__shfl((int)ent[1], src, 64);
+                        const uint32_t e2 = (uint32_t)__shfl((int)ent[2], src, 64);
+                        const uint32_t e3 = (uint32_t)__shfl((int)ent[3], src, 64);
+                        const uint32_t sel = (key[r] >> 6) & 3u;
+                        mine[r] = sel == 0u ? e0 : (sel == 1u ? e1 : (sel == 2u ? e2 : e3));
+                        rays4 |= (key[r] & 0xFFu) << (8u * r);
+                    }
+                    // one copy of the walk: each round takes slot 0, then the
+                    // slots move down (static register indices, no scratch)
+#pragma unroll 1
+                    for (uint32_t r = 0; r < 4u; ++r) {
+                        const uint32_t li = rays4 & 0xFFu;
+                        if (g0 + li < cnt) {
+                            double qprob;
+                            uint32_t j;
+                            const RayStart s = draw_ray_at(m, c, c.r0 + g0 + li, (size_t)mine[0], qprob, j, fetch);
+                            const WalkOut w = walk_ray(m, s.ax, s.ay, s.az, s.t, s.l, s.p0, s.p1, s.p2, c.sp.px,
+                                                       c.sp.py, s.ez, s.nr, s.fp, fetch);
+                            finish(w, qprob, j);
+                        }
+                        rays4 >>= 8;
+                        mine[0] = mine[1]; mine[1] = mine[2]; mine[2] = mine[3];
+                    }
+                };
+                for (uint32_t g0 = 0; g0 < cnt; g0 += 256u) {
                     if constexpr (kLds != 0u) {
                         // staged records, or the item's problem's compact records in global memory
-                        const auto fetch = staged_fetch<kLds>(kLds == kCompactGlobal ? m.cmp_rec : lds_rec, m.T);
-                        const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
-                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
-                                     r.fp, fetch);
+                        run_group(g0, staged_fetch<kLds>(kLds == kCompactGlobal ? m.cmp_rec : lds_rec, m.T));
                     } else {
-                        const GlobalTriangles fetch;
-                        const RayStart r = draw_ray(m, c, c.r0 + i, qprob, j, fetch);
-                        w = walk_ray(m, r.ax, r.ay, r.az, r.t, r.l, r.p0, r.p1, r.p2, c.sp.px, c.sp.py, r.ez, r.nr,
-                                     r.fp, fetch);
+                        run_group(g0, GlobalTriangles{});
                     }
-                    finish(w, qprob, j);
                 }
             } else {
                 LaneRay lr;
