@@ -268,10 +268,11 @@ __global__ void __launch_bounds__(kBlock) hase_importance_kernel(ImportanceParam
 
 // Walked importance of whole batches for meshes of up to kLdsTriangles
 // triangles: one persistent 1024-thread work-group per CU copies the compact
-// triangle records into LDS once, then its wavefronts take 64 consecutive
+// triangle records into LDS once, then its wavefronts take 256 consecutive
 // column items of the batch at a time, round-robin over the grid (a batch's
-// items are numbered slot-major, so the lanes of a wavefront walk
-// neighbouring triangle columns of one sample and slice).  Only the
+// items are numbered slot-major, so a wavefront's items are neighbouring
+// triangle columns of one sample and slice), and walk them in four rounds
+// of 64 sorted by estimated work.  Only the
 // importance values are written; hase_prefix_kernel forms each sample's
 // mixture prefix afterwards (not for transfer maps).
 constexpr int kStagedBlock = 1024;
@@ -292,29 +293,73 @@ __global__ void __launch_bounds__(kStagedBlock) hase_importance_staged_kernel(Im
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves_per_block = blockDim.x / 64u;
     const uint64_t wave = (uint64_t)blockIdx.x * waves_per_block + threadIdx.x / 64u;
-    const uint64_t stride = (uint64_t)gridDim.x * waves_per_block * 64u;
-    unsigned long long walks = 0, steps = 0;
-    for (uint64_t item = wave * 64u + lane; item < total; item += stride) {
-        const uint32_t slot = (uint32_t)(item / per_slot), it = (uint32_t)(item - (uint64_t)slot * per_slot);
-        const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
-        const SampleGeom sp = sample_point(p.pts, m.np, sample);
-        uint32_t t0, k;
-        int lv0, nlev;
-        ci.decode(it, t0, k, lv0, nlev);
-        float v[kLevelGroup];
-        steps += column_entries(m, fetch, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px, sp.py,
-                                sp.plane, p.transfer != 0, v);
-        walks += (unsigned long long)nlev;
-        float* imp = p.imp + (size_t)slot * n + (size_t)k * P + t0;
+    const uint64_t stride = (uint64_t)gridDim.x * waves_per_block * 256u;
+    unsigned long long steps = 0;
+    // Items 256 at a time per wavefront, sorted by estimated work (the
+    // column's triangle crossings, from its centroid's xy distance to the
+    // sample at 1.2 sqrt(T / mesh area) per cm, times its levels, plus their
+    // plane crossings) and taken as four rounds of 64: lanes of a round walk
+    // columns of similar length, whose plane crossings fall on similar steps.
+    for (uint64_t base = wave * 256u; base < total; base += stride) {
+        // slot and item of the group's first item (wavefront-uniform); item
+        // base + li is li further on, in the same slot or the next ones
+        const uint64_t slot0 = base / per_slot;
+        const uint32_t it0 = (uint32_t)(base - slot0 * per_slot);
+        auto locate = [&](uint32_t li, uint32_t& slot, uint32_t& it) {
+            slot = (uint32_t)slot0;
+            it = it0 + li;
+            while (it >= per_slot) {
+                it -= per_slot;
+                ++slot;
+            }
+            return slot < p.slots;
+        };
+        uint32_t key[4];
 #pragma unroll
-        for (int g = 0; g < kLevelGroup; ++g)
-            if (g < nlev) imp[(size_t)(lv0 + g) * T] = v[g];
+        for (uint32_t r = 0; r < 4u; ++r) {
+            const uint32_t li = r * 64u + lane;
+            key[r] = 0xFFFFFF00u | li;   // past the end: sorted last, skipped
+            uint32_t slot, it;
+            if (locate(li, slot, it)) {
+                const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
+                const SampleGeom sp = sample_point(p.pts, m.np, sample);
+                uint32_t t0, k;
+                int lv0, nlev;
+                ci.decode(it, t0, k, lv0, nlev);
+                const float2 a = fetch.vertex(m, t0, 0u), b = fetch.vertex(m, t0, 1u), c = fetch.vertex(m, t0, 2u);
+                const float dx = (a.x + b.x + c.x) * (1.0f / 3.0f) - sp.px, dy = (a.y + b.y + c.y) * (1.0f / 3.0f) - sp.py;
+                const float ze = (float)image_plane(sp.plane, (int)m.layers, (k + 1u) >> 1, k & 1u);
+                const float est = (float)nlev * (sqrtf(dx * dx + dy * dy) * m.steps_per_cm +
+                                                 fabsf((float)lv0 + 0.5f * (float)nlev - ze));
+                key[r] = ((uint32_t)fminf(est * 4.0f, 16777214.0f) << 8) | li;
+            }
+        }
+        wave_sort256(key, (int)lane);
+        uint32_t items4 = 0u;   // the four group indices, 8 bits each
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; ++r) items4 |= (key[r] & 0xFFu) << (8u * r);
+#pragma unroll 1
+        for (uint32_t r = 0; r < 4u; ++r, items4 >>= 8) {
+            uint32_t slot, it;
+            if (!locate(items4 & 0xFFu, slot, it)) continue;
+            const uint32_t sample = p.sample_of ? p.sample_of[slot] : p.sample_base + slot;
+            const SampleGeom sp = sample_point(p.pts, m.np, sample);
+            uint32_t t0, k;
+            int lv0, nlev;
+            ci.decode(it, t0, k, lv0, nlev);
+            float v[kLevelGroup];
+            steps += column_entries(m, fetch, p.c1, p.c2, p.n_lambda, p.inv_lambda, t0, lv0, nlev, k, sp.px, sp.py,
+                                    sp.plane, p.transfer != 0, v);
+            float* imp = p.imp + (size_t)slot * n + (size_t)k * P + t0;
+#pragma unroll
+            for (int g = 0; g < kLevelGroup; ++g)
+                if (g < nlev) imp[(size_t)(lv0 + g) * T] = v[g];
+        }
     }
-    const unsigned long long ws = wave_sum_u64(walks), ss = wave_sum_u64(steps);
-    if (lane == 0 && ws) {
-        atomicAdd(&p.counters[2], ws);
-        atomicAdd(&p.counters[3], ss);
-    }
+    // every entry of every slot is one level walk
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&p.counters[2], (unsigned long long)p.slots * n);
+    const unsigned long long ss = wave_sum_u64(steps);
+    if (lane == 0 && ss) atomicAdd(&p.counters[3], ss);
 }
 
 // beta * V proposal shared by every sample (one work-group): the mixture with
