@@ -890,6 +890,9 @@ def main():
                      "lane_utilisation": stored["lane_utilisation"],
                      "issue_per_cu_cycle": wave_insts / (dom["avg_ms"] * 1e-3 * CUS * CLOCK_HZ),
                      "issue_peak_per_cu_cycle": 1.0,
+                     "issue_peak_note": "nominal: one wave64 VALU instruction per CU per cycle at 2.4 GHz (four "
+                                        "SIMD16 units, four cycles each); measured rates slightly above it mean "
+                                        "it is not an exact ceiling on this chip",
                      "counter_run_issue_per_cu_cycle": stored["valu_issue_per_cu_cycle"]}
     roofline = {
         "bound": "valu", "kernel": DOMINANT, "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
